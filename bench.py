@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Throughput of the dual-quantization + Huffman path on one MI355X.
+
+Workload (default): 3D float32 field 512 x 512 x 512 (512 MiB), Gaussian
+random field with E(k) ~ k^-5/3 synthesised on the GPU from a seed, scaled to
+[-50, 50], compressed at r2r eb = 1e-4 with 8^3 Lorenzo blocks and a 4096-code
+dictionary.  A step is one compress of the field (input resident in HBM,
+archive written to HBM); decompression of the resulting archive is timed the
+same way and reported beside it.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GB/s compress/decompress (1 MI355X, eb=1e-4) vs CPU; %HBM peak; p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_avg(ctx, fn, reps):
+    acc = {}
+    for _ in range(reps):
+        fn()
+        for name, ms in ctx.timings():
+            acc.setdefault(name, []).append(ms)
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def cpu_baseline(x_host, eb, range_, dims, dict_size, block):
+    """The reference's own Compress/Decompress (oracle/_ref) when it was built,
+    else the C restatement, timed on this host's cores."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    eb_abs = eb * range_  # same eb_final as r2r; avoids the reference's file re-read
+    if po.ref_available():
+        po.ref().ref_set_threads(threads)
+        t0 = time.perf_counter()
+        a, _ = po.ref_compress(x_host, eb_abs, False, dict_size, block, threads=threads)
+        tc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        po.ref_decompress(a, None, False, dict_size, block)
+        td = time.perf_counter() - t0
+        kind, used = "reference", threads
+    else:
+        t0 = time.perf_counter()
+        a, _, _ = po.compress(x_host, eb_abs, False, dict_size, block)
+        tc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        po.decompress(a, x_host.shape, False, 0.0, dict_size, block)
+        td = time.perf_counter() - t0
+        kind, used = "port", 1
+    n = x_host.size
+    return {
+        "value": round(4 * n / tc / 1e9, 4),
+        "unit": "GB/s",
+        "cores": used,
+        "kind": kind,
+        "sample": f"one compress of the full {'x'.join(map(str, dims))} field (abs eb = eb*range, same eb_final, "
+                  f"no file IO) on {used} OpenMP threads; decompress {4 * n / td / 1e9:.4f} GB/s",
+        "compress_s": round(tc, 3),
+        "decompress_s": round(td, 3),
+        "decompress_gbps": round(4 * n / td / 1e9, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--shape", default="512,512,512", help="C order (z,y,x)")
+    ap.add_argument("--eb", type=float, default=1e-4)
+    ap.add_argument("--block", type=int, default=8)
+    ap.add_argument("--dict", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one field shards by slabs; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from vecsz_amd import Context, datagen, make_params
+
+    shape = tuple(int(s) for s in args.shape.split(","))
+    dims = tuple(reversed(shape))
+    torch.cuda.set_device(0)
+    log(f"generating GRF {shape} on the GPU")
+    x = datagen.grf(shape, seed=0, device="cuda")
+    torch.cuda.synchronize()
+    n = x.numel()
+    value_range = float((x.max() - x.min()).item())
+
+    ctx = Context(0)
+    p = make_params(dims, args.eb, "r2r", args.block, args.dict)
+    cap = ctx.compress_bound(p)
+    arch = torch.empty(cap, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    pd = make_params((1,), 1e-4, "r2r", args.block, args.dict, value_range)
+
+    def comp():
+        return ctx.compress_into(p, x.data_ptr(), arch.data_ptr(), cap)
+
+    size, nout = comp()
+
+    def decomp():
+        ctx.decompress_into(pd, arch.data_ptr(), size, y.data_ptr(), n)
+
+    # warmup
+    for _ in range(args.warmup):
+        comp()
+        decomp()
+    torch.cuda.synchronize()
+
+    # timed compress steps
+    ctx.enable_timing(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        size, nout = comp()
+    torch.cuda.synchronize()
+    tc = (time.perf_counter() - t0) / args.steps
+    lat = []
+    for _ in range(min(args.steps, 20)):
+        s = time.perf_counter()
+        comp()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - s)
+    lat.sort()
+
+    # timed decompress steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        decomp()
+    torch.cuda.synchronize()
+    td = (time.perf_counter() - t0) / args.steps
+
+    # correctness: the round trip must respect the error bound up to the
+    # reference's fp32 slack (bit-exact parity is tested in tests/).
+    err = (x.double() - y.double()).abs()
+    slack = x.double().abs() * 2.0 ** -22
+    eb_final = args.eb * value_range
+    bound_ok = bool((err <= eb_final + slack).all().item())
+
+    # per-stage device time (HIP events on the library's stream)
+    ctx.enable_timing(True)
+    cst = stage_avg(ctx, comp, 5)
+    dst = stage_avg(ctx, decomp, 5)
+    ctx.enable_timing(False)
+    log("compress stages (ms):", {k: round(v, 4) for k, v in cst.items()})
+    log("decompress stages (ms):", {k: round(v, 4) for k, v in dst.items()})
+
+    # algorithmic bytes per launch of each compress kernel
+    alg = {"range": 4 * n, "dualquant": 4 * n, "pack": size, "chunk_bits": 0, "scan": 0}
+    dom = max(cst, key=lambda k: cst[k])
+    achieved = alg.get(dom, 0) / (cst[dom] * 1e-3) / 1e9
+    roofline = {
+        "kernel": f"k_{dom}" if dom != "dualquant" else "k_dualquant_3d<8>",
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "alg_bytes": alg.get(dom, 0),
+        "avg_ms": round(cst[dom], 4),
+    }
+
+    comp_gbps = 4 * n / tc / 1e9
+    dec_gbps = 4 * n / td / 1e9
+    e2e_c = (4 * n + size) / tc / 1e9
+    e2e_d = (4 * n + size) / td / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(comp_gbps, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tc * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (GRF E(k)~k^-5/3, seed 0, generated on the GPU)",
+        "config": {"workload": f"3D f32 {'x'.join(map(str, dims))} GRF r2r eb={args.eb} compress",
+                   "dims_xyz": list(dims), "eb": args.eb, "mode": "r2r", "block": args.block,
+                   "dict_size": args.dict, "parallelism": "single-gpu"},
+        "decompress_gbps": round(dec_gbps, 2),
+        "decompress_ms": round(td * 1e3, 4),
+        "e2e_hbm_frac_compress": round(e2e_c / HBM_PEAK_GBS, 4),
+        "e2e_hbm_frac_decompress": round(e2e_d / HBM_PEAK_GBS, 4),
+        "archive_bytes": size,
+        "compression_ratio": round(4 * n / size, 4),
+        "n_outlier": nout,
+        "error_bound_ok": bound_ok,
+        "latency_ms": {"p50": round(lat[len(lat) // 2] * 1e3, 4), "p99": round(lat[-1] * 1e3, 4)},
+        "stages_ms": {"compress": {k: round(v, 4) for k, v in cst.items()},
+                      "decompress": {k: round(v, 4) for k, v in dst.items()}},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        x_host = x.cpu().numpy()
+        out["cpu_baseline"] = cpu_baseline(x_host, args.eb, value_range, dims, args.dict, args.block)
+        out["speedup_vs_cpu"] = round(comp_gbps / out["cpu_baseline"]["value"], 2)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

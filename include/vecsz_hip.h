@@ -1,0 +1,114 @@
+/*
+ * vecsz_hip.h -- C ABI of the MI355X (gfx950) implementation of vecSZ's
+ * dual-quantization + Huffman compression path.
+ *
+ * Every entry point replaces a piece of the reference's header-only
+ * interface; the reference function each one stands in for is cited beside
+ * it.  Archives are byte-identical to the reference's datapack::pack payload
+ * (src/utils/datapack.hh:17-29) and decompressed floats are bit-identical to
+ * the reference's Decompress output.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; no C++ or torch types.
+ *  - Device pointers (d_*) live on the context's GPU; host pointers (h_*)
+ *    are ordinary host memory.
+ *  - All calls are stream-ordered on the context's stream and return when
+ *    their results (sizes, status) are known on the host.
+ *  - Functions return a VECSZ_* status and never exit the process (the
+ *    reference calls exit(1) on IO errors and a zero r2r range,
+ *    src/types.cc:51-54, src/utils/io.hh:26-30).
+ *  - A context is not thread-safe; use one context per host thread.
+ */
+#ifndef VECSZ_HIP_H
+#define VECSZ_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VECSZ_OK 0
+#define VECSZ_EINVAL 1     /* bad argument / unsupported parameter */
+#define VECSZ_ENOMEM 2     /* device or host allocation failed, or out_cap too small */
+#define VECSZ_ECORRUPT 3   /* malformed or truncated archive */
+#define VECSZ_ECODELEN 4   /* a Huffman code longer than 64 bits (reference's >64-bit branch) */
+#define VECSZ_EHIP 5       /* HIP runtime error */
+#define VECSZ_ERANGE0 6    /* r2r with zero value range (reference exits, src/types.cc:51-54) */
+#define VECSZ_ENODEV 7     /* no usable GPU / library built without device code */
+
+#define VECSZ_MODE_ABS 0
+#define VECSZ_MODE_R2R 1
+
+/* Parameters that the reference carries in ArgParse
+ * (src/argument_parser/argparse.hh:45-98) and config_t (src/types.hh:126-140).
+ * Decompress reads ndim, dims and eb from the archive and takes mode,
+ * value_range, block and dict_size from here, because the archive does not
+ * record them (the reference takes them from its command line). */
+typedef struct vecsz_params {
+    int ndim;              /* 1, 2 or 3 */
+    uint64_t dims[3];      /* x (fastest) first; unused dims = 1 */
+    int mode;              /* VECSZ_MODE_ABS or VECSZ_MODE_R2R */
+    double eb;             /* user error bound, stored in the archive */
+    double value_range;    /* r2r: value range to use; <= 0 on compress = compute max-min of the input */
+    int block;             /* Lorenzo block size B: 8, 16, 32 or 64 (reference default 8) */
+    int dict_size;         /* quant-code dictionary (reference default 4096); radius = dict_size / 2 */
+} vecsz_params;
+
+typedef struct vecsz_ctx vecsz_ctx;
+
+/* Create a context bound to `device`; `stream` is a hipStream_t or NULL for a
+ * context-owned stream (default priority). */
+int vecsz_ctx_create(vecsz_ctx** ctx, int device, void* stream);
+int vecsz_ctx_destroy(vecsz_ctx* ctx);
+const char* vecsz_strerror(int status);
+/* Human-readable detail of the last failure on this context. */
+const char* vecsz_ctx_last_error(vecsz_ctx* ctx);
+
+/* Default parameters: abs, eb 1e-4, B 8, dict 4096
+ * (src/argument_parser/argparse.hh:62-77). */
+void vecsz_params_default(vecsz_params* p);
+
+/* Upper bound of the archive payload for `p` (no allocation). */
+int vecsz_compress_bound(const vecsz_params* p, uint64_t* bytes);
+
+/* Replaces vecsz::interface::Compress<float,int> (src/vecsz_interface.hh:31-229),
+ * lossless pass excluded: field d_in (device) -> archive payload d_out
+ * (device, out_cap bytes).  *out_bytes receives the payload size
+ * (Compress's *data_size) and *n_outlier the outlier count.  The input is
+ * not modified (the reference frees it and mutates it in 1D). */
+int vecsz_compress(vecsz_ctx* ctx, const vecsz_params* p, const float* d_in, uint8_t* d_out,
+                   uint64_t out_cap, uint64_t* out_bytes, uint64_t* n_outlier);
+
+/* Replaces vecsz::interface::Decompress<float,int> (src/vecsz_interface.hh:232-375),
+ * lossless pass excluded: archive payload d_in (device) -> field d_out
+ * (device, out_cap floats). */
+int vecsz_decompress(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
+                     float* d_out, uint64_t out_cap);
+
+/* Host-memory conveniences: stage through device buffers owned by the
+ * context.  Same semantics as the device entry points. */
+int vecsz_compress_host(vecsz_ctx* ctx, const vecsz_params* p, const float* h_in, uint8_t* h_out,
+                        uint64_t out_cap, uint64_t* out_bytes, uint64_t* n_outlier);
+int vecsz_decompress_host(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* h_in, uint64_t in_bytes,
+                          float* h_out, uint64_t out_cap);
+
+/* Parse an archive payload header on the host (datapack::unpack,
+ * src/utils/datapack.hh:33-56). */
+int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint64_t dims[3], double* eb,
+                       uint64_t* huff_size, uint64_t* n_outlier);
+
+/* Per-stage device timing of the last compress/decompress call, measured
+ * with HIP events on the context's stream when enabled (off by default).
+ * names/ms receive up to `cap` entries; returns the entry count. */
+int vecsz_ctx_enable_timing(vecsz_ctx* ctx, int on);
+int vecsz_ctx_timings(vecsz_ctx* ctx, const char** names, double* ms, int cap);
+
+/* Library build identification (gfx target and version string). */
+const char* vecsz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VECSZ_HIP_H */

@@ -1,0 +1,79 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point the
+header declares, and its host-only entry points behave."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from vecsz_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "vecsz_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(vecsz_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_version_and_strerror():
+    lib = _lib.load()
+    assert b"gfx950" in lib.vecsz_version()
+    for s in range(8):
+        assert lib.vecsz_strerror(s)
+
+
+def test_params_default_match_reference_defaults():
+    p = _lib.Params()
+    _lib.load().vecsz_params_default(ctypes.byref(p))
+    assert (p.eb, p.block, p.dict_size, p.mode) == (1e-4, 8, 4096, 0)
+
+
+def test_compress_bound():
+    from vecsz_amd.interface import make_params
+
+    p = make_params((64, 64, 64), 1e-4)
+    b = ctypes.c_uint64()
+    assert _lib.load().vecsz_compress_bound(ctypes.byref(p), ctypes.byref(b)) == 0
+    assert b.value >= 64 ** 3 * 12
+
+
+def test_archive_info_on_golden(golden):
+    from vecsz_amd import archive_info
+
+    for c in golden["cases"]:
+        info = archive_info(c["archive"])
+        assert info["dims"] == tuple(reversed(c["x"].shape))
+        assert info["eb"] == c["eb"]
+        assert info["n_outlier"] == c["n_outlier"]
+
+
+def test_archive_info_rejects_truncated(golden):
+    from vecsz_amd import VecszError, archive_info
+
+    a = golden["cases"][0]["archive"]
+    for cut in (0, 3, 10, 30, len(a) - 1):
+        with pytest.raises(VecszError):
+            archive_info(a[:cut])
+
+
+def test_no_gpu_context_fails_loudly():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from vecsz_amd import Context, VecszError
+
+    with pytest.raises(VecszError):
+        Context(0)

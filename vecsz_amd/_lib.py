@@ -1,0 +1,81 @@
+"""ctypes binding of libvecsz_hip.so (the C ABI declared in include/vecsz_hip.h)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvecsz_hip.so")
+
+VECSZ_OK = 0
+VECSZ_EINVAL = 1
+VECSZ_ENOMEM = 2
+VECSZ_ECORRUPT = 3
+VECSZ_ECODELEN = 4
+VECSZ_EHIP = 5
+VECSZ_ERANGE0 = 6
+VECSZ_ENODEV = 7
+
+MODE_ABS = 0
+MODE_R2R = 1
+
+# Every symbol include/vecsz_hip.h declares.
+EXPORTS = (
+    "vecsz_ctx_create", "vecsz_ctx_destroy", "vecsz_strerror", "vecsz_ctx_last_error", "vecsz_params_default",
+    "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
+    "vecsz_archive_info", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
+)
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("ndim", ctypes.c_int),
+        ("dims", ctypes.c_uint64 * 3),
+        ("mode", ctypes.c_int),
+        ("eb", ctypes.c_double),
+        ("value_range", ctypes.c_double),
+        ("block", ctypes.c_int),
+        ("dict_size", ctypes.c_int),
+    ]
+
+
+class VecszError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"vecsz status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def load():
+    """Loads the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m vecsz_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u64, u64p = ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)
+    P = ctypes.POINTER(Params)
+    lib.vecsz_ctx_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, vp]
+    lib.vecsz_ctx_destroy.argtypes = [vp]
+    lib.vecsz_strerror.argtypes = [ctypes.c_int]
+    lib.vecsz_strerror.restype = ctypes.c_char_p
+    lib.vecsz_ctx_last_error.argtypes = [vp]
+    lib.vecsz_ctx_last_error.restype = ctypes.c_char_p
+    lib.vecsz_params_default.argtypes = [P]
+    lib.vecsz_params_default.restype = None
+    lib.vecsz_compress_bound.argtypes = [P, u64p]
+    lib.vecsz_compress.argtypes = [vp, P, vp, vp, u64, u64p, u64p]
+    lib.vecsz_decompress.argtypes = [vp, P, vp, u64, vp, u64]
+    lib.vecsz_compress_host.argtypes = [vp, P, vp, vp, u64, u64p, u64p]
+    lib.vecsz_decompress_host.argtypes = [vp, P, vp, u64, vp, u64]
+    lib.vecsz_archive_info.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int), u64p, ctypes.POINTER(ctypes.c_double),
+                                       u64p, u64p]
+    lib.vecsz_ctx_enable_timing.argtypes = [vp, ctypes.c_int]
+    lib.vecsz_ctx_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.c_int]
+    lib.vecsz_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib

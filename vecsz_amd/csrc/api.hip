@@ -1,0 +1,816 @@
+// C ABI and host orchestration of the compress / decompress pipelines.
+//
+// Compress (vecsz::interface::Compress, src/vecsz_interface.hh:31-229):
+//   [r2r] k_range -> eb family on the host (src/types.cc:84-93)
+//   k_dualquant -> histogram to host -> codebook + tree bytes (codebook.cpp)
+//   header + tree H2D -> k_chunk_bits -> k_scan_chunks -> k_pack
+// Decompress (vecsz::interface::Decompress, src/vecsz_interface.hh:232-375):
+//   header + tree D2H -> decode tables -> k_decode_sync -> k_decode_check
+//   [-> k_decode_redo rounds] -> scan -> k_decode_write -> k_recon_* [-> fallback]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vecsz_hip.h"
+#include "codebook.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace vsz;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    bool ensure(size_t bytes) {
+        if (bytes <= cap && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 256);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+        cap = want;
+        return true;
+    }
+    template <typename T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct HostBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    bool ensure(size_t bytes) {
+        if (bytes <= cap && p) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = std::max<size_t>(bytes, 4096);
+        if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+        cap = want;
+        return true;
+    }
+    template <typename T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+struct Stage {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct vecsz_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string last_error;
+    // compress scratch
+    DevBuf codes, sparse, hist, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, cw_hi, cw_lo, cw_len;
+    // decompress scratch
+    DevBuf lut, child, sub_start, sub_end, sub_end2, sub_cnt, sub_zeros, sub_bnd, sub_zmask, sub_nbnd, sub_dirty, sym_off,
+        zero_off, scan_tmp, scan_tot, dec_sum, flagged, nflagged;
+    // host staging
+    DevBuf stage_in, stage_out;
+    HostBuf pin;
+    std::vector<uint64_t> hist_host;
+    // timing
+    bool timing = false;
+    std::vector<Stage> stages;
+    std::vector<hipEvent_t> event_pool;
+    size_t event_next = 0;
+    std::vector<std::string> tnames;
+    std::vector<double> tms;
+};
+
+namespace {
+
+int fail(vecsz_ctx* c, int code, const std::string& msg) {
+    if (c) c->last_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess)                                                                   \
+            return fail(ctx, VECSZ_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));    \
+    } while (0)
+
+hipEvent_t take_event(vecsz_ctx* c) {
+    if (c->event_next == c->event_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->event_pool.push_back(e);
+    }
+    return c->event_pool[c->event_next++];
+}
+
+struct StageScope {
+    vecsz_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr;
+    StageScope(vecsz_ctx* ctx, const char* n) : c(ctx), name(n) {
+        if (c->timing) {
+            a = take_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~StageScope() {
+        if (c->timing && a) {
+            hipEvent_t b = take_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->stages.push_back({name, a, b});
+            }
+        }
+    }
+};
+
+void timing_begin(vecsz_ctx* c) {
+    c->stages.clear();
+    c->event_next = 0;
+}
+
+void timing_end(vecsz_ctx* c) {
+    if (!c->timing) return;
+    c->tnames.clear();
+    c->tms.clear();
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& s : c->stages) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, s.a, s.b);
+        c->tnames.push_back(s.name);
+        c->tms.push_back(ms);
+    }
+}
+
+// |e| < thr  <=>  sqrtf(e*e) < R for every integral float e: the AVX
+// quantizability test of src/dualquant.hh:170-171.
+int vec_threshold(int R) {
+    const float Rf = (float)R;
+    for (int k = std::max(0, R - 64);; k++) {
+        volatile float e = (float)k;
+        float sq = e * e;
+        if (!(std::sqrt(sq) < Rf)) return k;
+    }
+}
+
+bool valid_block(int B) { return B == 8 || B == 16 || B == 32 || B == 64; }
+
+int check_params(vecsz_ctx* c, const vecsz_params* p, bool need_dims) {
+    if (!p) return fail(c, VECSZ_EINVAL, "params is NULL");
+    if (!valid_block(p->block)) return fail(c, VECSZ_EINVAL, "block must be 8, 16, 32 or 64");
+    if (p->dict_size < 2 || p->dict_size > 65536 || (p->dict_size & 1))
+        return fail(c, VECSZ_EINVAL, "dict_size must be even and in [2, 65536]");
+    if (p->mode != VECSZ_MODE_ABS && p->mode != VECSZ_MODE_R2R) return fail(c, VECSZ_EINVAL, "mode");
+    if (need_dims) {
+        if (p->ndim < 1 || p->ndim > 3) return fail(c, VECSZ_EINVAL, "ndim must be 1, 2 or 3");
+        for (int i = 0; i < 3; i++) {
+            uint64_t d = i < p->ndim ? p->dims[i] : 1;
+            if (d == 0 || d > 0x7FFFFFFFull) return fail(c, VECSZ_EINVAL, "dims must be in [1, 2^31)");
+        }
+    }
+    int R = p->dict_size / 2;
+    if (vec_threshold(R) != R) return fail(c, VECSZ_EINVAL, "dict_size not supported (AVX test differs from |e|<R)");
+    return VECSZ_OK;
+}
+
+uint64_t meta_bytes(int ndim) { return 4ull * (ndim + 1) + 8 + 24; }
+
+struct EbFamily {
+    double eb_final, r_d, twoeb;
+    float r_f;
+};
+
+EbFamily eb_family(double eb, bool r2r, double range) {
+    EbFamily f;
+    f.eb_final = eb;
+    if (r2r) f.eb_final *= range;
+    f.r_d = 1 / (2 * f.eb_final);
+    f.twoeb = 2 * f.eb_final;
+    f.r_f = (float)f.r_d;
+    return f;
+}
+
+float ord_to_float(uint32_t u) {
+    uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+// GetDatumValueRange semantics on a device field.
+int device_value_range(vecsz_ctx* c, const float* d_in, uint64_t n, double* range) {
+    if (!c->range.ensure(16)) return fail(c, VECSZ_ENOMEM, "range buffer");
+    uint32_t init[2] = {0xFFFFFFFFu, 0u};
+    HIPCHK(c, hipMemcpyAsync(c->range.p, init, 8, hipMemcpyHostToDevice, c->stream));
+    {
+        StageScope s(c, "range");
+        launch_range(d_in, n, c->range.as<uint32_t>(), c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    if (!c->pin.ensure(64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    uint32_t* h = c->pin.as<uint32_t>();
+    HIPCHK(c, hipMemcpyAsync(h, c->range.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h + 2, d_in, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float x0;
+    memcpy(&x0, h + 2, 4);
+    float mx, mn;
+    if (x0 != x0) {
+        mx = mn = x0;  // std::max_element keeps a leading NaN
+    } else {
+        mn = ord_to_float(h[0]);
+        mx = ord_to_float(h[1]);
+    }
+    float r = mx - mn;
+    *range = (double)r;
+    return VECSZ_OK;
+}
+
+void put_u32be(uint8_t* b, uint32_t v) {
+    b[0] = (uint8_t)(v >> 24);
+    b[1] = (uint8_t)(v >> 16);
+    b[2] = (uint8_t)(v >> 8);
+    b[3] = (uint8_t)v;
+}
+
+int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                    uint64_t* out_bytes, uint64_t* n_outlier) {
+    int rc = check_params(c, p, true);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    timing_begin(c);
+    const int ndim = p->ndim;
+    const uint64_t d0 = p->dims[0], d1 = ndim >= 2 ? p->dims[1] : 1, d2 = ndim >= 3 ? p->dims[2] : 1;
+    const uint64_t n = d0 * d1 * d2;
+    const int dict = p->dict_size, R = dict / 2, B = p->block;
+    const bool r2r = p->mode == VECSZ_MODE_R2R;
+
+    double range = 0;
+    if (r2r) {
+        if (p->value_range > 0) {
+            range = p->value_range;
+        } else {
+            rc = device_value_range(c, d_in, n, &range);
+            if (rc) return rc;
+        }
+        if (range == 0) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
+    }
+    const EbFamily eb = eb_family(p->eb, r2r, range);
+
+    if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
+        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+    if (!c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) || !c->bit_off.ensure(nchunks * 8) ||
+        !c->out_off.ensure(nchunks * 8) || !c->cw_hi.ensure((size_t)dict * 4) || !c->cw_lo.ensure((size_t)dict * 4) ||
+        !c->cw_len.ensure((size_t)dict))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+
+    DQArgs a;
+    a.d0 = d0;
+    a.d1 = d1;
+    a.d2 = d2;
+    a.n = n;
+    a.x8 = (d0 / 8) * 8;
+    a.r_f = eb.r_f;
+    a.r_d = eb.r_d;
+    a.radius = R;
+    a.radius_f = (float)R;
+    a.thr_vec = (float)vec_threshold(R);
+    a.thr_scalar = (float)R;
+    a.dict = dict;
+    a.ntiles_x = (d0 + 63) / 64;
+    a.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
+    a.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
+    uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : a.ntiles_x * a.ntiles_y * a.ntiles_z;
+    int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 1280));
+
+    HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)dict * 8, c->stream));
+    {
+        StageScope s(c, "dualquant");
+        launch_dualquant(ndim, B, d_in, c->codes.as<uint16_t>(), c->sparse.as<float>(),
+                         c->hist.as<unsigned long long>(), a, grid, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->hist_host.resize(dict);
+    if (!c->pin.ensure((size_t)dict * 8 + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    HIPCHK(c, hipMemcpyAsync(c->pin.p, c->hist.p, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(c->hist_host.data(), c->pin.p, (size_t)dict * 8);
+
+    Codebook cb;
+    rc = build_codebook(c->hist_host.data(), dict, cb);
+    if (rc == VECSZ_ECODELEN) return fail(c, rc, "Huffman code longer than 64 bits");
+    if (rc) return fail(c, rc, "codebook construction failed");
+
+    // Header: datapack metadata (huff_size / n_outlier filled by k_scan_chunks),
+    // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and tree.
+    const uint64_t mb = meta_bytes(ndim);
+    const uint64_t tb = cb.tree.size();
+    const uint64_t head = mb + 8 + tb;
+    if (head > out_cap) {
+        *out_bytes = head;
+        return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    }
+    std::vector<uint8_t> hdr(head, 0);
+    {
+        int32_t nd = ndim;
+        memcpy(&hdr[0], &nd, 4);
+        for (int i = 0; i < ndim; i++) {
+            int32_t d = (int32_t)p->dims[i];
+            memcpy(&hdr[4 + 4 * i], &d, 4);
+        }
+        memcpy(&hdr[4 * (ndim + 1)], &p->eb, 8);
+        // pad_len = 0; huff_size and n_outlier are written on the device
+        put_u32be(&hdr[mb], (uint32_t)cb.node_count);
+        put_u32be(&hdr[mb + 4], (uint32_t)dict);  // stateNum / 2
+        memcpy(&hdr[mb + 8], cb.tree.data(), tb);
+    }
+    HIPCHK(c, hipMemcpyAsync(d_out, hdr.data(), head, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cw_hi.p, cb.cw_hi.data(), (size_t)dict * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cw_lo.p, cb.cw_lo.data(), (size_t)dict * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->cw_len.p, cb.len.data(), (size_t)dict, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->err.p, 0, 16, c->stream));
+    {
+        StageScope s(c, "chunk_bits");
+        launch_chunk_bits(c->codes.as<uint16_t>(), n, c->cw_len.as<uint8_t>(), dict, c->chunk_bits.as<uint32_t>(),
+                          c->chunk_zeros.as<uint32_t>(), nchunks, c->stream);
+    }
+    {
+        StageScope s(c, "scan");
+        launch_scan_chunks(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), nchunks,
+                           c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb,
+                           tb, out_cap, c->err.as<uint32_t>(), c->stream);
+    }
+    {
+        StageScope s(c, "pack");
+        launch_pack(c->codes.as<uint16_t>(), n, c->cw_hi.as<uint32_t>(), c->cw_lo.as<uint32_t>(),
+                    c->cw_len.as<uint8_t>(), dict, c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), nchunks,
+                    c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out, c->err.as<uint32_t>(),
+                    std::max(cb.max_len, 1), c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    uint8_t* h = c->pin.as<uint8_t>();
+    HIPCHK(c, hipMemcpyAsync(h, c->summary.p, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h + 128, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    EncSummary S;
+    memcpy(&S, h, sizeof(S));
+    uint32_t e;
+    memcpy(&e, h + 128, 4);
+    timing_end(c);
+    *out_bytes = S.archive_size;
+    if (n_outlier) *n_outlier = S.n_outlier;
+    if (e) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    return VECSZ_OK;
+}
+
+int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint64_t dims[3], double* eb,
+               uint64_t* pad_len, uint64_t* huff_size, uint64_t* n_outlier) {
+    if (avail < 4) return VECSZ_ECORRUPT;
+    int32_t nd;
+    memcpy(&nd, h, 4);
+    if (nd < 1 || nd > 3) return VECSZ_ECORRUPT;
+    const uint64_t mb = meta_bytes(nd);
+    if (avail < mb) return VECSZ_ECORRUPT;
+    *ndim = nd;
+    dims[0] = dims[1] = dims[2] = 1;
+    for (int i = 0; i < nd; i++) {
+        int32_t d;
+        memcpy(&d, h + 4 + 4 * i, 4);
+        if (d <= 0) return VECSZ_ECORRUPT;
+        dims[i] = (uint64_t)d;
+    }
+    memcpy(eb, h + 4 * (nd + 1), 8);
+    uint64_t lens[3];
+    memcpy(lens, h + 4 * (nd + 1) + 8, 24);
+    *pad_len = lens[0];
+    *huff_size = lens[1];
+    *n_outlier = lens[2];
+    if (lens[0] > total || lens[1] > total || lens[2] > total) return VECSZ_ECORRUPT;
+    if (mb + 4 * lens[0] + lens[1] + 4 * lens[2] > total) return VECSZ_ECORRUPT;
+    return VECSZ_OK;
+}
+
+// Serial decode on the host of a stream whose chunk-parallel decode did not
+// synchronise (e.g. fixed-length codes): the true start, count and zero count
+// of every subsequence.
+int host_resync(vecsz_ctx* c, const DecodeTables& dt, const uint8_t* d_stream, uint64_t stream_bytes, uint64_t n,
+                uint64_t nsub) {
+    std::vector<uint8_t> st(stream_bytes + 16, 0);
+    HIPCHK(c, hipMemcpyAsync(st.data(), d_stream, stream_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint64_t> start(nsub, 0);
+    std::vector<uint32_t> cnt(nsub, 0), zeros(nsub, 0);
+    const uint64_t T = stream_bytes * 8;
+    uint64_t pos = 0, produced = 0;
+    auto bit = [&](uint64_t b) -> uint32_t { return (st[b >> 3] >> (7 - (b & 7))) & 1; };
+    uint64_t cur = 0;
+    start[0] = 0;
+    while (pos < T && produced < n + 8) {
+        while (cur + 1 < nsub && pos >= (cur + 1) * (uint64_t)kSubBits) {
+            cur++;
+            start[cur] = pos;
+        }
+        uint32_t node = 0;
+        uint32_t sym = 0;
+        for (;;) {
+            if (pos >= T + 64) return fail(c, VECSZ_ECORRUPT, "bitstream truncated");
+            uint32_t nx = dt.child[2 * node + (pos < T ? bit(pos) : 0)];
+            pos++;
+            if (nx == kNodeAbsent) return fail(c, VECSZ_ECORRUPT, "invalid code path");
+            if (nx & kNodeLeaf) {
+                sym = nx & 0xFFFF;
+                break;
+            }
+            node = nx;
+        }
+        cnt[cur]++;
+        zeros[cur] += sym == 0;
+        produced++;
+    }
+    for (uint64_t j = cur + 1; j < nsub; j++) start[j] = T;
+    HIPCHK(c, hipMemcpyAsync(c->sub_start.p, start.data(), nsub * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sub_cnt.p, cnt.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->sub_zeros.p, zeros.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VECSZ_OK;
+}
+
+bool tree_is_full(const DecodeTables& dt) {
+    // every internal node reachable from the root has both children
+    std::vector<uint32_t> stk{0};
+    std::vector<uint8_t> seen(dt.node_count, 0);
+    while (!stk.empty()) {
+        uint32_t nd = stk.back();
+        stk.pop_back();
+        if (seen[nd]) return false;  // cycle or shared node
+        seen[nd] = 1;
+        for (int b = 0; b < 2; b++) {
+            uint32_t ch = dt.child[2 * nd + b];
+            if (ch == kNodeAbsent) return false;
+            if (!(ch & kNodeLeaf)) stk.push_back(ch);
+        }
+    }
+    return true;
+}
+
+int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes, float* d_out,
+                      uint64_t out_cap) {
+    int rc = check_params(c, p, false);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    timing_begin(c);
+    if (!c->pin.ensure(1 << 16)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    uint8_t* h = c->pin.as<uint8_t>();
+    const uint64_t head_avail = std::min<uint64_t>(in_bytes, 56);
+    HIPCHK(c, hipMemcpyAsync(h, d_in, head_avail, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int ndim;
+    uint64_t dims[3], pad_len, huff_size, nout;
+    double ebu;
+    rc = parse_meta(h, head_avail, in_bytes, &ndim, dims, &ebu, &pad_len, &huff_size, &nout);
+    if (rc) return fail(c, rc, "archive header is corrupt");
+    if (pad_len != 0) return fail(c, VECSZ_EINVAL, "archives with padding values are not supported");
+    const uint64_t n = dims[0] * dims[1] * dims[2];
+    if (n > out_cap) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    const bool r2r = p->mode == VECSZ_MODE_R2R;
+    if (r2r && !(p->value_range != 0)) return fail(c, VECSZ_EINVAL, "r2r decompress needs params.value_range");
+    const EbFamily eb = eb_family(ebu, r2r, p->value_range);
+    const int B = p->block, R = p->dict_size / 2;
+
+    const uint64_t mb = meta_bytes(ndim);
+    const uint8_t* d_huff = d_in + mb + 4 * pad_len;
+    if (huff_size < 8) return fail(c, VECSZ_ECORRUPT, "Huffman section too short");
+    HIPCHK(c, hipMemcpyAsync(h, d_huff, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int32_t ncs = (int32_t)(((uint32_t)h[0] << 24) | ((uint32_t)h[1] << 16) | ((uint32_t)h[2] << 8) | h[3]);
+    if (ncs <= 0) return fail(c, VECSZ_ECORRUPT, "Huffman node count");
+    const uint64_t tb = tree_byte_size((uint64_t)ncs);
+    if (8 + tb > huff_size) return fail(c, VECSZ_ECORRUPT, "Huffman tree truncated");
+    std::vector<uint8_t> hh(8 + tb);
+    HIPCHK(c, hipMemcpyAsync(hh.data(), d_huff, 8 + tb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DecodeTables dt;
+    rc = build_decode_tables(hh.data(), huff_size, dt);
+    if (rc) return fail(c, rc, "Huffman tree is corrupt");
+
+    if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    const uint8_t* d_stream = d_huff + 8 + tb;
+    const uint64_t stream_bytes = huff_size - 8 - tb;
+    const uint8_t* d_outl = d_huff + huff_size;
+    uint16_t* codes = c->codes.as<uint16_t>();
+    float* sparse = c->sparse.as<float>();
+
+    if (dt.root_leaf) {
+        StageScope s(c, "decode");
+        launch_fill_u16(codes, n, (uint16_t)dt.root_sym, c->stream);
+        if (dt.root_sym == 0) {
+            if (nout < n) return fail(c, VECSZ_ECORRUPT, "outlier section shorter than the field");
+            launch_scatter_all_outliers(d_outl, n, sparse, c->stream);
+        }
+    } else {
+        if (!tree_is_full(dt)) return fail(c, VECSZ_ECORRUPT, "Huffman tree is not full");
+        const uint64_t T = stream_bytes * 8;
+        const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
+        if (!c->lut.ensure(dt.lut.size() * 4) || !c->child.ensure(dt.child.size() * 4) ||
+            !c->sub_start.ensure(nsub * 8) || !c->sub_end.ensure(nsub * 8) || !c->sub_end2.ensure(nsub * 8) ||
+            !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) ||
+            !c->sub_bnd.ensure(nsub * kSyncRecord * 2) || !c->sub_zmask.ensure(nsub * 2) ||
+            !c->sub_nbnd.ensure(nsub) || !c->sub_dirty.ensure(nsub * 2) || !c->sym_off.ensure(nsub * 8) ||
+            !c->zero_off.ensure(nsub * 8) || !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) ||
+            !c->scan_tot.ensure(16) || !c->dec_sum.ensure(sizeof(DecSummary)))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        HIPCHK(c, hipMemcpyAsync(c->lut.p, dt.lut.data(), dt.lut.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->child.p, dt.child.data(), dt.child.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->dec_sum.p, 0, sizeof(DecSummary), c->stream));
+        DecodeArgs da;
+        da.stream = d_stream;
+        da.stream_bits = T;
+        da.n = n;
+        da.lut = c->lut.as<uint32_t>();
+        da.child = c->child.as<uint32_t>();
+        da.nsub = nsub;
+        da.outliers = d_outl;
+        da.n_outlier = nout;
+        SubState ss;
+        ss.start = c->sub_start.as<uint64_t>();
+        ss.end = c->sub_end.as<uint64_t>();
+        ss.cnt = c->sub_cnt.as<uint32_t>();
+        ss.zeros = c->sub_zeros.as<uint32_t>();
+        ss.bnd = c->sub_bnd.as<uint16_t>();
+        ss.zmask = c->sub_zmask.as<uint16_t>();
+        ss.nbnd = c->sub_nbnd.as<uint8_t>();
+        ss.dirty = c->sub_dirty.as<uint8_t>();
+        DecSummary* dsum = c->dec_sum.as<DecSummary>();
+        {
+            StageScope s(c, "decode_sync");
+            launch_decode_sync(da, ss, dsum, c->stream);
+            launch_decode_check(da, ss, dsum, c->stream);
+        }
+        HIPCHK(c, hipGetLastError());
+        DecSummary ds;
+        HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        memcpy(&ds, h, sizeof(ds));
+        if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
+        uint64_t* e_in = ss.end;
+        uint64_t* e_out = c->sub_end2.as<uint64_t>();
+        uint8_t* dcur = ss.dirty;
+        uint8_t* dnxt = ss.dirty + nsub;
+        int rounds = 0;
+        while (ds.n_changed > 0 && rounds < 32) {
+            HIPCHK(c, hipMemsetAsync(&dsum->n_changed, 0, 4, c->stream));
+            {
+                StageScope s(c, "decode_redo");
+                launch_decode_redo(da, ss, e_in, e_out, dcur, dnxt, dsum, c->stream);
+            }
+            HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            memcpy(&ds, h, sizeof(ds));
+            if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
+            std::swap(e_in, e_out);
+            std::swap(dcur, dnxt);
+            rounds++;
+        }
+        if (ds.n_changed > 0) {
+            rc = host_resync(c, dt, d_stream, stream_bytes, n, nsub);
+            if (rc) return rc;
+        }
+        {
+            StageScope s(c, "decode_scan");
+            launch_scan_u32x2(ss.cnt, ss.zeros, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
+                              c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream);
+        }
+        {
+            StageScope s(c, "decode_write");
+            launch_decode_write(da, ss, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(), codes, sparse, dsum,
+                                c->stream);
+        }
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(h + 64, c->scan_tot.p, 16, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        memcpy(&ds, h, sizeof(ds));
+        uint64_t tot[2];
+        memcpy(tot, h + 64, 16);
+        if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
+        if (tot[0] < n) return fail(c, VECSZ_ECORRUPT, "bitstream holds fewer symbols than the field");
+    }
+
+    // Lorenzo reconstruction.
+    const uint64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    ReconArgs ra;
+    ra.d0 = d0;
+    ra.d1 = d1;
+    ra.d2 = d2;
+    ra.n = n;
+    ra.radius = R;
+    ra.twoeb = eb.twoeb;
+    ra.ntiles_x = (d0 + 63) / 64;
+    ra.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
+    ra.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
+    const uint64_t nblocks = ((d0 + B - 1) / B) * (ndim >= 2 ? (d1 + B - 1) / B : 1) * (ndim >= 3 ? (d2 + B - 1) / B : 1);
+    if (!c->flagged.ensure(nblocks * 4 + 16) || !c->nflagged.ensure(16))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    HIPCHK(c, hipMemsetAsync(c->nflagged.p, 0, 4, c->stream));
+    const uint64_t work = ra.ntiles_x * ra.ntiles_y * ra.ntiles_z;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 2048));
+    {
+        StageScope s(c, "reconstruct");
+        launch_reconstruct(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(), c->nflagged.as<uint32_t>(),
+                           grid, c->stream);
+        launch_reconstruct_fallback(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(),
+                                    c->nflagged.as<uint32_t>(), c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timing_end(c);
+    return VECSZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* vecsz_version(void) { return "vecsz_amd 0.1 gfx950"; }
+
+const char* vecsz_strerror(int s) {
+    switch (s) {
+        case VECSZ_OK: return "ok";
+        case VECSZ_EINVAL: return "invalid argument";
+        case VECSZ_ENOMEM: return "out of memory or buffer too small";
+        case VECSZ_ECORRUPT: return "corrupt archive";
+        case VECSZ_ECODELEN: return "Huffman code longer than 64 bits";
+        case VECSZ_EHIP: return "HIP runtime error";
+        case VECSZ_ERANGE0: return "zero value range in r2r mode";
+        case VECSZ_ENODEV: return "no usable GPU";
+        default: return "unknown status";
+    }
+}
+
+void vecsz_params_default(vecsz_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->ndim = 1;
+    p->dims[0] = p->dims[1] = p->dims[2] = 1;
+    p->mode = VECSZ_MODE_ABS;
+    p->eb = 1e-4;
+    p->value_range = 0;
+    p->block = 8;
+    p->dict_size = 4096;
+}
+
+int vecsz_ctx_create(vecsz_ctx** out, int device, void* stream) {
+    if (!out) return VECSZ_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return VECSZ_ENODEV;
+    }
+    if (device < 0 || device >= ndev) return VECSZ_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return VECSZ_EHIP;
+    vecsz_ctx* c = new vecsz_ctx();
+    c->device = device;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            return VECSZ_EHIP;
+        }
+        c->own_stream = true;
+    }
+    *out = c;
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_destroy(vecsz_ctx* c) {
+    if (!c) return VECSZ_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VECSZ_OK;
+}
+
+const char* vecsz_ctx_last_error(vecsz_ctx* c) { return c ? c->last_error.c_str() : "no context"; }
+
+int vecsz_compress_bound(const vecsz_params* p, uint64_t* bytes) {
+    if (!p || !bytes || p->ndim < 1 || p->ndim > 3 || p->dict_size < 2 || p->dict_size > 65536) return VECSZ_EINVAL;
+    uint64_t n = 1;
+    for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
+    const uint64_t nsym = std::min<uint64_t>((uint64_t)p->dict_size, n);
+    const uint64_t tree = tree_byte_size(2 * nsym - 1);
+    // every symbol at most 64 bits, every element possibly an outlier
+    *bytes = meta_bytes(p->ndim) + 8 + tree + n * 8 + n * 4 + 8;
+    return VECSZ_OK;
+}
+
+int vecsz_compress(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                   uint64_t* out_bytes, uint64_t* n_outlier) {
+    if (!c || !d_in || !d_out || !out_bytes) return fail(c, VECSZ_EINVAL, "NULL argument");
+    return compress_device(c, p, d_in, d_out, out_cap, out_bytes, n_outlier);
+}
+
+int vecsz_decompress(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes, float* d_out,
+                     uint64_t out_cap) {
+    if (!c || !d_in || !d_out) return fail(c, VECSZ_EINVAL, "NULL argument");
+    return decompress_device(c, p, d_in, in_bytes, d_out, out_cap);
+}
+
+int vecsz_compress_host(vecsz_ctx* c, const vecsz_params* p, const float* h_in, uint8_t* h_out, uint64_t out_cap,
+                        uint64_t* out_bytes, uint64_t* n_outlier) {
+    if (!c || !p || !h_in || !h_out || !out_bytes) return fail(c, VECSZ_EINVAL, "NULL argument");
+    int rc = check_params(c, p, true);
+    if (rc) return rc;
+    uint64_t n = 1;
+    for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
+    uint64_t bound;
+    vecsz_compress_bound(p, &bound);
+    if (!c->stage_in.ensure(n * 4) || !c->stage_out.ensure(bound)) return fail(c, VECSZ_ENOMEM, "staging buffers");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->stage_in.p, h_in, n * 4, hipMemcpyHostToDevice, c->stream));
+    uint64_t sz = 0;
+    rc = compress_device(c, p, c->stage_in.as<float>(), c->stage_out.as<uint8_t>(), bound, &sz, n_outlier);
+    *out_bytes = sz;
+    if (rc) return rc;
+    if (sz > out_cap) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    HIPCHK(c, hipMemcpyAsync(h_out, c->stage_out.p, sz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VECSZ_OK;
+}
+
+int vecsz_decompress_host(vecsz_ctx* c, const vecsz_params* p, const uint8_t* h_in, uint64_t in_bytes, float* h_out,
+                          uint64_t out_cap) {
+    if (!c || !p || !h_in || !h_out) return fail(c, VECSZ_EINVAL, "NULL argument");
+    int ndim;
+    uint64_t dims[3], pad, huff, nout;
+    double eb;
+    int rc = parse_meta(h_in, in_bytes, in_bytes, &ndim, dims, &eb, &pad, &huff, &nout);
+    if (rc) return fail(c, rc, "archive header is corrupt");
+    const uint64_t n = dims[0] * dims[1] * dims[2];
+    if (n > out_cap) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    if (!c->stage_in.ensure(in_bytes + 64) || !c->stage_out.ensure(n * 4)) return fail(c, VECSZ_ENOMEM, "staging buffers");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(c->stage_in.p, h_in, in_bytes, hipMemcpyHostToDevice, c->stream));
+    rc = decompress_device(c, p, c->stage_in.as<uint8_t>(), in_bytes, c->stage_out.as<float>(), n);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(h_out, c->stage_out.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VECSZ_OK;
+}
+
+int vecsz_archive_info(const uint8_t* h, uint64_t bytes, int* ndim, uint64_t dims[3], double* eb, uint64_t* huff_size,
+                       uint64_t* n_outlier) {
+    if (!h || !ndim || !dims || !eb || !huff_size || !n_outlier) return VECSZ_EINVAL;
+    uint64_t pad;
+    return parse_meta(h, bytes, bytes, ndim, dims, eb, &pad, huff_size, n_outlier);
+}
+
+int vecsz_ctx_enable_timing(vecsz_ctx* c, int on) {
+    if (!c) return VECSZ_EINVAL;
+    c->timing = on != 0;
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_timings(vecsz_ctx* c, const char** names, double* ms, int cap) {
+    if (!c) return 0;
+    int n = (int)std::min<size_t>(c->tnames.size(), (size_t)cap);
+    for (int i = 0; i < n; i++) {
+        if (names) names[i] = c->tnames[i].c_str();
+        if (ms) ms[i] = c->tms[i];
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Host-side Huffman codebook construction and decode-table preparation.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+namespace vsz {
+
+struct Codebook {
+    int dict = 0;        // symbols live in [0, dict)
+    int nsym = 0;        // symbols with nonzero frequency
+    int node_count = 0;  // 2 * nsym - 1 (src/huffman.hh:643-645)
+    int max_len = 0;
+    std::vector<uint32_t> cw_hi, cw_lo;  // codeword bits, right-aligned, split at 32
+    std::vector<uint8_t> len;            // codeword length (0 for a one-symbol tree)
+    std::vector<uint8_t> tree;           // serialized tree (src/huffman.hh:425-502)
+};
+
+// Builds the reference's tree from a symbol histogram of `dict` bins.
+// Returns a VECSZ_* status.
+int build_codebook(const uint64_t* freq, int dict, Codebook& cb);
+
+struct DecodeTables {
+    int node_count = 0;
+    uint64_t tree_bytes = 0;
+    bool root_leaf = false;
+    uint32_t root_sym = 0;
+    std::vector<uint32_t> lut;    // 1 << kLutBits entries
+    std::vector<uint32_t> child;  // 2 * node_count entries
+};
+
+// Parses the Huffman section of an archive (header + serialized tree) into
+// decoder tables.  `huff` points at the section, `huff_size` is its length.
+int build_decode_tables(const uint8_t* huff, uint64_t huff_size, DecodeTables& dt);
+
+// Byte size of a serialized tree with `node_count` nodes.
+uint64_t tree_byte_size(uint64_t node_count);
+
+}  // namespace vsz

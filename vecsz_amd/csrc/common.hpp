@@ -1,0 +1,62 @@
+// Internal definitions shared by the host orchestration and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace vsz {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kChunk = 4096;       // symbols per encode chunk (one workgroup)
+constexpr int kEncThreads = 256;   // encode workgroup size
+constexpr int kSymPerThread = kChunk / kEncThreads;  // 16
+constexpr int kLutBits = 12;       // first-level decode table width
+constexpr int kSubBits = 1024;     // bits per decode subsequence
+constexpr int kSyncRecord = 16;    // codeword boundaries recorded per subsequence
+
+// Scalars of one dual-quantization launch.  Names follow the reference's
+// dims_L16 / ebs_L4 (src/constants.hh:14-31).
+struct DQArgs {
+    uint64_t d0, d1, d2;   // DIM0 (x), DIM1, DIM2
+    uint64_t n;            // LEN
+    uint64_t x8;           // 8 * floor(DIM0 / 8): first x of the partial AVX group
+    float r_f;             // (float) 1/(2 eb_final)
+    double r_d;            // 1/(2 eb_final)
+    float radius_f;        // (float) radius
+    float thr_vec;         // |e| < thr_vec  <=>  sqrt(e*e) < radius  for integral e
+    float thr_scalar;      // |e| < thr_scalar (= radius)
+    int radius;
+    int dict;              // histogram bins
+    uint64_t ntiles_x, ntiles_y, ntiles_z;
+};
+
+// Per-field encode summary written by the scan kernel.
+struct EncSummary {
+    uint64_t total_bits;
+    uint64_t n_outlier;
+    uint64_t huff_size;
+    uint64_t archive_size;
+    uint64_t outlier_byte_off;  // byte offset of the outlier section in the archive
+    uint64_t stream_bit_off;    // absolute bit offset of the bitstream in the archive
+    uint64_t stream_end_byte;   // first byte after the bitstream
+};
+
+// Decoder summary.
+struct DecSummary {
+    uint32_t error;       // nonzero: corrupt stream
+    uint32_t n_changed;   // sync rounds: subsequences whose end moved
+    uint64_t total_syms;  // symbols found by the sync pass
+};
+
+// Lookup-table entry of the decoder (kLutBits-bit window):
+//  bit 31 clear: leaf reached; bits 16..23 = code length (1..kLutBits), bits 0..15 = symbol
+//  bit 31 set  : bits 0..30 = index of the internal node reached after kLutBits bits
+//  0           : invalid path (corrupt tree)
+constexpr uint32_t kLutSlow = 0x80000000u;
+// Node table entry for the bit-serial slow path: child[2*i + bit]
+//  bit 31 set: leaf, bits 0..15 symbol; otherwise internal node index; 0xFFFFFFFF = absent.
+constexpr uint32_t kNodeLeaf = 0x80000000u;
+constexpr uint32_t kNodeAbsent = 0xFFFFFFFFu;
+
+}  // namespace vsz

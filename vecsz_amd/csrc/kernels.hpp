@@ -1,0 +1,69 @@
+// Launch wrappers of the gfx950 kernels (host-callable).
+#pragma once
+
+#include "common.hpp"
+
+namespace vsz {
+
+constexpr int kMaxLdsHistBins = 8192;   // LDS-private histogram up to this dict size
+constexpr int kMaxLdsLenTab = 16384;    // LDS code-length table up to this dict size
+
+int dq_lds_bytes(int dict);
+void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st);
+void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
+                      const DQArgs& a, int grid, hipStream_t st);
+void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
+                       uint64_t nchunks, hipStream_t st);
+void launch_scan_chunks(const uint32_t* cb, const uint32_t* cz, uint64_t nchunks, uint64_t* bit_off, uint64_t* out_off,
+                        EncSummary* sum, uint8_t* archive, uint64_t meta_bytes, uint64_t tree_bytes, uint64_t out_cap,
+                        uint32_t* err, hipStream_t st);
+void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
+                 int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
+                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st);
+
+// Decoder (kernels_decompress.hip)
+struct DecodeArgs {
+    const uint8_t* stream;   // first byte of the bitstream (any alignment)
+    uint64_t stream_bits;    // 8 * stream bytes
+    uint64_t n;              // symbols to decode
+    const uint32_t* lut;     // 1 << kLutBits entries
+    const uint32_t* child;   // 2 * node_count entries
+    uint64_t nsub;           // subsequences of kSubBits bits
+    const uint8_t* outliers; // outlier section of the archive (any alignment)
+    uint64_t n_outlier;
+};
+
+struct SubState {
+    uint64_t* start;    // true (or assumed) start bit of each subsequence
+    uint64_t* end;      // end bit reached by its decode
+    uint32_t* cnt;      // symbols whose codeword starts in [start, end)
+    uint32_t* zeros;    // of which are symbol 0
+    uint16_t* bnd;      // first kSyncRecord codeword starts (offset from start), per subsequence
+    uint16_t* zmask;    // symbol-0 flags of those codewords
+    uint8_t* nbnd;      // recorded boundary count
+    uint8_t* dirty;     // needs a re-decode from end[j-1]
+};
+
+void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
+void launch_decode_check(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
+void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
+                        uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st);
+void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
+                       uint64_t* totals, hipStream_t st);
+void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
+                         uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st);
+void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st);
+void launch_scatter_all_outliers(const uint8_t* outliers, uint64_t n, float* sparse, hipStream_t st);
+
+struct ReconArgs {
+    uint64_t d0, d1, d2, n;
+    int radius;
+    double twoeb;
+    uint64_t ntiles_x, ntiles_y, ntiles_z;
+};
+void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* sparse, float* out, const ReconArgs& a,
+                        uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st);
+void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
+                                 const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st);
+
+}  // namespace vsz

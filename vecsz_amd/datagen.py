@@ -1,0 +1,159 @@
+"""Seeded synthetic fields for the workloads this library is measured on.
+
+All fields are float32, C-order numpy arrays whose LAST axis is x (the
+fastest-varying dimension the archive lists first).
+
+* ``sinusoids_2d`` -- sum of 8 random-phase plane waves (wavelength U(50, 900)
+  px, amplitude U(0, 1), random direction) plus N(0, 1e-3^2) noise; the 2D
+  1800x3600 (y, x) field compressed at abs eb 1e-4.
+* ``grf`` -- Gaussian random field synthesised by FFT whose shell-integrated
+  energy spectrum is E(k) ~ k^-5/3 (so the d-dimensional power spectrum is
+  P(k) ~ k^-(5/3 + d - 1)), min/max scaled to [-50, 50]; the 3D 512^3 and
+  1024x512x512 fields and the batched mix.
+* ``walk_spikes_1d`` -- random walk with N(0, 1e-2^2) steps plus 0.1% seeded
+  spikes of magnitude U(10, 1000) and random sign; the 2^28-element 1D stream.
+
+Generation uses numpy on the host by default; ``device="cuda"`` generates the
+same kind of field with torch on the GPU (fast for 512 MiB+ fields; the bits
+differ from the numpy path, so parity tests always compare on the very array
+they compressed).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def sinusoids_2d(ny: int = 1800, nx: int = 3600, seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    y = np.arange(ny, dtype=np.float64)[:, None]
+    x = np.arange(nx, dtype=np.float64)[None, :]
+    f = np.zeros((ny, nx), dtype=np.float64)
+    for _ in range(8):
+        lam = rng.uniform(50.0, 900.0)
+        amp = rng.uniform(0.0, 1.0)
+        phase = rng.uniform(0.0, 2 * math.pi)
+        theta = rng.uniform(0.0, 2 * math.pi)
+        kx, ky = math.cos(theta) * 2 * math.pi / lam, math.sin(theta) * 2 * math.pi / lam
+        f += amp * np.sin(kx * x + ky * y + phase)
+    f += rng.normal(0.0, 1e-3, size=(ny, nx))
+    return np.ascontiguousarray(f.astype(np.float32))
+
+
+def _grf_numpy(shape, seed, lo, hi):
+    rng = np.random.default_rng(seed)
+    d = len(shape)
+    white = rng.standard_normal(size=shape, dtype=np.float32)
+    spec = np.fft.rfftn(white)
+    ks = [np.fft.fftfreq(n) for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1])]
+    k2 = np.zeros(spec.shape, dtype=np.float64)
+    for i, kv in enumerate(ks):
+        sh = [1] * d
+        sh[i] = kv.size
+        k2 = k2 + (kv.reshape(sh) ** 2)
+    slope = (5.0 / 3.0 + d - 1) / 2.0  # amplitude exponent: sqrt of P(k)
+    with np.errstate(divide="ignore"):
+        amp = np.where(k2 > 0, k2 ** (-slope / 2.0), 0.0)
+    spec *= amp
+    f = np.fft.irfftn(spec, s=shape, axes=tuple(range(d)))
+    del spec, amp, k2
+    fmin, fmax = float(f.min()), float(f.max())
+    f = (f - fmin) * ((hi - lo) / (fmax - fmin)) + lo
+    return np.ascontiguousarray(f.astype(np.float32))
+
+
+def _grf_torch(shape, seed, lo, hi):
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    d = len(shape)
+    white = torch.randn(shape, generator=g, device="cuda", dtype=torch.float32)
+    spec = torch.fft.rfftn(white)
+    del white
+    ks = [torch.fft.fftfreq(n, device="cuda") for n in shape[:-1]] + [torch.fft.rfftfreq(shape[-1], device="cuda")]
+    k2 = torch.zeros(spec.shape, device="cuda", dtype=torch.float32)
+    for i, kv in enumerate(ks):
+        sh = [1] * d
+        sh[i] = kv.numel()
+        k2 += kv.reshape(sh) ** 2
+    slope = (5.0 / 3.0 + d - 1) / 2.0
+    amp = torch.where(k2 > 0, k2.clamp_min(1e-30) ** (-slope / 2.0), torch.zeros_like(k2))
+    del k2
+    spec *= amp
+    del amp
+    f = torch.fft.irfftn(spec, s=shape)
+    del spec
+    fmin, fmax = f.min(), f.max()
+    f = (f - fmin) * ((hi - lo) / (fmax - fmin)) + lo
+    return f.contiguous()
+
+
+def grf(shape, seed: int = 0, lo: float = -50.0, hi: float = 50.0, device: str = "cpu"):
+    """Gaussian random field, E(k) ~ k^-5/3, scaled to [lo, hi].
+
+    ``device="cpu"`` returns a numpy array; ``device="cuda"`` returns a torch
+    tensor on the GPU."""
+    shape = tuple(int(s) for s in shape)
+    if device == "cpu":
+        return _grf_numpy(shape, seed, lo, hi)
+    return _grf_torch(shape, seed, lo, hi)
+
+
+def walk_spikes_1d(n: int = 1 << 28, seed: int = 0, sigma: float = 1e-2, spike_frac: float = 1e-3,
+                   device: str = "cpu"):
+    """Random walk with N(0, sigma^2) steps plus seeded spikes."""
+    if device == "cpu":
+        rng = np.random.default_rng(seed)
+        x = np.cumsum(rng.normal(0.0, sigma, size=n))
+        k = int(n * spike_frac)
+        pos = rng.choice(n, size=k, replace=False)
+        mag = rng.uniform(10.0, 1000.0, size=k) * rng.choice([-1.0, 1.0], size=k)
+        x[pos] += mag
+        return np.ascontiguousarray(x.astype(np.float32))
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    steps = torch.randn(n, generator=g, device="cuda", dtype=torch.float64) * sigma
+    x = torch.cumsum(steps, 0)
+    del steps
+    k = int(n * spike_frac)
+    pos = torch.randperm(n, generator=g, device="cuda")[:k]
+    mag = (torch.rand(k, generator=g, device="cuda", dtype=torch.float64) * 990.0 + 10.0)
+    sign = torch.where(torch.rand(k, generator=g, device="cuda") < 0.5, -1.0, 1.0).to(torch.float64)
+    x[pos] += mag * sign
+    return x.to(torch.float32).contiguous()
+
+
+def batch_mix(n_fields: int = 96, seed: int = 0, target_bytes: float = 8 * 2**30, tol: float = 0.10):
+    """Shapes and error bounds of the batched mix: 2D sizes log-uniform from
+    64^2 to 4096^2 and 3D from 32^3 to 256x384x384, redrawn with successive
+    seeds until the total is within `tol` of `target_bytes`.  Returns a list
+    of dicts {shape (C order), eb, r2r} and the seed that was accepted."""
+    for attempt in range(10000):
+        rng = np.random.default_rng(seed + attempt)
+        fields = []
+        total = 0
+        for _ in range(n_fields):
+            if rng.random() < 0.5:
+                e = rng.uniform(math.log(64 * 64), math.log(4096 * 4096))
+                side = int(round(math.sqrt(math.exp(e))))
+                ny = max(64, min(4096, int(side * rng.uniform(0.8, 1.25))))
+                nx = max(64, min(4096, int(math.exp(e) / ny)))
+                shape = (ny, nx)
+            else:
+                e = rng.uniform(math.log(32 ** 3), math.log(256 * 384 * 384))
+                n = math.exp(e)
+                nz = max(32, min(256, int(round(n ** (1 / 3) * rng.uniform(0.7, 1.0)))))
+                rest = n / nz
+                ny = max(32, min(384, int(round(math.sqrt(rest) * rng.uniform(0.85, 1.15)))))
+                nx = max(32, min(384, int(round(rest / ny))))
+                shape = (nz, ny, nx)
+            eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+            fields.append({"shape": shape, "eb": eb, "r2r": True})
+            total += 4 * int(np.prod(shape))
+        if abs(total - target_bytes) <= tol * target_bytes:
+            return fields, seed + attempt
+    raise RuntimeError("no draw within tolerance")

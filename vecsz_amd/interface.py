@@ -1,0 +1,262 @@
+"""Host-side mirror of the reference's `vecsz::interface` (src/vecsz_interface.hh).
+
+`Compress(ap, data_in)` and `Decompress(ap, archive)` keep the reference's
+names and argument meaning: `ap` carries the fields of the reference's
+ArgParse that the path reads (ndim, dim4, eb, mode, dict_size, block_size),
+and the results are the archive payload (datapack::pack layout) and the
+reconstructed field.  They run on the GPU through libvecsz_hip.so; there is no
+CPU fallback, and a missing library raises.
+
+`Context` is the lower-level API: device tensors in, device tensors out, on
+one HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import Params, VecszError
+
+try:  # torch supplies device buffers and the stream; the library does the work
+    import torch
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _mode_id(mode) -> int:
+    if isinstance(mode, int):
+        return mode
+    m = str(mode).lower()
+    if m == "abs":
+        return _lib.MODE_ABS
+    if m in ("r2r", "vrrel"):
+        return _lib.MODE_R2R
+    raise ValueError(f"mode must be 'abs' or 'r2r', got {mode!r}")
+
+
+def make_params(dims_xfirst: Sequence[int], eb: float, mode="abs", block: int = 8, dict_size: int = 4096,
+                value_range: float = 0.0) -> Params:
+    p = Params()
+    d = [int(v) for v in dims_xfirst]
+    if not 1 <= len(d) <= 3:
+        raise ValueError("1 to 3 dimensions")
+    p.ndim = len(d)
+    for i in range(3):
+        p.dims[i] = d[i] if i < len(d) else 1
+    p.mode = _mode_id(mode)
+    p.eb = float(eb)
+    p.value_range = float(value_range)
+    p.block = int(block)
+    p.dict_size = int(dict_size)
+    return p
+
+
+def dims_of(shape) -> tuple:
+    """C-order shape (slowest first) -> archive dims (x first)."""
+    return tuple(int(s) for s in reversed(tuple(shape)))
+
+
+def archive_info(archive) -> dict:
+    lib = _lib.load()
+    b = bytes(archive) if not isinstance(archive, (bytes, bytearray)) else bytes(archive)
+    nd = ctypes.c_int()
+    dims = (ctypes.c_uint64 * 3)()
+    eb = ctypes.c_double()
+    hs = ctypes.c_uint64()
+    no = ctypes.c_uint64()
+    rc = lib.vecsz_archive_info(b, len(b), ctypes.byref(nd), dims, ctypes.byref(eb), ctypes.byref(hs), ctypes.byref(no))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return {"ndim": nd.value, "dims": tuple(dims[i] for i in range(nd.value)), "eb": eb.value,
+            "huff_size": hs.value, "n_outlier": no.value}
+
+
+class Context:
+    """One GPU, one HIP stream, reusable device scratch."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self._lib = _lib.load()
+        self.device = device
+        handle = None
+        if stream is None and torch is not None and torch.cuda.is_available():
+            stream = torch.cuda.current_stream(device)
+        if stream is not None:
+            handle = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
+        self.stream = stream
+        ctx = ctypes.c_void_p()
+        rc = self._lib.vecsz_ctx_create(ctypes.byref(ctx), device, ctypes.c_void_p(handle) if handle else None)
+        if rc:
+            raise VecszError(rc, self._lib.vecsz_strerror(rc).decode())
+        self._ctx = ctx
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.vecsz_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc:
+            msg = self._lib.vecsz_ctx_last_error(self._ctx).decode()
+            raise VecszError(rc, msg or self._lib.vecsz_strerror(rc).decode())
+
+    def enable_timing(self, on: bool = True):
+        self._check(self._lib.vecsz_ctx_enable_timing(self._ctx, int(on)))
+
+    def timings(self) -> list:
+        names = (ctypes.c_char_p * 64)()
+        ms = (ctypes.c_double * 64)()
+        n = self._lib.vecsz_ctx_timings(self._ctx, names, ms, 64)
+        return [(names[i].decode(), ms[i]) for i in range(n)]
+
+    # -------------------------------------------------------------- device API
+
+    def compress_bound(self, params: Params) -> int:
+        b = ctypes.c_uint64()
+        self._check(self._lib.vecsz_compress_bound(ctypes.byref(params), ctypes.byref(b)))
+        return b.value
+
+    def compress_into(self, params: Params, d_in_ptr: int, d_out_ptr: int, out_cap: int):
+        """Raw device pointers; returns (archive bytes, n_outlier)."""
+        size = ctypes.c_uint64()
+        nout = ctypes.c_uint64()
+        rc = self._lib.vecsz_compress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr),
+                                      ctypes.c_void_p(d_out_ptr), out_cap, ctypes.byref(size), ctypes.byref(nout))
+        self._check(rc)
+        return size.value, nout.value
+
+    def decompress_into(self, params: Params, d_in_ptr: int, in_bytes: int, d_out_ptr: int, out_cap: int):
+        rc = self._lib.vecsz_decompress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr), in_bytes,
+                                        ctypes.c_void_p(d_out_ptr), out_cap)
+        self._check(rc)
+
+    def compress(self, x, eb: float = 1e-4, mode="abs", block: int = 8, dict_size: int = 4096,
+                 value_range: float = 0.0, out=None):
+        """Compress a float32 field.
+
+        torch CUDA tensor -> (uint8 CUDA tensor holding the archive, n_outlier);
+        numpy array       -> (bytes, n_outlier) through host staging."""
+        if torch is not None and isinstance(x, torch.Tensor):
+            if not x.is_cuda or x.dtype != torch.float32:
+                raise TypeError("expected a float32 CUDA tensor")
+            x = x.contiguous()
+            p = make_params(dims_of(x.shape), eb, mode, block, dict_size, value_range)
+            cap = self.compress_bound(p)
+            if out is None or out.numel() < cap:
+                out = torch.empty(cap, dtype=torch.uint8, device=x.device)
+            size, nout = self.compress_into(p, x.data_ptr(), out.data_ptr(), out.numel())
+            return out[:size], nout
+        a = np.ascontiguousarray(x, dtype=np.float32)
+        p = make_params(dims_of(a.shape), eb, mode, block, dict_size, value_range)
+        cap = self.compress_bound(p)
+        buf = np.empty(cap, dtype=np.uint8)
+        size = ctypes.c_uint64()
+        nout = ctypes.c_uint64()
+        rc = self._lib.vecsz_compress_host(self._ctx, ctypes.byref(p), a.ctypes.data_as(ctypes.c_void_p),
+                                           buf.ctypes.data_as(ctypes.c_void_p), cap, ctypes.byref(size),
+                                           ctypes.byref(nout))
+        self._check(rc)
+        return buf[:size.value].tobytes(), nout.value
+
+    def decompress(self, archive, mode="abs", value_range: float = 0.0, block: int = 8, dict_size: int = 4096,
+                   out=None):
+        """Decompress an archive payload.
+
+        uint8 CUDA tensor -> float32 CUDA tensor (C-order shape from the archive);
+        bytes / numpy     -> numpy array."""
+        p = make_params((1,), 1e-4, mode, block, dict_size, value_range)
+        if torch is not None and isinstance(archive, torch.Tensor):
+            if not archive.is_cuda or archive.dtype != torch.uint8:
+                raise TypeError("expected a uint8 CUDA tensor")
+            head = archive[:64].cpu().numpy().tobytes()
+            info = archive_info(head + b"\0" * 64)
+            shape = tuple(reversed(info["dims"]))
+            n = int(np.prod(shape))
+            if out is None:
+                out = torch.empty(shape, dtype=torch.float32, device=archive.device)
+            self.decompress_into(p, archive.data_ptr(), archive.numel(), out.data_ptr(), out.numel())
+            return out.view(shape) if out.numel() == n else out
+        b = bytes(archive)
+        info = archive_info(b)
+        shape = tuple(reversed(info["dims"]))
+        n = int(np.prod(shape))
+        y = np.empty(n, dtype=np.float32)
+        rc = self._lib.vecsz_decompress_host(self._ctx, ctypes.byref(p), b, len(b), y.ctypes.data_as(ctypes.c_void_p), n)
+        self._check(rc)
+        return y.reshape(shape)
+
+
+# ---------------------------------------------------------------- reference-shaped API
+
+@dataclass
+class Dim4:
+    _0: int = 1
+    _1: int = 1
+    _2: int = 1
+    _3: int = 1
+
+
+@dataclass
+class ArgParse:
+    """The ArgParse fields the path reads (src/argument_parser/argparse.hh:45-98),
+    with the reference's defaults (:62-77).  `value_range` stands in for the
+    reference's re-read of the input file in r2r mode."""
+    ndim: int = -1
+    dim4: Dim4 = field(default_factory=Dim4)
+    eb: float = 1e-4
+    mode: str = "abs"
+    dict_size: int = 4096
+    radius: int = 2048
+    block_size: int = 8
+    value_range: float = 0.0
+    device: int = 0
+
+
+_default_ctx: Optional[Context] = None
+
+
+def _ctx(ap: ArgParse) -> Context:
+    global _default_ctx
+    if _default_ctx is None or _default_ctx.device != ap.device:
+        _default_ctx = Context(ap.device)
+    return _default_ctx
+
+
+def Compress(ap: ArgParse, data_in):
+    """vecsz::interface::Compress<float,int> (src/vecsz_interface.hh:31-229) without
+    the lossless pass.  Returns (archive, num_outlier, data_size, lossless_size=0)."""
+    dims = [ap.dim4._0, ap.dim4._1, ap.dim4._2][: ap.ndim]
+    if torch is not None and isinstance(data_in, torch.Tensor):
+        x = data_in.reshape(tuple(reversed(dims)))
+    else:
+        x = np.asarray(data_in, dtype=np.float32).reshape(tuple(reversed(dims)))
+    arch, nout = _ctx(ap).compress(x, ap.eb, ap.mode, ap.block_size, ap.dict_size, ap.value_range)
+    size = arch.numel() if hasattr(arch, "numel") else len(arch)
+    return arch, nout, size, 0
+
+
+def Decompress(ap: ArgParse, tmp_data, target_size: int = 0, lossless_size: int = 0):
+    """vecsz::interface::Decompress<float,int> (src/vecsz_interface.hh:232-375).
+    Sets ap.ndim / ap.dim4 / ap.eb from the archive like the reference does
+    (:262-268) and returns the reconstructed field."""
+    if lossless_size:
+        raise NotImplementedError("the zstd/gzip pass is not part of this library")
+    y = _ctx(ap).decompress(tmp_data, ap.mode, ap.value_range, ap.block_size, ap.dict_size)
+    dims = tuple(reversed(tuple(y.shape)))
+    ap.ndim = len(dims)
+    ap.dim4 = Dim4(*(list(dims) + [1] * (4 - len(dims))))
+    if torch is not None and isinstance(tmp_data, torch.Tensor):
+        head = tmp_data[:64].cpu().numpy().tobytes() + b"\0" * 64
+    else:
+        head = bytes(tmp_data)
+    ap.eb = archive_info(head)["eb"]
+    return y
