@@ -1,0 +1,147 @@
+// vecsz_interface.hh -- drop-in replacement of the reference's header-only
+// interface (src/vecsz_interface.hh) that runs the path on an MI355X through
+// the C ABI of libvecsz_hip.so (include/vecsz_hip.h).
+//
+// Same names, template parameters and argument meaning as the reference:
+//   void* vecsz::interface::Compress<T,Q>(argparse* ap, T* data_in, size_t* num_outlier,
+//                                         size_t* data_size, unsigned long* lossless_size)
+//   void* vecsz::interface::Decompress<T,Q>(argparse* ap, unsigned char* tmp_data,
+//                                           size_t target_size, unsigned long lossless_size)
+// The argparse type is a template parameter so this header does not depend on
+// the reference's own headers; the reference's ArgParse binds to it unchanged.
+//
+// Contract kept from the reference (src/vecsz_interface.hh):
+//  - Compress returns a `new unsigned char[*data_size]` archive payload
+//    (datapack::pack layout), increments *num_outlier by the outlier count,
+//    and delete[]s data_in (src/vecsz_interface.hh:222).
+//  - Decompress sets ap->ndim, ap->dim4 and ap->eb from the archive
+//    (:262-268) and returns `new T[len]`.
+//  - r2r: Compress takes the value range from data_in itself (the reference
+//    re-reads ap->files.input_file, which holds the same floats, :49-50);
+//    Decompress re-reads ap->files.input_file like the reference (:280-284).
+// Differences: errors throw std::runtime_error instead of exit(1); the
+// zstd/gzip pass (ap->szwf.lossless_pass) is not performed here and makes
+// Compress throw; only T = float is supported (the only type the reference
+// instantiates, src/vecsz.cc:37).
+#ifndef VECSZ_AMD_INTERFACE_HH
+#define VECSZ_AMD_INTERFACE_HH
+
+#include <cstdint>
+#include <cstring>
+#include <fstream>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "vecsz_hip.h"
+
+namespace vecsz {
+namespace interface {
+
+namespace detail {
+
+inline vecsz_ctx* context() {
+    static vecsz_ctx* ctx = nullptr;
+    if (!ctx) {
+        int rc = vecsz_ctx_create(&ctx, 0, nullptr);
+        if (rc != VECSZ_OK) throw std::runtime_error(std::string("vecsz: ") + vecsz_strerror(rc));
+    }
+    return ctx;
+}
+
+inline void check(int rc) {
+    if (rc != VECSZ_OK)
+        throw std::runtime_error(std::string("vecsz: ") + vecsz_strerror(rc) + ": " +
+                                 vecsz_ctx_last_error(context()));
+}
+
+// GetDatumValueRange<float> (src/types.cc:27-35) on memory.
+inline double value_range(const float* d, size_t n) {
+    float mx = d[0], mn = d[0];
+    for (size_t i = 1; i < n; i++) {
+        if (mx < d[i]) mx = d[i];
+        if (d[i] < mn) mn = d[i];
+    }
+    float r = mx - mn;
+    return (double)r;
+}
+
+inline double value_range_of_file(const std::string& path, size_t n) {
+    std::vector<float> buf(n);
+    std::ifstream ifs(path.c_str(), std::ios::binary | std::ios::in);
+    if (!ifs.is_open()) throw std::runtime_error("vecsz: cannot open " + path);
+    ifs.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(n * sizeof(float)));
+    return value_range(buf.data(), n);
+}
+
+template <typename AP>
+vecsz_params params_of(const AP* ap) {
+    vecsz_params p;
+    vecsz_params_default(&p);
+    p.ndim = ap->ndim;
+    p.dims[0] = (uint64_t)ap->dim4._0;
+    p.dims[1] = (uint64_t)ap->dim4._1;
+    p.dims[2] = (uint64_t)ap->dim4._2;
+    p.mode = ap->mode == "r2r" ? VECSZ_MODE_R2R : VECSZ_MODE_ABS;
+    p.eb = ap->eb;
+    p.block = ap->block_size;
+    p.dict_size = ap->dict_size;
+    return p;
+}
+
+}  // namespace detail
+
+template <typename T, typename Q, typename AP>
+void* Compress(AP* ap, T* data_in, size_t* num_outlier, size_t* data_size, unsigned long* lossless_size) {
+    static_assert(std::is_same<T, float>::value, "vecSZ's path is float32");
+    if (ap->szwf.lossless_pass) throw std::runtime_error("vecsz: the zstd/gzip pass is not part of this library");
+    vecsz_params p = detail::params_of(ap);
+    size_t len = 1;
+    for (int i = 0; i < p.ndim; i++) len *= (size_t)p.dims[i];
+    if (p.mode == VECSZ_MODE_R2R) p.value_range = detail::value_range(data_in, len);
+    uint64_t bound = 0;
+    detail::check(vecsz_compress_bound(&p, &bound));
+    std::vector<uint8_t> tmp(bound);
+    uint64_t size = 0, nout = 0;
+    detail::check(vecsz_compress_host(detail::context(), &p, data_in, tmp.data(), bound, &size, &nout));
+    unsigned char* out = new unsigned char[size];
+    std::memcpy(out, tmp.data(), size);
+    *num_outlier += (size_t)nout;
+    *data_size = (size_t)size;
+    if (lossless_size) *lossless_size = 0;
+    delete[] data_in;
+    return out;
+}
+
+template <typename T, typename Q, typename AP>
+void* Decompress(AP* ap, unsigned char* tmp_data, size_t target_size, unsigned long lossless_size) {
+    static_assert(std::is_same<T, float>::value, "vecSZ's path is float32");
+    if (lossless_size) throw std::runtime_error("vecsz: the zstd/gzip pass is not part of this library");
+    int ndim = 0;
+    uint64_t dims[3];
+    double eb = 0;
+    uint64_t huff = 0, nout = 0;
+    detail::check(vecsz_archive_info(tmp_data, target_size, &ndim, dims, &eb, &huff, &nout));
+    ap->ndim = ndim;
+    ap->dim4._0 = (int)dims[0];
+    ap->dim4._1 = ndim >= 2 ? (int)dims[1] : 1;
+    ap->dim4._2 = ndim >= 3 ? (int)dims[2] : 1;
+    ap->dim4._3 = 1;
+    ap->eb = eb;
+    vecsz_params p = detail::params_of(ap);
+    const size_t len = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
+    if (p.mode == VECSZ_MODE_R2R) p.value_range = detail::value_range_of_file(ap->files.input_file, len);
+    T* xdata = new T[len];
+    int rc = vecsz_decompress_host(detail::context(), &p, tmp_data, target_size, xdata, len);
+    if (rc != VECSZ_OK) {
+        delete[] xdata;
+        detail::check(rc);
+    }
+    return xdata;
+}
+
+}  // namespace interface
+}  // namespace vecsz
+
+#endif

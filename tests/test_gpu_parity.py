@@ -89,6 +89,8 @@ def test_shapes_vs_oracle(gpu_ctx, shape):
     rng = np.random.default_rng(sum(shape))
     x = np.cumsum(rng.normal(0, 0.05, int(np.prod(shape)))).astype(np.float32).reshape(shape)
     for eb, r2r in ((1e-3, False), (1e-4, True)):
+        if r2r and x.size == 1:
+            continue  # zero value range: the reference exits, this library returns VECSZ_ERANGE0
         _check_vs_oracle(gpu_ctx, x, eb, r2r, tag=f"walk{shape}")
 
 
@@ -122,6 +124,17 @@ def test_special_values_vs_oracle(gpu_ctx):
     _check_vs_oracle(gpu_ctx, big, 0.37, False, tag="beyond 2^24")
     _check_vs_oracle(gpu_ctx, big[0], 0.37, False, tag="beyond 2^24 2d")
     _check_vs_oracle(gpu_ctx, big.ravel()[:5000].copy(), 0.37, False, tag="beyond 2^24 1d")
+
+
+@pytest.mark.parametrize("shape", [(300,), (20, 37), (9, 10, 21)])
+def test_negative_zero_outliers(gpu_ctx, shape):
+    """-0.0 right after a jump of >= R quanta is an outlier whose value is
+    -0.0; the reference reconstructs it as -0.0."""
+    x = np.zeros(shape, np.float32).ravel()
+    x[5::7] = 5000.0
+    x[6::7] = -0.0
+    x = x.reshape(shape)
+    _check_vs_oracle(gpu_ctx, x, 0.5, False, tag=f"-0 outliers {shape}")
 
 
 def test_c1_sinusoids_full_size(gpu_ctx):

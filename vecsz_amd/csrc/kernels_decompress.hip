@@ -337,6 +337,10 @@ constexpr int32_t kExact = 1 << 21;
 
 __device__ __forceinline__ float scale_out(int32_t s, double twoeb) { return (float)((double)(float)s * twoeb); }
 
+// An outlier's s is its stored value itself, sign of zero included (a
+// non-outlier s is never -0.0: pred + (float)(code - R) with a +0.0 addend).
+__device__ __forceinline__ float scale_outlier(float v, double twoeb) { return (float)((double)v * twoeb); }
+
 // Inclusive segmented scan over segments of B lanes: a lane with f set starts
 // a new run.
 template <int B>
@@ -409,8 +413,9 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
                     const int32_t Z = zz > 0 ? prevplane[yy] : 0;
                     const int32_t YZ = (yy > 0 && zz > 0) ? zrow : 0;
                     int32_t g0;
+                    float v = 0.f;
                     if (isout) {
-                        const float v = sparse[idx];
+                        v = sparse[idx];
                         if (!exact_outlier(v)) bad = true;
                         g0 = (int32_t)v - Y - Z + YZ;
                     } else {
@@ -419,7 +424,7 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
                     const int32_t G = seg_scan<B>(g0, isout || bstart);
                     const int32_t sv = G + Y + Z - YZ;
                     if (sv >= kExact || sv <= -kExact) bad = true;
-                    if (xin) out[idx] = scale_out(sv, a.twoeb);
+                    if (xin) out[idx] = isout ? scale_outlier(v, a.twoeb) : scale_out(sv, a.twoeb);
                     zrow = Z;
                     prevplane[yy] = sv;
                     srow = sv;
@@ -457,8 +462,9 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
                 const bool isout = xin && c == 0;
                 const int32_t N = yy > 0 ? srow : 0;
                 int32_t g0;
+                float v = 0.f;
                 if (isout) {
-                    const float v = sparse[idx];
+                    v = sparse[idx];
                     if (!exact_outlier(v)) bad = true;
                     g0 = (int32_t)v - N;
                 } else {
@@ -467,7 +473,7 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
                 const int32_t G = seg_scan<B>(g0, isout || bstart);
                 const int32_t sv = G + N;
                 if (sv >= kExact || sv <= -kExact) bad = true;
-                if (xin) out[idx] = scale_out(sv, a.twoeb);
+                if (xin) out[idx] = isout ? scale_outlier(v, a.twoeb) : scale_out(sv, a.twoeb);
                 srow = sv;
             }
         }
@@ -492,8 +498,9 @@ __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ c
         const uint32_t c = xin ? codes[x] : (uint32_t)a.radius;
         const bool isout = xin && c == 0;
         int32_t g0;
+        float v = 0.f;
         if (isout) {
-            const float v = sparse[x];
+            v = sparse[x];
             if (!exact_outlier(v)) bad = true;
             g0 = (int32_t)v;
         } else {
@@ -501,7 +508,7 @@ __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ c
         }
         const int32_t sv = seg_scan<B>(g0, isout || bstart);
         if (sv >= kExact || sv <= -kExact) bad = true;
-        if (xin) out[x] = scale_out(sv, a.twoeb);
+        if (xin) out[x] = isout ? scale_outlier(v, a.twoeb) : scale_out(sv, a.twoeb);
         flag_blocks(bad, B, t * (64 / B), 0, 0, nbx, 1, flagged, nflagged);
     }
 }

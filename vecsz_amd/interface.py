@@ -75,6 +75,19 @@ def archive_info(archive) -> dict:
             "huff_size": hs.value, "n_outlier": no.value}
 
 
+def header_fields(head: bytes) -> dict:
+    """ndim, dims (x first) and eb from the first bytes of an archive payload
+    (datapack layout, src/utils/datapack.hh:33-47); no section-size checks."""
+    nd = int(np.frombuffer(head[:4], dtype=np.int32)[0])
+    if not 1 <= nd <= 3 or len(head) < 4 * (nd + 1) + 8:
+        raise VecszError(_lib.VECSZ_ECORRUPT, "archive header is corrupt")
+    dims = tuple(int(v) for v in np.frombuffer(head[4:4 + 4 * nd], dtype=np.int32))
+    if min(dims) <= 0:
+        raise VecszError(_lib.VECSZ_ECORRUPT, "archive header is corrupt")
+    eb = float(np.frombuffer(head[4 * (nd + 1):4 * (nd + 1) + 8], dtype=np.float64)[0])
+    return {"ndim": nd, "dims": dims, "eb": eb}
+
+
 class Context:
     """One GPU, one HIP stream, reusable device scratch."""
 
@@ -178,7 +191,7 @@ class Context:
             if not archive.is_cuda or archive.dtype != torch.uint8:
                 raise TypeError("expected a uint8 CUDA tensor")
             head = archive[:64].cpu().numpy().tobytes()
-            info = archive_info(head + b"\0" * 64)
+            info = header_fields(head)
             shape = tuple(reversed(info["dims"]))
             n = int(np.prod(shape))
             if out is None:
@@ -255,8 +268,8 @@ def Decompress(ap: ArgParse, tmp_data, target_size: int = 0, lossless_size: int 
     ap.ndim = len(dims)
     ap.dim4 = Dim4(*(list(dims) + [1] * (4 - len(dims))))
     if torch is not None and isinstance(tmp_data, torch.Tensor):
-        head = tmp_data[:64].cpu().numpy().tobytes() + b"\0" * 64
+        head = tmp_data[:64].cpu().numpy().tobytes()
     else:
-        head = bytes(tmp_data)
-    ap.eb = archive_info(head)["eb"]
+        head = bytes(tmp_data[:64])
+    ap.eb = header_fields(head)["eb"]
     return y
