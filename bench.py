@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--dict", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain device copy of the field (known bytes) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one field shards by slabs; "
@@ -120,6 +122,10 @@ def main():
 
     def decomp():
         ctx.decompress_into(pd, arch.data_ptr(), size, y.data_ptr(), n)
+
+    if args.calib_copy:
+        y.copy_(x)  # reads and writes exactly 4*N bytes: the yardstick for FETCH_SIZE / WRITE_SIZE
+        torch.cuda.synchronize()
 
     # warmup
     for _ in range(args.warmup):

@@ -3,13 +3,14 @@
 // Compress (vecsz::interface::Compress, src/vecsz_interface.hh:31-229):
 //   [r2r] k_range -> eb family on the host (src/types.cc:84-93)
 //   k_dualquant -> histogram to host -> codebook + tree bytes (codebook.cpp)
-//   header + tree H2D -> k_chunk_bits -> k_scan_chunks -> k_pack
+//   header + tree H2D -> k_chunk_bits -> chunk scan -> k_enc_summary -> k_pack
 // Decompress (vecsz::interface::Decompress, src/vecsz_interface.hh:232-375):
 //   header + tree D2H -> decode tables -> k_decode_sync -> k_decode_check
 //   [-> k_decode_redo rounds] -> scan -> k_decode_write -> k_recon_* [-> fallback]
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -90,13 +91,14 @@ struct vecsz_ctx {
     bool own_stream = false;
     std::string last_error;
     // compress scratch
-    DevBuf codes, sparse, hist, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, cw_hi, cw_lo, cw_len;
+    DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
+    Codebook cb;
     // decompress scratch
     DevBuf lut, child, sub_start, sub_end, sub_end2, sub_cnt, sub_zeros, sub_bnd, sub_zmask, sub_nbnd, sub_dirty, sym_off,
         zero_off, scan_tmp, scan_tot, dec_sum, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
-    HostBuf pin;
+    HostBuf pin, pin2;
     std::vector<uint64_t> hist_host;
     // timing
     bool timing = false;
@@ -105,6 +107,8 @@ struct vecsz_ctx {
     size_t event_next = 0;
     std::vector<std::string> tnames;
     std::vector<double> tms;
+    std::vector<std::string> host_names;
+    std::vector<double> host_ms;
 };
 
 namespace {
@@ -154,6 +158,8 @@ struct StageScope {
 void timing_begin(vecsz_ctx* c) {
     c->stages.clear();
     c->event_next = 0;
+    c->host_names.clear();
+    c->host_ms.clear();
 }
 
 void timing_end(vecsz_ctx* c) {
@@ -166,6 +172,10 @@ void timing_end(vecsz_ctx* c) {
         (void)hipEventElapsedTime(&ms, s.a, s.b);
         c->tnames.push_back(s.name);
         c->tms.push_back(ms);
+    }
+    for (size_t i = 0; i < c->host_names.size(); i++) {
+        c->tnames.push_back(c->host_names[i]);
+        c->tms.push_back(c->host_ms[i]);
     }
 }
 
@@ -217,48 +227,25 @@ EbFamily eb_family(double eb, bool r2r, double range) {
     return f;
 }
 
-float ord_to_float(uint32_t u) {
-    uint32_t b = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
-    float f;
-    memcpy(&f, &b, 4);
-    return f;
-}
-
-// GetDatumValueRange semantics on a device field.
-int device_value_range(vecsz_ctx* c, const float* d_in, uint64_t n, double* range) {
-    if (!c->range.ensure(16)) return fail(c, VECSZ_ENOMEM, "range buffer");
-    uint32_t init[2] = {0xFFFFFFFFu, 0u};
-    HIPCHK(c, hipMemcpyAsync(c->range.p, init, 8, hipMemcpyHostToDevice, c->stream));
-    {
-        StageScope s(c, "range");
-        launch_range(d_in, n, c->range.as<uint32_t>(), c->stream);
-    }
-    HIPCHK(c, hipGetLastError());
-    if (!c->pin.ensure(64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
-    uint32_t* h = c->pin.as<uint32_t>();
-    HIPCHK(c, hipMemcpyAsync(h, c->range.p, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(h + 2, d_in, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    float x0;
-    memcpy(&x0, h + 2, 4);
-    float mx, mn;
-    if (x0 != x0) {
-        mx = mn = x0;  // std::max_element keeps a leading NaN
-    } else {
-        mn = ord_to_float(h[0]);
-        mx = ord_to_float(h[1]);
-    }
-    float r = mx - mn;
-    *range = (double)r;
-    return VECSZ_OK;
-}
-
 void put_u32be(uint8_t* b, uint32_t v) {
     b[0] = (uint8_t)(v >> 24);
     b[1] = (uint8_t)(v >> 16);
     b[2] = (uint8_t)(v >> 8);
     b[3] = (uint8_t)v;
 }
+
+struct HostTimer {
+    vecsz_ctx* c;
+    const char* name;
+    std::chrono::steady_clock::time_point t0;
+    HostTimer(vecsz_ctx* ctx, const char* n) : c(ctx), name(n), t0(std::chrono::steady_clock::now()) {}
+    ~HostTimer() {
+        if (!c->timing) return;
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        c->host_names.push_back(name);
+        c->host_ms.push_back(ms);
+    }
+};
 
 int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
                     uint64_t* out_bytes, uint64_t* n_outlier) {
@@ -271,27 +258,17 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint64_t n = d0 * d1 * d2;
     const int dict = p->dict_size, R = dict / 2, B = p->block;
     const bool r2r = p->mode == VECSZ_MODE_R2R;
-
-    double range = 0;
-    if (r2r) {
-        if (p->value_range > 0) {
-            range = p->value_range;
-        } else {
-            rc = device_value_range(c, d_in, n, &range);
-            if (rc) return rc;
-        }
-        if (range == 0) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
-    }
-    const EbFamily eb = eb_family(p->eb, r2r, range);
+    const bool device_range = r2r && !(p->value_range > 0);
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
 
     if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
-        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16))
+        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) || !c->range.ensure(16) ||
+        !c->ebdev.ensure(sizeof(EbDev)) || !c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
+        !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
+        !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
+        !c->book.ensure((size_t)dict * 9 + 64))
         return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-    if (!c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) || !c->bit_off.ensure(nchunks * 8) ||
-        !c->out_off.ensure(nchunks * 8) || !c->cw_hi.ensure((size_t)dict * 4) || !c->cw_lo.ensure((size_t)dict * 4) ||
-        !c->cw_len.ensure((size_t)dict))
-        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    if (!c->pin.ensure((size_t)dict * 8 + 4096)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
 
     DQArgs a;
     a.d0 = d0;
@@ -299,8 +276,6 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     a.d2 = d2;
     a.n = n;
     a.x8 = (d0 / 8) * 8;
-    a.r_f = eb.r_f;
-    a.r_d = eb.r_d;
     a.radius = R;
     a.radius_f = (float)R;
     a.thr_vec = (float)vec_threshold(R);
@@ -309,29 +284,58 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     a.ntiles_x = (d0 + 63) / 64;
     a.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
     a.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
-    uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : a.ntiles_x * a.ntiles_y * a.ntiles_z;
-    int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 1280));
+    a.reverse = device_range ? 1 : 0;  // the range pass just streamed the field forward
+    a.ebd = nullptr;
+    EbFamily eb{};
+    if (device_range) {
+        // min/max and the eb family stay on the device: no stop before the dual quant
+        StageScope s(c, "range");
+        launch_range(d_in, n, c->range.as<uint32_t>(), c->stream);
+        launch_ebfam(c->range.as<uint32_t>(), d_in, p->eb, c->ebdev.as<EbDev>(), c->stream);
+        a.ebd = c->ebdev.as<EbDev>();
+        a.r_f = 0;
+        a.r_d = 0;
+    } else {
+        const double range = r2r ? p->value_range : 0.0;
+        eb = eb_family(p->eb, r2r, range);
+        a.r_f = eb.r_f;
+        a.r_d = eb.r_d;
+    }
+    const bool x4 = dualquant_use_x4(ndim, B, d_in, d0);
+    const uint64_t tiles_x = x4 ? (d0 + 255) / 256 : a.ntiles_x;
+    const uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : tiles_x * a.ntiles_y * a.ntiles_z;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 768));
+    if (!c->partials.ensure((size_t)grid * dict * 4)) return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
 
     HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)dict * 8, c->stream));
     {
         StageScope s(c, "dualquant");
         launch_dualquant(ndim, B, d_in, c->codes.as<uint16_t>(), c->sparse.as<float>(),
-                         c->hist.as<unsigned long long>(), a, grid, c->stream);
+                         c->hist.as<unsigned long long>(), c->partials.as<uint32_t>(), a, grid, c->stream);
     }
     HIPCHK(c, hipGetLastError());
-    c->hist_host.resize(dict);
-    if (!c->pin.ensure((size_t)dict * 8 + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
-    HIPCHK(c, hipMemcpyAsync(c->pin.p, c->hist.p, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
+    uint8_t* hp = c->pin.as<uint8_t>();
+    HIPCHK(c, hipMemcpyAsync(hp, c->hist.p, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
+    if (device_range)
+        HIPCHK(c, hipMemcpyAsync(hp + (size_t)dict * 8, c->ebdev.p, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    memcpy(c->hist_host.data(), c->pin.p, (size_t)dict * 8);
+    if (device_range) {
+        EbDev e;
+        memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
+        if (e.zero_range) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
+    }
 
-    Codebook cb;
-    rc = build_codebook(c->hist_host.data(), dict, cb);
+    Codebook& cb = c->cb;
+    {
+        HostTimer ht(c, "host_codebook");
+        rc = build_codebook(reinterpret_cast<const uint64_t*>(hp), dict, cb);
+    }
     if (rc == VECSZ_ECODELEN) return fail(c, rc, "Huffman code longer than 64 bits");
     if (rc) return fail(c, rc, "codebook construction failed");
 
-    // Header: datapack metadata (huff_size / n_outlier filled by k_scan_chunks),
-    // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and tree.
+    // Header: datapack metadata (huff_size / n_outlier filled by k_enc_summary),
+    // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and
+    // tree; then the codebook tables.  Both staged in pinned memory.
     const uint64_t mb = meta_bytes(ndim);
     const uint64_t tb = cb.tree.size();
     const uint64_t head = mb + 8 + tb;
@@ -339,8 +343,11 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         *out_bytes = head;
         return fail(c, VECSZ_ENOMEM, "output buffer too small");
     }
-    std::vector<uint8_t> hdr(head, 0);
+    const size_t book_bytes = (size_t)dict * 9;
+    if (!c->pin.ensure(head + book_bytes + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    uint8_t* hdr = c->pin.as<uint8_t>();
     {
+        memset(hdr, 0, mb);
         int32_t nd = ndim;
         memcpy(&hdr[0], &nd, 4);
         for (int i = 0; i < ndim; i++) {
@@ -348,36 +355,42 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
             memcpy(&hdr[4 + 4 * i], &d, 4);
         }
         memcpy(&hdr[4 * (ndim + 1)], &p->eb, 8);
-        // pad_len = 0; huff_size and n_outlier are written on the device
         put_u32be(&hdr[mb], (uint32_t)cb.node_count);
         put_u32be(&hdr[mb + 4], (uint32_t)dict);  // stateNum / 2
         memcpy(&hdr[mb + 8], cb.tree.data(), tb);
+        uint8_t* bk = hdr + head;
+        memcpy(bk, cb.cw_hi.data(), (size_t)dict * 4);
+        memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
+        memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
     }
-    HIPCHK(c, hipMemcpyAsync(d_out, hdr.data(), head, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->cw_hi.p, cb.cw_hi.data(), (size_t)dict * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->cw_lo.p, cb.cw_lo.data(), (size_t)dict * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->cw_len.p, cb.len.data(), (size_t)dict, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->book.p, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
+    const uint32_t* cw_hi = c->book.as<uint32_t>();
+    const uint32_t* cw_lo = cw_hi + dict;
+    const uint8_t* cw_len = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
     HIPCHK(c, hipMemsetAsync(c->err.p, 0, 16, c->stream));
     {
         StageScope s(c, "chunk_bits");
-        launch_chunk_bits(c->codes.as<uint16_t>(), n, c->cw_len.as<uint8_t>(), dict, c->chunk_bits.as<uint32_t>(),
+        launch_chunk_bits(c->codes.as<uint16_t>(), n, cw_len, dict, c->chunk_bits.as<uint32_t>(),
                           c->chunk_zeros.as<uint32_t>(), nchunks, c->stream);
     }
     {
         StageScope s(c, "scan");
-        launch_scan_chunks(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), nchunks,
-                           c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb,
-                           tb, out_cap, c->err.as<uint32_t>(), c->stream);
+        launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), nchunks,
+                          c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
+                          c->scan_tot.as<uint64_t>(), c->stream);
+        launch_enc_summary(c->scan_tot.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb, tb, out_cap,
+                           c->err.as<uint32_t>(), c->stream);
     }
     {
         StageScope s(c, "pack");
-        launch_pack(c->codes.as<uint16_t>(), n, c->cw_hi.as<uint32_t>(), c->cw_lo.as<uint32_t>(),
-                    c->cw_len.as<uint8_t>(), dict, c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), nchunks,
-                    c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out, c->err.as<uint32_t>(),
-                    std::max(cb.max_len, 1), c->stream);
+        launch_pack(c->codes.as<uint16_t>(), n, cw_hi, cw_lo, cw_len, dict, c->bit_off.as<uint64_t>(),
+                    c->out_off.as<uint64_t>(), nchunks, c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out,
+                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->stream);
     }
     HIPCHK(c, hipGetLastError());
-    uint8_t* h = c->pin.as<uint8_t>();
+    if (!c->pin2.ensure(256)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    uint8_t* h = c->pin2.as<uint8_t>();
     HIPCHK(c, hipMemcpyAsync(h, c->summary.p, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(h + 128, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

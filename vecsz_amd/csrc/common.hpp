@@ -8,12 +8,23 @@
 namespace vsz {
 
 constexpr int kWave = 64;          // CDNA wavefront width
-constexpr int kChunk = 4096;       // symbols per encode chunk (one workgroup)
-constexpr int kEncThreads = 256;   // encode workgroup size
-constexpr int kSymPerThread = kChunk / kEncThreads;  // 16
+constexpr int kChunk = 1024;       // symbols per encode chunk (one wavefront)
+constexpr int kSymPerThread = kChunk / kWave;  // 16 consecutive symbols per lane
 constexpr int kLutBits = 12;       // first-level decode table width
 constexpr int kSubBits = 1024;     // bits per decode subsequence
 constexpr int kSyncRecord = 16;    // codeword boundaries recorded per subsequence
+
+// Error-bound family computed on the device from the value range
+// (ChangeToRelativeMode + InitializeErrorBoundFamily, src/types.cc:49-60,
+// :84-93), so the compress pipeline does not stop for the range.
+struct EbDev {
+    double range;     // (double)(float max - float min)
+    double eb_final;
+    double r_d;       // 1 / (2 eb_final)
+    double twoeb;     // 2 eb_final
+    float r_f;        // (float) r_d
+    uint32_t zero_range;
+};
 
 // Scalars of one dual-quantization launch.  Names follow the reference's
 // dims_L16 / ebs_L4 (src/constants.hh:14-31).
@@ -28,6 +39,8 @@ struct DQArgs {
     float thr_scalar;      // |e| < thr_scalar (= radius)
     int radius;
     int dict;              // histogram bins
+    int reverse;           // visit tiles last-to-first (reuses what a preceding pass left in cache)
+    const EbDev* ebd;      // when set, r_f / r_d come from here (r2r range computed on the device)
     uint64_t ntiles_x, ntiles_y, ntiles_z;
 };
 

@@ -7,16 +7,18 @@ namespace vsz {
 
 constexpr int kMaxLdsHistBins = 8192;   // LDS-private histogram up to this dict size
 constexpr int kMaxLdsLenTab = 16384;    // LDS code-length table up to this dict size
+constexpr int kMaxLdsBook = 4096;       // LDS codebook (k_pack) up to this dict size
 
 int dq_lds_bytes(int dict);
 void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st);
+void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hipStream_t st);
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
-                      const DQArgs& a, int grid, hipStream_t st);
+                      uint32_t* partials, const DQArgs& a, int grid, hipStream_t st);
+bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0);
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
                        uint64_t nchunks, hipStream_t st);
-void launch_scan_chunks(const uint32_t* cb, const uint32_t* cz, uint64_t nchunks, uint64_t* bit_off, uint64_t* out_off,
-                        EncSummary* sum, uint8_t* archive, uint64_t meta_bytes, uint64_t tree_bytes, uint64_t out_cap,
-                        uint32_t* err, hipStream_t st);
+void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
+                        uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st);
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st);

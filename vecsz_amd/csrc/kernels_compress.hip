@@ -5,7 +5,7 @@
 //                  quant-code histogram (c_lorenzo_{1,2,3}d1l, src/dualquant.hh:18-913;
 //                  initHuffman's frequency loop, src/huffman.hh:239-242)
 //   k_chunk_bits   bits and outliers per 4096-symbol chunk (HuffmanEncode sizes, src/huffman.hh:268-341)
-//   k_scan_chunks  exclusive scan of chunk totals + archive sizes (src/vecsz_interface.hh:189-202)
+//   k_enc_summary  archive sizes from the scanned chunk totals (src/vecsz_interface.hh:189-202)
 //   k_pack         MSB-first bit packing into the archive + outlier compaction in index order
 //                  (HuffmanEncode; outlier gather, src/vecsz_interface.hh:166-170)
 //
@@ -85,6 +85,38 @@ __global__ void __launch_bounds__(256) k_range(const float* __restrict__ x, uint
     }
 }
 
+// r2r error-bound family from the device-side min/max.  std::max_element /
+// std::min_element keep a leading NaN (then the range is NaN); otherwise
+// they ignore NaNs, which is what the NaN-skipping reduction found.
+__global__ void k_ebfam(const uint32_t* __restrict__ mm, const float* __restrict__ x, double eb, EbDev* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float x0 = x[0];
+    float mn, mx;
+    if (x0 != x0) {
+        mn = mx = x0;
+    } else {
+        mn = ord2f(mm[0]);
+        mx = ord2f(mm[1]);
+    }
+    const float r = mx - mn;
+    EbDev e;
+    e.range = (double)r;
+    e.zero_range = e.range == 0;
+    e.eb_final = eb * e.range;
+    e.r_d = 1 / (2 * e.eb_final);
+    e.twoeb = 2 * e.eb_final;
+    e.r_f = (float)e.r_d;
+    *out = e;
+}
+
+// Replaces a's r_f / r_d by the device-computed family when there is one.
+__device__ __forceinline__ void dq_resolve(DQArgs& a) {
+    if (a.ebd) {
+        a.r_f = a.ebd->r_f;
+        a.r_d = a.ebd->r_d;
+    }
+}
+
 // ------------------------------------------------------------------ histogram
 
 // LDS histogram: a private copy of every bin plus 64 lane-private copies of
@@ -101,11 +133,13 @@ struct LdsHist {
     unsigned long long* global;
 
     __device__ __forceinline__ void add(int bin) {
-        int h = bin - lo;
-        if ((unsigned)h < (unsigned)kHotBins) {
+        const int h = bin - lo;
+        const bool is_hot = (unsigned)h < (unsigned)kHotBins;
+        if (lds_cold) {
+            // hot and cold bins are contiguous in LDS: one atomic, no branch
+            atomicAdd(&hot[is_hot ? h * 64 + (int)lane_id() : kHotBins * 64 + bin], 1u);
+        } else if (is_hot) {
             atomicAdd(&hot[h * 64 + lane_id()], 1u);
-        } else if (lds_cold) {
-            atomicAdd(&cold[bin], 1u);
         } else {
             atomicAdd(&global[bin], 1ull);
         }
@@ -124,7 +158,10 @@ __device__ void hist_init(LdsHist& H, uint32_t* lds, int dict, int radius, unsig
     __syncthreads();
 }
 
-__device__ void hist_flush(LdsHist& H) {
+// Folds the lane-private hot copies into the bins, then writes this
+// workgroup's histogram densely to partials[blockIdx.x][*] (k_hist_reduce sums
+// them) -- or, for dictionaries too large for LDS, adds to the global bins.
+__device__ void hist_flush(LdsHist& H, uint32_t* __restrict__ partials) {
     __syncthreads();
     for (int h = threadIdx.x; h < kHotBins; h += blockDim.x) {
         uint32_t s = 0;
@@ -139,11 +176,24 @@ __device__ void hist_flush(LdsHist& H) {
     }
     __syncthreads();
     if (H.lds_cold) {
-        for (int b = threadIdx.x; b < H.dict; b += blockDim.x) {
-            uint32_t v = H.cold[b];
-            if (v) atomicAdd(&H.global[b], (unsigned long long)v);
-        }
+        uint32_t* dst = partials + (uint64_t)blockIdx.x * H.dict;
+        for (int b = threadIdx.x; b < H.dict; b += blockDim.x) dst[b] = H.cold[b];
     }
+}
+
+// Sums the per-workgroup histograms: blockIdx.x picks 256 bins, blockIdx.y a
+// group of kHistRows partial rows; one 64-bit atomic per bin and group.
+constexpr int kHistRows = 32;
+
+__global__ void __launch_bounds__(256) k_hist_reduce(const uint32_t* __restrict__ partials, int nparts, int dict,
+                                                     unsigned long long* __restrict__ hist) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= dict) return;
+    const int g0 = blockIdx.y * kHistRows, g1 = min(nparts, g0 + kHistRows);
+    uint32_t s = 0;
+#pragma unroll 8
+    for (int g = g0; g < g1; g++) s += partials[(uint64_t)g * dict + b];
+    if (s) atomicAdd(&hist[b], (unsigned long long)s);
 }
 
 // Postquant of one element given its Lorenzo prediction (src/dualquant.hh:169-187):
@@ -164,16 +214,18 @@ __device__ __forceinline__ uint32_t postquant(float q, float pred, float thr, fl
 template <int B>
 __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      DQArgs a) {
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
     extern __shared__ uint32_t lds[];
     LdsHist H;
     hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
     const uint32_t lane = lane_id();
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
+        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
         const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
@@ -233,7 +285,7 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
             }
         }
     }
-    hist_flush(H);
+    hist_flush(H, partials);
 }
 
 // ------------------------------------------------------------------ 2D
@@ -244,15 +296,17 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
 template <int B>
 __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      DQArgs a) {
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
     extern __shared__ uint32_t lds[];
     LdsHist H;
     hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
     const uint32_t lane = lane_id();
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
+        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = t / a.ntiles_x;
         const uint64_t x = tx * 64 + lane;
@@ -285,7 +339,215 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
             }
         }
     }
-    hist_flush(H);
+    hist_flush(H, partials);
+}
+
+// ------------------------------------------------------------------ 4-wide tiles
+
+// Same per-element arithmetic as the kernels above, with each lane owning 4
+// consecutive x (one float4 load, one 8-byte code store), so a wavefront covers
+// 256 x and moves 1 KiB per row.  Used when DIM0 % 4 == 0 and the field is
+// 16-byte aligned.  Since 8*floor(DIM0/8) is a multiple of 4, the 4 elements of
+// a lane all take the same prequant/postquant form, and only element 0 can be
+// a block start.
+// One 256 x B x B tile.  SCAL is true only for tiles that contain lanes of
+// the partial x-group (x >= 8*floor(DIM0/8)); every other tile runs the AVX
+// forms alone, without evaluating the scalar ones.
+template <int B, bool SCAL>
+__device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                            float* __restrict__ sparse, LdsHist& H, const DQArgs& a, uint64_t x,
+                                            uint64_t y0, uint64_t z0) {
+    const uint64_t sy = a.d0, sz = a.d0 * a.d1;
+    const bool xin = x < a.d0;
+    const bool scal = SCAL && x >= a.x8;
+    const bool xstart = (x % B) == 0;
+    const float thr = scal ? a.thr_scalar : a.thr_vec;
+    const int ny = (int)min((uint64_t)B, a.d1 - y0);
+    const int nz = (int)min((uint64_t)B, a.d2 - z0);
+    float pp[B][4];
+#pragma unroll
+    for (int j = 0; j < B; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) pp[j][k] = 0.f;
+    // rows of the next plane are loaded while the current one is processed
+    float4 vnext[B];
+#pragma unroll
+    for (int yy = 0; yy < B; yy++)
+        vnext[yy] = (xin && yy < ny) ? *reinterpret_cast<const float4*>(in + x + (y0 + yy) * sy + z0 * sz)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int zz = 0; zz < nz; zz++) {
+        const uint64_t z = z0 + zz;
+        float4 vrow[B];
+#pragma unroll
+        for (int yy = 0; yy < B; yy++) vrow[yy] = vnext[yy];
+        if (zz + 1 < nz) {
+#pragma unroll
+            for (int yy = 0; yy < B; yy++)
+                vnext[yy] = (xin && yy < ny)
+                                ? *reinterpret_cast<const float4*>(in + x + (y0 + yy) * sy + (z + 1) * sz)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        float qr[4] = {0.f, 0.f, 0.f, 0.f}, xr[4] = {0.f, 0.f, 0.f, 0.f};
+        float zr[4] = {0.f, 0.f, 0.f, 0.f}, xzr[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int yy = 0; yy < B; yy++) {
+            if (yy < ny) {
+                const uint64_t idx = x + (y0 + yy) * sy + z * sz;
+                const float4 v = vrow[yy];
+                float q[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    q[k] = (SCAL && scal) ? prequant_scalar(q[k], a.r_d) : prequant_vec(q[k], a.r_f);
+                float Z[4], X[4], XZ[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) Z[k] = zz > 0 ? pp[yy][k] : 0.f;
+                X[0] = wave_shr1(q[3]);
+                XZ[0] = wave_shr1(Z[3]);
+                if (xstart) {
+                    X[0] = 0.f;
+                    XZ[0] = 0.f;
+                }
+#pragma unroll
+                for (int k = 1; k < 4; k++) {
+                    X[k] = q[k - 1];
+                    XZ[k] = Z[k - 1];
+                }
+                uint32_t c[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float Y = yy > 0 ? qr[k] : 0.f;
+                    const float XY = yy > 0 ? xr[k] : 0.f;
+                    const float YZ = yy > 0 ? zr[k] : 0.f;
+                    const float XYZ = yy > 0 ? xzr[k] : 0.f;
+                    float pred;
+                    if (SCAL && scal) {
+                        pred = XYZ - XY - XZ[k] - YZ + X[k] + Y + Z[k];
+                    } else {
+                        const float dist1 = X[k] + (Y + Z[k]);
+                        const float dist2 = XY + (XZ[k] + YZ);
+                        pred = (XYZ - dist2) + dist1;
+                    }
+                    c[k] = postquant(q[k], pred, thr, a.radius_f);
+                }
+                if (xin) {
+                    ushort4 cv;
+                    cv.x = (uint16_t)c[0];
+                    cv.y = (uint16_t)c[1];
+                    cv.z = (uint16_t)c[2];
+                    cv.w = (uint16_t)c[3];
+                    *reinterpret_cast<ushort4*>(codes + idx) = cv;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (c[k] == 0) sparse[idx + k] = q[k];
+                        H.add((int)c[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    pp[yy][k] = q[k];
+                    qr[k] = q[k];
+                    xr[k] = X[k];
+                    zr[k] = Z[k];
+                    xzr[k] = XZ[k];
+                }
+            }
+        }
+    }
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                         float* __restrict__ sparse,
+                                                         unsigned long long* __restrict__ hist,
+                                                         uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
+        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
+        const uint64_t tx = t % a.ntiles_x;
+        const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
+        const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
+        const uint64_t x = tx * 256 + 4 * lane;
+        // the partial x-group only exists in the last tile column
+        if (tx * 256 + 256 > a.x8 && a.x8 < a.d0)
+            dq3_x4_tile<B, true>(in, codes, sparse, H, a, x, ty * B, tz * B);
+        else
+            dq3_x4_tile<B, false>(in, codes, sparse, H, a, x, ty * B, tz * B);
+    }
+    hist_flush(H, partials);
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                         float* __restrict__ sparse,
+                                                         unsigned long long* __restrict__ hist,
+                                                         uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
+        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
+        const uint64_t tx = t % a.ntiles_x;
+        const uint64_t ty = t / a.ntiles_x;
+        const uint64_t x = tx * 256 + 4 * lane;
+        const uint64_t y0 = ty * B;
+        const bool xin = x < a.d0;
+        const bool lost = x >= a.x8;
+        const bool xstart = (x % B) == 0;
+        float qr[4] = {0.f, 0.f, 0.f, 0.f}, xr[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int yy = 0; yy < B; yy++) {
+            const uint64_t y = y0 + yy;
+            if (y >= a.d1) break;
+            const uint64_t idx = x + y * a.d0;
+            float4 v = xin ? *reinterpret_cast<const float4*>(in + idx) : make_float4(0.f, 0.f, 0.f, 0.f);
+            float q[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) q[k] = prequant_vec(q[k], a.r_f);
+            float W[4];
+            W[0] = wave_shr1(q[3]);
+            if (xstart) W[0] = 0.f;
+#pragma unroll
+            for (int k = 1; k < 4; k++) W[k] = q[k - 1];
+            uint32_t c[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float N = yy > 0 ? qr[k] : 0.f;
+                const float NW = yy > 0 ? xr[k] : 0.f;
+                const float pred = W[k] + (N - NW);
+                c[k] = lost ? 0u : postquant(q[k], pred, a.thr_vec, a.radius_f);
+            }
+            if (xin) {
+                ushort4 cv;
+                cv.x = (uint16_t)c[0];
+                cv.y = (uint16_t)c[1];
+                cv.z = (uint16_t)c[2];
+                cv.w = (uint16_t)c[3];
+                *reinterpret_cast<ushort4*>(codes + idx) = cv;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (c[k] == 0) sparse[idx + k] = lost ? 0.f : q[k];
+                    H.add((int)c[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                qr[k] = q[k];
+                xr[k] = W[k];
+            }
+        }
+    }
+    hist_flush(H, partials);
 }
 
 // ------------------------------------------------------------------ 1D
@@ -298,10 +560,11 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
 template <int B>
 __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      DQArgs a) {
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
     extern __shared__ uint32_t lds[];
     LdsHist H;
     hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
     const uint32_t lane = lane_id();
     const uint64_t ngroups = (a.n + 7) / 8;
     const bool aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
@@ -341,7 +604,7 @@ __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ 
             }
         }
     }
-    hist_flush(H);
+    hist_flush(H, partials);
 }
 
 // ------------------------------------------------------------------ encode
@@ -379,257 +642,255 @@ __device__ __forceinline__ T block_excl_scan(T v, T* total, T* smem /* >= 4 */) 
     return off + inc - v;
 }
 
-__global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__ codes, uint64_t n,
-                                                    const uint8_t* __restrict__ len_tab, int dict,
-                                                    uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ chunk_zeros) {
-    __shared__ uint8_t slen[kMaxLdsLenTab];
-    __shared__ uint32_t sred[8];
-    const bool lds_tab = dict <= kMaxLdsLenTab;
-    if (lds_tab)
-        for (int i = threadIdx.x; i < dict; i += blockDim.x) slen[i] = len_tab[i];
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kChunk + threadIdx.x * kSymPerThread;
-    uint32_t bits = 0, zeros = 0;
+// Bits and outliers of each 1024-symbol chunk; one wavefront per chunk, 16
+// consecutive symbols per lane.  Persistent workgroups load the code-length
+// table once; LDS_TAB fixes its address space at compile time (ds_read, not
+// FLAT).
+__device__ __forceinline__ void load_syms(const uint16_t* __restrict__ codes, uint64_t n, uint64_t base,
+                                          uint32_t* sym) {
     if (base + kSymPerThread <= n) {
         const uint4* p = reinterpret_cast<const uint4*>(codes + base);
-        uint4 u[2] = {p[0], p[1]};
-        const uint32_t* w = reinterpret_cast<const uint32_t*>(u);
+        const uint4 u0 = p[0], u1 = p[1];
+        const uint32_t w[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            uint32_t s0 = w[j] & 0xFFFF, s1 = w[j] >> 16;
-            bits += lds_tab ? slen[s0] : len_tab[s0];
-            bits += lds_tab ? slen[s1] : len_tab[s1];
-            zeros += (s0 == 0) + (s1 == 0);
+            sym[2 * j] = w[j] & 0xFFFF;
+            sym[2 * j + 1] = w[j] >> 16;
         }
     } else {
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) sym[j] = (base + j < n) ? codes[base + j] : 0xFFFFFFFFu;
+    }
+}
+
+template <bool LDS_TAB>
+__global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__ codes, uint64_t n, uint64_t nchunks,
+                                                    const uint8_t* __restrict__ len_tab, int dict,
+                                                    uint32_t* __restrict__ chunk_bits, uint32_t* __restrict__ chunk_zeros) {
+    __shared__ uint8_t slen[LDS_TAB ? kMaxLdsLenTab : 4];
+    if (LDS_TAB)
+        for (int i = threadIdx.x; i < dict; i += blockDim.x) slen[i] = len_tab[i];
+    __syncthreads();
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    for (uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k < nchunks; k += nw) {
+        uint32_t sym[kSymPerThread];
+        load_syms(codes, n, k * kChunk + lane * kSymPerThread, sym);
+        uint32_t bits = 0, zeros = 0;
+#pragma unroll
         for (int j = 0; j < kSymPerThread; j++) {
-            uint64_t i = base + j;
-            if (i < n) {
-                uint32_t s = codes[i];
-                bits += lds_tab ? slen[s] : len_tab[s];
-                zeros += s == 0;
+            const uint32_t sj = sym[j];
+            if (sj != 0xFFFFFFFFu) {
+                bits += LDS_TAB ? slen[sj] : len_tab[sj];
+                zeros += sj == 0;
             }
         }
-    }
-    for (int d = 32; d > 0; d >>= 1) {
-        bits += __shfl_xor(bits, d);
-        zeros += __shfl_xor(zeros, d);
-    }
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 0) {
-        sred[w] = bits;
-        sred[4 + w] = zeros;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        chunk_bits[blockIdx.x] = sred[0] + sred[1] + sred[2] + sred[3];
-        chunk_zeros[blockIdx.x] = sred[4] + sred[5] + sred[6] + sred[7];
-    }
-}
-
-// Single workgroup of 1024 threads: exclusive scan of per-chunk bits and
-// outliers, then the archive sizes of src/vecsz_interface.hh:189-202 and the
-// two length fields of the payload header (src/utils/datapack.hh:288).
-__global__ void __launch_bounds__(1024) k_scan_chunks(const uint32_t* __restrict__ chunk_bits,
-                                                      const uint32_t* __restrict__ chunk_zeros, uint64_t nchunks,
-                                                      uint64_t* __restrict__ bit_off, uint64_t* __restrict__ out_off,
-                                                      EncSummary* __restrict__ sum, uint8_t* __restrict__ archive,
-                                                      uint64_t meta_bytes, uint64_t tree_bytes, uint64_t out_cap,
-                                                      uint32_t* __restrict__ err) {
-    __shared__ uint64_t sb[16], sz[16];
-    const uint64_t per = (nchunks + blockDim.x - 1) / blockDim.x;
-    const uint64_t lo = threadIdx.x * per, hi = min(nchunks, lo + per);
-    uint64_t b = 0, z = 0;
-    for (uint64_t i = lo; i < hi; i++) {
-        b += chunk_bits[i];
-        z += chunk_zeros[i];
-    }
-    uint64_t ib = wave_incl_scan(b), iz = wave_incl_scan(z);
-    const int w = threadIdx.x >> 6;
-    if (lane_id() == 63) {
-        sb[w] = ib;
-        sz[w] = iz;
-    }
-    __syncthreads();
-    uint64_t ob = 0, oz = 0, tb = 0, tz = 0;
-    const int nw = blockDim.x >> 6;
-    for (int i = 0; i < nw; i++) {
-        if (i < w) {
-            ob += sb[i];
-            oz += sz[i];
+        for (int d = 32; d > 0; d >>= 1) {
+            bits += __shfl_xor(bits, d);
+            zeros += __shfl_xor(zeros, d);
         }
-        tb += sb[i];
-        tz += sz[i];
-    }
-    ob += ib - b;
-    oz += iz - z;
-    for (uint64_t i = lo; i < hi; i++) {
-        bit_off[i] = ob;
-        out_off[i] = oz;
-        ob += chunk_bits[i];
-        oz += chunk_zeros[i];
-    }
-    if (threadIdx.x == 0) {
-        const uint64_t enc = (tb + 7) / 8;
-        const uint64_t huff = 8 + tree_bytes + enc;
-        const uint64_t stream_byte = meta_bytes + 8 + tree_bytes;
-        EncSummary s;
-        s.total_bits = tb;
-        s.n_outlier = tz;
-        s.huff_size = huff;
-        s.archive_size = meta_bytes + huff + 4 * tz;
-        s.outlier_byte_off = meta_bytes + huff;
-        s.stream_bit_off = stream_byte * 8;
-        s.stream_end_byte = stream_byte + enc;
-        *sum = s;
-        if (s.archive_size > out_cap) {
-            *err = 1;
-        } else {
-            // huff_size and n_outlier fields: the last two u64 of the metadata
-            uint8_t* f = archive + meta_bytes - 16;
-            for (int i = 0; i < 8; i++) f[i] = (uint8_t)(huff >> (8 * i));
-            for (int i = 0; i < 8; i++) f[8 + i] = (uint8_t)(tz >> (8 * i));
+        if (lane == 0) {
+            chunk_bits[k] = bits;
+            chunk_zeros[k] = zeros;
         }
     }
 }
 
-// OR `l` (1..32) right-aligned bits `v` into the big-endian bit buffer at bit p.
-__device__ __forceinline__ void emit_bits(uint32_t* words, uint64_t p, uint32_t v, uint32_t l, uint64_t wlimit) {
-    const uint64_t w = p >> 5;
-    const uint32_t o = (uint32_t)(p & 31);
-    if (w >= wlimit) return;
-    if (o + l <= 32) {
-        atomicOr(&words[w], v << (32 - o - l));
+// Archive sizes of src/vecsz_interface.hh:189-202 from the scanned chunk
+// totals, and the two length fields of the payload header
+// (src/utils/datapack.hh:288) written in place.
+__global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
+                              uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t tree_bytes,
+                              uint64_t out_cap, uint32_t* __restrict__ err) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t tb = totals[0], tz = totals[1];
+    const uint64_t enc = (tb + 7) / 8;
+    const uint64_t huff = 8 + tree_bytes + enc;
+    const uint64_t stream_byte = meta_bytes + 8 + tree_bytes;
+    EncSummary s;
+    s.total_bits = tb;
+    s.n_outlier = tz;
+    s.huff_size = huff;
+    s.archive_size = meta_bytes + huff + 4 * tz;
+    s.outlier_byte_off = meta_bytes + huff;
+    s.stream_bit_off = stream_byte * 8;
+    s.stream_end_byte = stream_byte + enc;
+    *sum = s;
+    if (s.archive_size > out_cap) {
+        *err = 1;
     } else {
-        const uint32_t spill = o + l - 32;
-        atomicOr(&words[w], v >> spill);
-        if (w + 1 < wlimit) atomicOr(&words[w + 1], v << (32 - spill));
+        uint8_t* f = archive + meta_bytes - 16;
+        for (int i = 0; i < 8; i++) f[i] = (uint8_t)(huff >> (8 * i));
+        for (int i = 0; i < 8; i++) f[8 + i] = (uint8_t)(tz >> (8 * i));
     }
 }
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
 
-// One workgroup per chunk.  A chunk's bits start at absolute archive bit A_k;
-// it writes the 32-bit words whose first bit lies in [A_k, A_k+1), filling the
-// tail of its last word with the first codewords of the next chunk, so no word
-// is written by two workgroups and no atomics touch global memory.
+// Bit packing, one wavefront per 1024-symbol chunk, no barriers or atomics.
+// Lane L holds 16 consecutive symbols; a wavefront scan gives its absolute
+// first bit p_L = A_k + (bits of lanes < L).  Every 32-bit word of the stream
+// is written once, by the lane holding its first bit: that lane appends its
+// codewords to a word-aligned accumulator, stores each completed word, and
+// completes its last word with the leading bits of the codewords that follow
+// its range (read back from the code array).  Words go straight to global
+// memory; each lane's stores are contiguous and the wavefront's cover a
+// contiguous span, which the L2 merges into full lines.
+constexpr uint32_t kLongCode = 63;  // LDS table marker: codeword longer than 26 bits
+
+struct Code {
+    uint32_t lo, hi, len;  // codeword = hi:lo, right-aligned, len bits
+};
+
+template <bool LDS_BOOK>
+__device__ __forceinline__ Code lookup(uint32_t s, const uint32_t* tab, const uint32_t* __restrict__ cw_hi,
+                                       const uint32_t* __restrict__ cw_lo, const uint8_t* __restrict__ len_tab) {
+    Code c;
+    if (LDS_BOOK) {
+        const uint32_t e = tab[s];
+        c.len = e & 63;
+        c.lo = e >> 6;
+        c.hi = 0;
+        if (c.len == kLongCode) {
+            c.len = len_tab[s];
+            c.lo = cw_lo[s];
+            c.hi = cw_hi[s];
+        }
+    } else {
+        c.len = len_tab[s];
+        c.lo = cw_lo[s];
+        c.hi = c.len > 32 ? cw_hi[s] : 0;
+    }
+    return c;
+}
+
+// Appends l (1..32) right-aligned bits v to a right-aligned accumulator of
+// nacc (< 32) pending bits; a completed 32-bit word goes to `sink`.
+template <typename Sink>
+__device__ __forceinline__ void acc_put(uint64_t& acc, int& nacc, uint32_t v, int l, Sink&& sink) {
+    acc = (acc << l) | v;
+    nacc += l;
+    if (nacc >= 32) {
+        nacc -= 32;
+        sink((uint32_t)(acc >> nacc));
+        acc &= (1ull << nacc) - 1;
+    }
+}
+
+template <typename Sink>
+__device__ __forceinline__ void acc_code(uint64_t& acc, int& nacc, const Code& c, Sink&& sink) {
+    if (c.len > 32) {
+        acc_put(acc, nacc, c.hi, (int)c.len - 32, sink);
+        acc_put(acc, nacc, c.lo, 32, sink);
+    } else if (c.len > 0) {
+        acc_put(acc, nacc, c.lo, (int)c.len, sink);
+    }
+}
+
+template <bool LDS_BOOK>
 __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes, uint64_t n,
                                               const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
                                               const uint8_t* __restrict__ len_tab, int dict,
                                               const uint64_t* __restrict__ bit_off, const uint64_t* __restrict__ out_off,
                                               uint64_t nchunks, const EncSummary* __restrict__ sum,
                                               const float* __restrict__ sparse, uint8_t* __restrict__ archive,
-                                              const uint32_t* __restrict__ err, uint32_t lds_words) {
+                                              const uint32_t* __restrict__ err) {
     if (*err) return;
-    extern __shared__ uint32_t words[];
-    __shared__ uint64_t sscan[8];
+    __shared__ uint32_t tab[LDS_BOOK ? kMaxLdsBook : 1];
+    if (LDS_BOOK)
+        for (int i = threadIdx.x; i < dict; i += blockDim.x) {
+            const uint32_t l = len_tab[i];
+            tab[i] = l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
+        }
+    __syncthreads();
     const EncSummary S = *sum;
-    const uint64_t k = blockIdx.x;
     const uint64_t A0 = S.stream_bit_off;
-    const uint64_t Ak = A0 + bit_off[k];
-    const uint64_t An = A0 + (k + 1 < nchunks ? bit_off[k + 1] : S.total_bits);
-    const uint64_t Wb = Ak >> 5;
-    const uint64_t base_rel = Ak & ~31ull;  // bit position of LDS word 0
-    for (uint32_t i = threadIdx.x; i < lds_words; i += blockDim.x) words[i] = 0;
-    __syncthreads();
-
-    // This thread's symbols.
-    const uint64_t base = k * kChunk + threadIdx.x * kSymPerThread;
-    uint16_t sym[kSymPerThread];
-    if (base + kSymPerThread <= n) {
-        const uint4* p = reinterpret_cast<const uint4*>(codes + base);
-        uint4 u[2] = {p[0], p[1]};
-        const uint16_t* s16 = reinterpret_cast<const uint16_t*>(u);
-#pragma unroll
-        for (int j = 0; j < kSymPerThread; j++) sym[j] = s16[j];
-    } else {
-#pragma unroll
-        for (int j = 0; j < kSymPerThread; j++) sym[j] = (base + j < n) ? codes[base + j] : 0xFFFF;
-    }
-    uint32_t tbits = 0, tz = 0;
-#pragma unroll
-    for (int j = 0; j < kSymPerThread; j++) {
-        if (base + j < n) {
-            tbits += len_tab[sym[j]];
-            tz += sym[j] == 0;
-        }
-    }
-    uint64_t wg_bits, wg_z;
-    uint64_t my_off = block_excl_scan<uint64_t>(tbits, &wg_bits, sscan);
-    uint64_t my_z = block_excl_scan<uint64_t>(tz, &wg_z, sscan + 4);
-
-    // Emit this thread's codewords.
-    uint64_t p = (Ak - base_rel) + my_off;
-    uint64_t r = out_off[k] + my_z;
-#pragma unroll
-    for (int j = 0; j < kSymPerThread; j++) {
-        const uint64_t i = base + j;
-        if (i >= n) break;
-        const uint32_t s = sym[j];
-        const uint32_t l = len_tab[s];
-        if (l > 32) {
-            emit_bits(words, p, cw_hi[s], l - 32, lds_words);
-            emit_bits(words, p + (l - 32), cw_lo[s], 32, lds_words);
-        } else if (l > 0) {
-            emit_bits(words, p, cw_lo[s], l, lds_words);
-        }
-        p += l;
-        if (s == 0) {
-            // outlier compaction, index order
-            float v = sparse[i];
-            uint32_t bitsv = __float_as_uint(v);
-            uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
-            dst[0] = (uint8_t)bitsv;
-            dst[1] = (uint8_t)(bitsv >> 8);
-            dst[2] = (uint8_t)(bitsv >> 16);
-            dst[3] = (uint8_t)(bitsv >> 24);
-            r++;
-        }
-    }
-    if (S.total_bits == 0) return;
-
-    // Owned words: first bit in [Ak, An) (chunk 0 also owns the word holding A0).
-    const uint64_t w_first = (k == 0) ? (A0 >> 5) : ((Ak + 31) >> 5);
-    const uint64_t w_end = (An + 31) >> 5;  // exclusive
-    if (w_first >= w_end) return;
-    // Tail of the last owned word: codewords of the following chunk.
-    const uint64_t tail_bits = w_end * 32 - An;
-    if (tail_bits > 0 && k + 1 < nchunks && threadIdx.x < 64) {
-        const uint64_t nb = (k + 1) * kChunk;
-        const uint64_t i = nb + threadIdx.x;
-        uint32_t s = 0, l = 0;
-        if (threadIdx.x < 32 && i < n) {
-            s = codes[i];
-            l = len_tab[s];
-        }
-        uint32_t incl = wave_incl_scan(l);
-        uint64_t q = (An - base_rel) + (incl - l);
-        if (threadIdx.x < 32 && i < n && (incl - l) < tail_bits) {
-            if (l > 32) {
-                emit_bits(words, q, cw_hi[s], l - 32, lds_words);
-                emit_bits(words, q + (l - 32), cw_lo[s], 32, lds_words);
-            } else if (l > 0) {
-                emit_bits(words, q, cw_lo[s], l, lds_words);
-            }
-        }
-    }
-    __syncthreads();
+    const uint64_t W0 = A0 >> 5;  // word holding the stream's first bit (shared with the tree bytes)
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
-    for (uint64_t W = w_first + threadIdx.x; W < w_end; W += blockDim.x) {
-        uint32_t v = words[W - Wb];
-        if (W == (A0 >> 5) && (A0 & 31)) {
-            // merge with the tree bytes that precede the stream in this word
-            const uint32_t keep = (uint32_t)(A0 & 31);
-            const uint32_t old = bswap32(out32[W]);
-            const uint32_t mask = ~0u << (32 - keep);
-            v = (old & mask) | (v & ~mask);
+    const uint32_t lane = lane_id();
+    const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
+
+    auto store_word = [&](uint64_t W, uint32_t v) {
+        if (W == W0 && (A0 & 31)) {
+            // keep the tree bytes that precede the stream in this word
+            const uint32_t mask = ~0u << (32 - (uint32_t)(A0 & 31));
+            v = (bswap32(out32[W]) & mask) | (v & ~mask);
         }
         if ((W + 1) * 4 > S.stream_end_byte) {
-            uint8_t* b = archive + W * 4;
+            uint8_t* bp = archive + W * 4;
             for (int j = 0; j < 4; j++)
-                if (W * 4 + j < S.stream_end_byte) b[j] = (uint8_t)(v >> (24 - 8 * j));
+                if (W * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(v >> (24 - 8 * j));
         } else {
             out32[W] = bswap32(v);
+        }
+    };
+
+    for (uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k < nchunks; k += nw) {
+        const uint64_t base = k * kChunk + lane * kSymPerThread;
+        uint32_t sym[kSymPerThread];
+        load_syms(codes, n, base, sym);
+        uint32_t tbits = 0, tz = 0;
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) {
+            if (sym[j] != 0xFFFFFFFFu) {
+                tbits += lookup<LDS_BOOK>(sym[j], tab, cw_hi, cw_lo, len_tab).len;
+                tz += sym[j] == 0;
+            }
+        }
+        // bits (low 17) and outliers (high 15) in one wavefront scan
+        const uint32_t v = tbits | (tz << 17);
+        const uint32_t incl = wave_incl_scan(v);
+        const uint32_t excl = incl - v;
+        const uint64_t start = A0 + bit_off[k] + (excl & 0x1FFFF);
+        uint64_t r = out_off[k] + (excl >> 17);
+
+        const uint32_t o = (uint32_t)(start & 31);
+        uint64_t W = start >> 5;
+        const bool own_first = (o == 0) || (start == A0);
+        bool first = true;
+        uint64_t acc = 0;
+        int nacc = (int)o;  // bits of the previous owner stand in as zeros
+        auto sink = [&](uint32_t word) {
+            if (!first || own_first) store_word(W, word);
+            first = false;
+            W++;
+        };
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) {
+            const uint32_t sj = sym[j];
+            if (sj == 0xFFFFFFFFu) break;
+            if (S.total_bits) acc_code(acc, nacc, lookup<LDS_BOOK>(sj, tab, cw_hi, cw_lo, len_tab), sink);
+            if (sj == 0) {
+                // outlier compaction, index order
+                const uint32_t bitsv = __float_as_uint(sparse[base + j]);
+                uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
+                dst[0] = (uint8_t)bitsv;
+                dst[1] = (uint8_t)(bitsv >> 8);
+                dst[2] = (uint8_t)(bitsv >> 16);
+                dst[3] = (uint8_t)(bitsv >> 24);
+                r++;
+            }
+        }
+        if (S.total_bits && nacc > 0 && (!first || own_first)) {
+            // complete the last word with the bits that follow this lane's range
+            uint64_t sacc = 0;
+            int sn = 0;
+            for (uint64_t i = base + kSymPerThread; sn < 32 && i < n; i++) {
+                const Code c = lookup<LDS_BOOK>(codes[i], tab, cw_hi, cw_lo, len_tab);
+                if (c.len > 32) {
+                    sacc = (sacc << (c.len - 32)) | c.hi;
+                    sn += (int)c.len - 32;
+                    if (sn < 32) {
+                        sacc = (sacc << 32) | c.lo;
+                        sn += 32;
+                    }
+                } else {
+                    sacc = (sacc << c.len) | c.lo;
+                    sn += (int)c.len;
+                }
+            }
+            const uint32_t succ = sn >= 32 ? (uint32_t)(sacc >> (sn - 32)) : (uint32_t)(sacc << (32 - sn));
+            store_word(W, (uint32_t)(acc << (32 - nacc)) | (succ >> nacc));
         }
     }
 }
@@ -640,19 +901,49 @@ int dq_lds_bytes(int dict) {
     return (kHotBins * 64 + (dict <= kMaxLdsHistBins ? dict : 0)) * 4;
 }
 
+__global__ void k_range_init(uint32_t* out) {
+    if (threadIdx.x == 0) {
+        out[0] = 0xFFFFFFFFu;
+        out[1] = 0u;
+    }
+}
+
 void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st) {
     uint64_t n4 = (n / 4 + 255) / 256;
     int grid = (int)std::min<uint64_t>(std::max<uint64_t>(n4, 1), 2048);
+    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(64), 0, st, out);
     hipLaunchKernelGGL(k_range, dim3(grid), dim3(256), 0, st, x, n, out);
 }
 
+void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ebfam, dim3(1), dim3(64), 0, st, mm, x, eb, out);
+}
+
+bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0) {
+    if (ndim == 1) return false;
+    if ((d0 % 4) != 0 || (reinterpret_cast<uintptr_t>(in) & 15) != 0 || d0 < 128) return false;
+    if (ndim == 3 && B > 16) return false;
+    return true;
+}
+
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
-                      const DQArgs& a, int grid, hipStream_t st) {
-    const int lds = dq_lds_bytes(a.dict);
-#define DQ_CASE(ND, KERN, BB)                                                                    \
-    if (ndim == ND && B == BB) {                                                                 \
-        hipLaunchKernelGGL(KERN<BB>, dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, a); \
-        return;                                                                                  \
+                      uint32_t* partials, const DQArgs& a0, int grid, hipStream_t st) {
+    const int lds = dq_lds_bytes(a0.dict);
+    DQArgs a = a0;
+    const bool x4 = dualquant_use_x4(ndim, B, in, a.d0);
+    if (x4) a.ntiles_x = (a.d0 + 255) / 256;
+#define DQ_CASE(ND, KERN, BB)                                                                              \
+    if (ndim == ND && B == BB) {                                                                           \
+        hipLaunchKernelGGL(KERN<BB>, dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
+        goto launched;                                                                                     \
+    }
+    if (x4) {
+        DQ_CASE(3, k_dualquant_3d_x4, 8)
+        DQ_CASE(3, k_dualquant_3d_x4, 16)
+        DQ_CASE(2, k_dualquant_2d_x4, 8)
+        DQ_CASE(2, k_dualquant_2d_x4, 16)
+        DQ_CASE(2, k_dualquant_2d_x4, 32)
+        DQ_CASE(2, k_dualquant_2d_x4, 64)
     }
     DQ_CASE(3, k_dualquant_3d, 8)
     DQ_CASE(3, k_dualquant_3d, 16)
@@ -667,26 +958,39 @@ void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* 
     DQ_CASE(1, k_dualquant_1d, 32)
     DQ_CASE(1, k_dualquant_1d, 64)
 #undef DQ_CASE
+    return;
+launched:
+    if (a.dict <= kMaxLdsHistBins)
+        hipLaunchKernelGGL(k_hist_reduce, dim3((a.dict + 255) / 256, (grid + kHistRows - 1) / kHistRows), dim3(256), 0,
+                           st, partials, grid, a.dict, hist);
 }
 
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
                        uint64_t nchunks, hipStream_t st) {
-    hipLaunchKernelGGL(k_chunk_bits, dim3((unsigned)nchunks), dim3(256), 0, st, codes, n, len_tab, dict, cb, cz);
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, 2048));
+    if (dict <= kMaxLdsLenTab)
+        hipLaunchKernelGGL(k_chunk_bits<true>, dim3(grid), dim3(256), 0, st, codes, n, nchunks, len_tab, dict, cb, cz);
+    else
+        hipLaunchKernelGGL(k_chunk_bits<false>, dim3(grid), dim3(256), 0, st, codes, n, nchunks, len_tab, dict, cb, cz);
 }
 
-void launch_scan_chunks(const uint32_t* cb, const uint32_t* cz, uint64_t nchunks, uint64_t* bit_off, uint64_t* out_off,
-                        EncSummary* sum, uint8_t* archive, uint64_t meta_bytes, uint64_t tree_bytes, uint64_t out_cap,
-                        uint32_t* err, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, st, cb, cz, nchunks, bit_off, out_off, sum, archive,
-                       meta_bytes, tree_bytes, out_cap, err);
+void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
+                        uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st) {
+    hipLaunchKernelGGL(k_enc_summary, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, tree_bytes, out_cap,
+                       err);
 }
 
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st) {
-    uint32_t lds_words = (uint32_t)(((uint64_t)kChunk * (uint64_t)max_len + 64) / 32 + 4);
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)nchunks), dim3(256), lds_words * 4, st, codes, n, cw_hi, cw_lo, len_tab,
-                       dict, bit_off, out_off, nchunks, sum, sparse, archive, err, lds_words);
+    (void)max_len;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, 2048));
+    if (dict <= kMaxLdsBook)
+        hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), 0, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
+                           out_off, nchunks, sum, sparse, archive, err);
+    else
+        hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), 0, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
+                           out_off, nchunks, sum, sparse, archive, err);
 }
 
 }  // namespace vsz
