@@ -99,6 +99,17 @@ int vecsz_decompress_host(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* 
 int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint64_t dims[3], double* eb,
                        uint64_t* huff_size, uint64_t* n_outlier);
 
+/* Huffman codebook of a quant-code histogram, built on the host exactly as the
+ * reference builds it: HuffmanTree init / build_tree / build_code
+ * (src/huffman.hh:239-262, :200-227) and the tree serialization written into
+ * the archive (encode_withTree, src/huffman.hh:425-502).  Part of every
+ * compress call; exported so the codebook can be checked and reused on its
+ * own.  h_code_bits / h_code_len receive `dict` entries (right-aligned
+ * codeword, length; length 0 for absent symbols and for a one-symbol tree).
+ * With h_tree NULL only *tree_bytes and *node_count are set. */
+int vecsz_huffman_codebook(const uint64_t* h_freq, int dict, uint64_t* h_code_bits, uint8_t* h_code_len,
+                           uint8_t* h_tree, uint64_t tree_cap, uint64_t* tree_bytes, int* node_count);
+
 /* Per-stage device timing of the last compress/decompress call, measured
  * with HIP events on the context's stream when enabled (off by default).
  * names/ms receive up to `cap` entries; returns the entry count. */

@@ -810,6 +810,25 @@ int vecsz_archive_info(const uint8_t* h, uint64_t bytes, int* ndim, uint64_t dim
     return parse_meta(h, bytes, bytes, ndim, dims, eb, &pad, huff_size, n_outlier);
 }
 
+int vecsz_huffman_codebook(const uint64_t* freq, int dict, uint64_t* code_bits, uint8_t* code_len, uint8_t* tree,
+                           uint64_t tree_cap, uint64_t* tree_bytes, int* node_count) {
+    if (!freq || dict < 2 || dict > 65536 || !tree_bytes || !node_count) return VECSZ_EINVAL;
+    Codebook cb;
+    const int rc = build_codebook(freq, dict, cb);
+    if (rc) return rc;
+    *tree_bytes = cb.tree.size();
+    *node_count = cb.node_count;
+    for (int s = 0; s < dict; s++) {
+        if (code_bits) code_bits[s] = ((uint64_t)cb.cw_hi[s] << 32) | cb.cw_lo[s];
+        if (code_len) code_len[s] = cb.len[s];
+    }
+    if (tree) {
+        if (tree_cap < cb.tree.size()) return VECSZ_ENOMEM;
+        memcpy(tree, cb.tree.data(), cb.tree.size());
+    }
+    return VECSZ_OK;
+}
+
 int vecsz_ctx_enable_timing(vecsz_ctx* c, int on) {
     if (!c) return VECSZ_EINVAL;
     c->timing = on != 0;

@@ -26,6 +26,15 @@ struct HeapEntry {
     int32_t n;
 };
 
+// Per-thread work arrays, kept between calls so a codebook build allocates
+// nothing once the first field of a given symbol count has been seen.
+struct Work {
+    std::vector<int32_t> sym, kid, len, size, idx;
+    std::vector<uint64_t> bits;
+    std::vector<HeapEntry> heap;
+};
+thread_local Work t_work;
+
 }  // namespace
 
 uint64_t tree_byte_size(uint64_t nc) {
@@ -43,10 +52,16 @@ int build_codebook(const uint64_t* freq, int dict, Codebook& cb) {
     cb.nsym = nsym;
     if (nsym == 0) return VECSZ_EINVAL;
 
-    const int cap = 2 * nsym;
-    std::vector<int32_t> left(cap, -1), right(cap, -1), sym(cap, -1);
-    std::vector<HeapEntry> qq(nsym + 2);
-    int n_nodes = 0;
+    // Node numbering: leaves 0..nsym-1 in ascending symbol order, then one
+    // internal node per merge, so a parent always outnumbers its children and
+    // the root is the last node.
+    const int nc = 2 * nsym - 1;
+    std::vector<int32_t>& sym = t_work.sym;
+    std::vector<int32_t>& kid = t_work.kid;  // kid[2(n - nsym)] = left, +1 = right
+    sym.resize(nsym);
+    kid.resize(2 * (size_t)nsym);
+    std::vector<HeapEntry>& qq = t_work.heap;
+    qq.resize(nsym + 2);
     int qend = 1;
 
     auto qinsert = [&](HeapEntry e) {
@@ -58,113 +73,102 @@ int build_codebook(const uint64_t* freq, int dict, Codebook& cb) {
         }
         qq[i] = e;
     };
+    // The hole walk is the hot loop: the child choice is data dependent and
+    // unpredictable, so it is computed as an index increment, not a branch.
+    // qq[l + 1] with l + 1 == qend is still inside the array.
     auto qremove = [&]() -> HeapEntry {
         HeapEntry top = qq[1];
-        qend--;
+        const int end = --qend;
         int i = 1, l;
-        while ((l = i << 1) < qend) {
-            if (l + 1 < qend && qq[l + 1].f < qq[l].f) l++;
+        while ((l = i << 1) < end) {
+            l += (int)((l + 1 < end) & (qq[l + 1].f < qq[l].f));
             qq[i] = qq[l];
             i = l;
         }
-        qq[i] = qq[qend];
+        qq[i] = qq[end];
         return top;
     };
 
-    for (int s = 0; s < dict; s++) {
+    for (int s = 0, n = 0; s < dict; s++) {
         if (!freq[s]) continue;
-        int n = n_nodes++;
         sym[n] = s;
         qinsert({freq[s], n});
+        n++;
     }
-    while (qend > 2) {
+    for (int n = nsym; qend > 2; n++) {
         HeapEntry b = qremove();  // first removed: right child
         HeapEntry a = qremove();  // second removed: left child
-        int n = n_nodes++;
-        left[n] = a.n;
-        right[n] = b.n;
+        kid[2 * (n - nsym)] = a.n;
+        kid[2 * (n - nsym) + 1] = b.n;
         qinsert({a.f + b.f, n});
     }
-    const int32_t root = qq[1].n;
 
-    // Codewords (DFS, left first).
-    struct Item {
-        int32_t n;
-        int len;
-        uint64_t bits;
-    };
-    std::vector<Item> stk;
-    stk.reserve(cap);
-    stk.push_back({root, 0, 0});
-    int max_len = 0;
-    bool too_long = false;
-    while (!stk.empty()) {
-        Item it = stk.back();
-        stk.pop_back();
-        if (sym[it.n] >= 0) {
-            int s = sym[it.n];
-            if (it.len > 64) {
-                too_long = true;
-                continue;
-            }
-            cb.len[s] = (uint8_t)it.len;
-            if (it.len > 32) {
-                cb.cw_hi[s] = (uint32_t)(it.bits >> 32);
-                cb.cw_lo[s] = (uint32_t)it.bits;
-            } else {
-                cb.cw_lo[s] = (uint32_t)it.bits;
-            }
-            if (it.len > max_len) max_len = it.len;
-            continue;
-        }
-        stk.push_back({right[it.n], it.len + 1, (it.bits << 1) | 1});
-        stk.push_back({left[it.n], it.len + 1, it.bits << 1});
+    // Codewords top-down in descending node order (parents before children).
+    std::vector<uint64_t>& bits = t_work.bits;
+    std::vector<int32_t>& len = t_work.len;
+    bits.resize(nc);
+    len.resize(nc);
+    bits[nc - 1] = 0;
+    len[nc - 1] = 0;
+    for (int n = nc - 1; n >= nsym; n--) {
+        const int l = kid[2 * (n - nsym)], r = kid[2 * (n - nsym) + 1];
+        const int ln = len[n] + 1;
+        len[l] = ln;
+        len[r] = ln;
+        bits[l] = bits[n] << 1;
+        bits[r] = (bits[n] << 1) | 1;
     }
-    if (too_long) return VECSZ_ECODELEN;
+    int max_len = 0;
+    for (int n = 0; n < nsym; n++) {
+        const int L = len[n];
+        if (L > 64) return VECSZ_ECODELEN;
+        const int s = sym[n];
+        cb.len[s] = (uint8_t)L;
+        cb.cw_lo[s] = (uint32_t)bits[n];
+        if (L > 32) cb.cw_hi[s] = (uint32_t)(bits[n] >> 32);
+        max_len = L > max_len ? L : max_len;
+    }
     cb.max_len = max_len;
 
-    // Serialization: preorder numbering, root 0, each child takes ++n_inode
-    // when it is visited (pad_tree_*, src/huffman.hh:374-423).
-    const int nc = 2 * nsym - 1;
+    // Serialization (pad_tree_*, src/huffman.hh:374-423): nodes are numbered
+    // in preorder, left subtree first, root 0.  With subtree sizes that is
+    // idx(left) = idx(parent) + 1, idx(right) = idx(left) + size(left).
+    std::vector<int32_t>& size = t_work.size;
+    std::vector<int32_t>& idx = t_work.idx;
+    size.resize(nc);
+    idx.resize(nc);
+    for (int n = 0; n < nsym; n++) size[n] = 1;
+    for (int n = nsym; n < nc; n++) size[n] = 1 + size[kid[2 * (n - nsym)]] + size[kid[2 * (n - nsym) + 1]];
+    idx[nc - 1] = 0;
+    for (int n = nc - 1; n >= nsym; n--) {
+        const int l = kid[2 * (n - nsym)], r = kid[2 * (n - nsym) + 1];
+        idx[l] = idx[n] + 1;
+        idx[r] = idx[n] + 1 + size[l];
+    }
     cb.node_count = nc;
     const int w = nc <= 256 ? 1 : (nc <= 65536 ? 2 : 4);
-    std::vector<uint32_t> L(nc, 0), R(nc, 0), C(nc, 0);
-    std::vector<uint8_t> T(nc, 0);
-    struct Visit {
-        int32_t n;
-        int32_t parent;
-        int side;
+    cb.tree.assign(tree_byte_size(nc), 0);  // byte 0: CPU_sysEndianType (src/huffman.hh:72)
+    uint8_t* Lp = cb.tree.data() + 1;
+    uint8_t* Rp = Lp + (size_t)nc * w;
+    uint8_t* Cp = Rp + (size_t)nc * w;
+    uint8_t* Tp = Cp + (size_t)nc * 4;
+    auto put = [&](uint8_t* base, int i, uint32_t v) {
+        for (int b = 0; b < w; b++) base[(size_t)w * i + b] = (uint8_t)(v >> (8 * b));
     };
-    std::vector<Visit> vs;
-    vs.reserve(cap);
-    vs.push_back({root, -1, 0});
-    uint32_t n_inode = 0;
-    bool first = true;
-    while (!vs.empty()) {
-        Visit v = vs.back();
-        vs.pop_back();
-        uint32_t idx = first ? 0 : ++n_inode;
-        first = false;
-        if (v.parent >= 0) (v.side ? R : L)[v.parent] = idx;
-        bool leaf = sym[v.n] >= 0;
-        C[idx] = leaf ? (uint32_t)sym[v.n] : 0;
-        T[idx] = leaf ? 1 : 0;
-        if (!leaf) {
-            vs.push_back({right[v.n], (int32_t)idx, 1});
-            vs.push_back({left[v.n], (int32_t)idx, 0});
-        }
+    for (int n = 0; n < nsym; n++) {
+        const int i = idx[n];
+        const uint32_t s = (uint32_t)sym[n];
+        Cp[4 * (size_t)i] = (uint8_t)s;
+        Cp[4 * (size_t)i + 1] = (uint8_t)(s >> 8);
+        Cp[4 * (size_t)i + 2] = (uint8_t)(s >> 16);
+        Cp[4 * (size_t)i + 3] = (uint8_t)(s >> 24);
+        Tp[i] = 1;
     }
-    cb.tree.assign(tree_byte_size(nc), 0);
-    uint8_t* o = cb.tree.data();
-    *o++ = 0;  // CPU_sysEndianType (src/huffman.hh:72)
-    for (int pass = 0; pass < 2; pass++) {
-        const std::vector<uint32_t>& A = pass == 0 ? L : R;
-        for (int i = 0; i < nc; i++)
-            for (int b = 0; b < w; b++) *o++ = (uint8_t)(A[i] >> (8 * b));
+    for (int n = nsym; n < nc; n++) {
+        const int i = idx[n];
+        put(Lp, i, (uint32_t)idx[kid[2 * (n - nsym)]]);
+        put(Rp, i, (uint32_t)idx[kid[2 * (n - nsym) + 1]]);
     }
-    for (int i = 0; i < nc; i++)
-        for (int b = 0; b < 4; b++) *o++ = (uint8_t)(C[i] >> (8 * b));
-    for (int i = 0; i < nc; i++) *o++ = T[i];
     return VECSZ_OK;
 }
 

@@ -725,66 +725,80 @@ __global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* _
     }
 }
 
-__device__ __forceinline__ uint32_t bswap32(uint32_t v) { return __builtin_bswap32(v); }
-
-// Bit packing, one wavefront per 1024-symbol chunk, no barriers or atomics.
-// Lane L holds 16 consecutive symbols; a wavefront scan gives its absolute
-// first bit p_L = A_k + (bits of lanes < L).  Every 32-bit word of the stream
-// is written once, by the lane holding its first bit: that lane appends its
-// codewords to a word-aligned accumulator, stores each completed word, and
-// completes its last word with the leading bits of the codewords that follow
-// its range (read back from the code array).  Words go straight to global
-// memory; each lane's stores are contiguous and the wavefront's cover a
-// contiguous span, which the L2 merges into full lines.
-constexpr uint32_t kLongCode = 63;  // LDS table marker: codeword longer than 26 bits
-
+// Bit packing, one wavefront per 1024-symbol chunk, no workgroup barriers or
+// global atomics.  Lane L holds 16 consecutive symbols; a wavefront scan gives
+// its first bit inside the chunk.  The lanes OR their codewords into a
+// wave-private LDS image of the 32-bit words the chunk touches; the wavefront
+// completes the chunk's last word with the leading bits of the next chunk
+// (fewer than 32 bits, always inside the next chunk's first 32 codewords since
+// a codeword has at least one bit) and stores the image with coalesced dword
+// writes.  A word shared by two chunks belongs to the earlier one; the stream's
+// first word keeps the tree bytes in front of it.
 struct Code {
     uint32_t lo, hi, len;  // codeword = hi:lo, right-aligned, len bits
 };
 
-template <bool LDS_BOOK>
-__device__ __forceinline__ Code lookup(uint32_t s, const uint32_t* tab, const uint32_t* __restrict__ cw_hi,
-                                       const uint32_t* __restrict__ cw_lo, const uint8_t* __restrict__ len_tab) {
+// LDS_BOOK: LDS table entry (codeword << 6) | length for codewords of at most
+// 26 bits; longer ones (rare symbols) carry kLongCode and are read from the
+// global tables.
+constexpr uint32_t kLongCode = 63;
+
+__device__ __forceinline__ Code lookup_global(uint32_t s, const uint32_t* __restrict__ cw_hi,
+                                              const uint32_t* __restrict__ cw_lo, const uint8_t* __restrict__ len_tab) {
     Code c;
-    if (LDS_BOOK) {
-        const uint32_t e = tab[s];
-        c.len = e & 63;
-        c.lo = e >> 6;
-        c.hi = 0;
-        if (c.len == kLongCode) {
-            c.len = len_tab[s];
-            c.lo = cw_lo[s];
-            c.hi = cw_hi[s];
-        }
-    } else {
-        c.len = len_tab[s];
-        c.lo = cw_lo[s];
-        c.hi = c.len > 32 ? cw_hi[s] : 0;
-    }
+    c.len = len_tab[s];
+    c.lo = cw_lo[s];
+    c.hi = c.len > 32 ? cw_hi[s] : 0;
     return c;
 }
 
-// Appends l (1..32) right-aligned bits v to a right-aligned accumulator of
-// nacc (< 32) pending bits; a completed 32-bit word goes to `sink`.
-template <typename Sink>
-__device__ __forceinline__ void acc_put(uint64_t& acc, int& nacc, uint32_t v, int l, Sink&& sink) {
-    acc = (acc << l) | v;
-    nacc += l;
-    if (nacc >= 32) {
-        nacc -= 32;
-        sink((uint32_t)(acc >> nacc));
-        acc &= (1ull << nacc) - 1;
+template <bool LDS_BOOK>
+__device__ __forceinline__ Code lookup(uint32_t s, const uint32_t* tab, const uint32_t* __restrict__ cw_hi,
+                                       const uint32_t* __restrict__ cw_lo, const uint8_t* __restrict__ len_tab) {
+    if (LDS_BOOK) {
+        const uint32_t e = tab[s];
+        if ((e & 63) != kLongCode) return Code{e >> 6, 0, e & 63};
     }
+    return lookup_global(s, cw_hi, cw_lo, len_tab);
 }
 
-template <typename Sink>
-__device__ __forceinline__ void acc_code(uint64_t& acc, int& nacc, const Code& c, Sink&& sink) {
-    if (c.len > 32) {
-        acc_put(acc, nacc, c.hi, (int)c.len - 32, sink);
-        acc_put(acc, nacc, c.lo, 32, sink);
-    } else if (c.len > 0) {
-        acc_put(acc, nacc, c.lo, (int)c.len, sink);
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Raw codes of one chunk as lane L sees them: its 16 symbols (two 16-byte
+// loads), how many of them exist, and, for lanes 0..31, symbol L of the next
+// chunk (the next chunk's leading bits complete this chunk's last word).
+struct ChunkRaw {
+    uint32_t w[8];
+    uint32_t nvalid;
+    uint32_t next;  // 0xFFFFFFFF: none
+    uint64_t bit_off, out_off;
+};
+
+__device__ __forceinline__ void load_chunk(const uint16_t* __restrict__ codes, uint64_t n, uint64_t nchunks,
+                                           const uint64_t* __restrict__ bit_off,
+                                           const uint64_t* __restrict__ out_off, uint64_t k, uint32_t lane,
+                                           ChunkRaw& r) {
+    const uint64_t base = k * kChunk + lane * kSymPerThread;
+    r.nvalid = base >= n ? 0u : (uint32_t)min<uint64_t>(kSymPerThread, n - base);
+    if (r.nvalid == kSymPerThread) {
+        const uint4* p = reinterpret_cast<const uint4*>(codes + base);
+        const uint4 u0 = p[0], u1 = p[1];
+        r.w[0] = u0.x, r.w[1] = u0.y, r.w[2] = u0.z, r.w[3] = u0.w;
+        r.w[4] = u1.x, r.w[5] = u1.y, r.w[6] = u1.z, r.w[7] = u1.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) r.w[j] = 0;
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++)
+            if ((uint32_t)j < r.nvalid) r.w[j >> 1] |= (uint32_t)codes[base + j] << (16 * (j & 1));
     }
+    const uint64_t i = (k + 1) * kChunk + lane;
+    r.next = (lane < 32 && k + 1 < nchunks && i < n) ? (uint32_t)codes[i] : 0xFFFFFFFFu;
+    r.bit_off = bit_off[k];
+    r.out_off = out_off[k];
 }
 
 template <bool LDS_BOOK>
@@ -794,104 +808,165 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
                                               const uint64_t* __restrict__ bit_off, const uint64_t* __restrict__ out_off,
                                               uint64_t nchunks, const EncSummary* __restrict__ sum,
                                               const float* __restrict__ sparse, uint8_t* __restrict__ archive,
-                                              const uint32_t* __restrict__ err) {
+                                              const uint32_t* __restrict__ err, uint32_t stage_words) {
     if (*err) return;
-    __shared__ uint32_t tab[LDS_BOOK ? kMaxLdsBook : 1];
+    extern __shared__ uint32_t lds[];
+    uint32_t* tab = lds;
     if (LDS_BOOK)
         for (int i = threadIdx.x; i < dict; i += blockDim.x) {
             const uint32_t l = len_tab[i];
             tab[i] = l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
         }
     __syncthreads();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* stage = lds + (LDS_BOOK ? dict : 0) + wave * stage_words;
     const EncSummary S = *sum;
-    const uint64_t A0 = S.stream_bit_off;
-    const uint64_t W0 = A0 >> 5;  // word holding the stream's first bit (shared with the tree bytes)
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint32_t lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
+    uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + wave;
+    if (k >= nchunks) return;  // no workgroup barrier follows
 
-    auto store_word = [&](uint64_t W, uint32_t v) {
-        if (W == W0 && (A0 & 31)) {
-            // keep the tree bytes that precede the stream in this word
-            const uint32_t mask = ~0u << (32 - (uint32_t)(A0 & 31));
-            v = (bswap32(out32[W]) & mask) | (v & ~mask);
-        }
-        if ((W + 1) * 4 > S.stream_end_byte) {
-            uint8_t* bp = archive + W * 4;
-            for (int j = 0; j < 4; j++)
-                if (W * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(v >> (24 - 8 * j));
-        } else {
-            out32[W] = bswap32(v);
-        }
-    };
-
-    for (uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k < nchunks; k += nw) {
+    // software pipeline: the next chunk's loads are in flight while this one is packed
+    ChunkRaw nxt;
+    load_chunk(codes, n, nchunks, bit_off, out_off, k, lane, nxt);
+    for (; k < nchunks; k += nw) {
+        const ChunkRaw cur = nxt;
+        if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, k + nw, lane, nxt);
         const uint64_t base = k * kChunk + lane * kSymPerThread;
-        uint32_t sym[kSymPerThread];
-        load_syms(codes, n, base, sym);
-        uint32_t tbits = 0, tz = 0;
+
+        // LDS_BOOK keeps the raw table entry (one register per symbol)
+        uint32_t ent[LDS_BOOK ? kSymPerThread : 1];
+        Code c[LDS_BOOK ? 1 : kSymPerThread];
+        uint32_t tbits = 0, tz = 0, long_mask = 0;
 #pragma unroll
         for (int j = 0; j < kSymPerThread; j++) {
-            if (sym[j] != 0xFFFFFFFFu) {
-                tbits += lookup<LDS_BOOK>(sym[j], tab, cw_hi, cw_lo, len_tab).len;
-                tz += sym[j] == 0;
+            const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+            const bool valid = (uint32_t)j < cur.nvalid;
+            if constexpr (LDS_BOOK) {
+                ent[j] = valid ? tab[sj] : 0u;
+                const uint32_t l = ent[j] & 63;
+                tbits += l;
+                long_mask |= (uint32_t)(l == kLongCode) << j;
+            } else {
+                c[j] = Code{0, 0, 0};
+                if (valid) c[j] = lookup_global(sj, cw_hi, cw_lo, len_tab);
+                tbits += c[j].len;
             }
+            tz += valid && sj == 0;
+        }
+        if (LDS_BOOK && long_mask) {
+            // codewords of more than 26 bits: true lengths from the global table
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++)
+                if ((long_mask >> j) & 1) tbits += (uint32_t)len_tab[(cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF] - kLongCode;
         }
         // bits (low 17) and outliers (high 15) in one wavefront scan
         const uint32_t v = tbits | (tz << 17);
         const uint32_t incl = wave_incl_scan(v);
         const uint32_t excl = incl - v;
-        const uint64_t start = A0 + bit_off[k] + (excl & 0x1FFFF);
-        uint64_t r = out_off[k] + (excl >> 17);
+        const uint32_t cbits = __shfl(incl, 63) & 0x1FFFF;
 
-        const uint32_t o = (uint32_t)(start & 31);
-        uint64_t W = start >> 5;
-        const bool own_first = (o == 0) || (start == A0);
-        bool first = true;
-        uint64_t acc = 0;
-        int nacc = (int)o;  // bits of the previous owner stand in as zeros
-        auto sink = [&](uint32_t word) {
-            if (!first || own_first) store_word(W, word);
-            first = false;
-            W++;
-        };
+        // outlier compaction, index order
+        if (tz) {
+            uint64_t r = cur.out_off + (excl >> 17);
 #pragma unroll
-        for (int j = 0; j < kSymPerThread; j++) {
-            const uint32_t sj = sym[j];
-            if (sj == 0xFFFFFFFFu) break;
-            if (S.total_bits) acc_code(acc, nacc, lookup<LDS_BOOK>(sj, tab, cw_hi, cw_lo, len_tab), sink);
-            if (sj == 0) {
-                // outlier compaction, index order
-                const uint32_t bitsv = __float_as_uint(sparse[base + j]);
-                uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
-                dst[0] = (uint8_t)bitsv;
-                dst[1] = (uint8_t)(bitsv >> 8);
-                dst[2] = (uint8_t)(bitsv >> 16);
-                dst[3] = (uint8_t)(bitsv >> 24);
-                r++;
-            }
-        }
-        if (S.total_bits && nacc > 0 && (!first || own_first)) {
-            // complete the last word with the bits that follow this lane's range
-            uint64_t sacc = 0;
-            int sn = 0;
-            for (uint64_t i = base + kSymPerThread; sn < 32 && i < n; i++) {
-                const Code c = lookup<LDS_BOOK>(codes[i], tab, cw_hi, cw_lo, len_tab);
-                if (c.len > 32) {
-                    sacc = (sacc << (c.len - 32)) | c.hi;
-                    sn += (int)c.len - 32;
-                    if (sn < 32) {
-                        sacc = (sacc << 32) | c.lo;
-                        sn += 32;
-                    }
-                } else {
-                    sacc = (sacc << c.len) | c.lo;
-                    sn += (int)c.len;
+            for (int j = 0; j < kSymPerThread; j++) {
+                const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+                if ((uint32_t)j < cur.nvalid && sj == 0) {
+                    const uint32_t bv = __float_as_uint(sparse[base + j]);
+                    uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
+                    dst[0] = (uint8_t)bv;
+                    dst[1] = (uint8_t)(bv >> 8);
+                    dst[2] = (uint8_t)(bv >> 16);
+                    dst[3] = (uint8_t)(bv >> 24);
+                    r++;
                 }
             }
-            const uint32_t succ = sn >= 32 ? (uint32_t)(sacc >> (sn - 32)) : (uint32_t)(sacc << (32 - sn));
-            store_word(W, (uint32_t)(acc << (32 - nacc)) | (succ >> nacc));
         }
+        if (S.total_bits == 0) continue;  // one-symbol tree: no bitstream
+
+        const uint64_t Ak = S.stream_bit_off + cur.bit_off;
+        const uint64_t wbase = Ak >> 5;
+        const uint32_t a0 = (uint32_t)(Ak & 31);
+        const uint32_t nwords = (a0 + cbits + 31) >> 5;
+        for (uint32_t i = lane; i < nwords; i += kWave) stage[i] = 0;
+        wave_lds_sync();
+
+        // Each codeword is ORed into the one or two stage words it overlaps
+        // (codeword at bit offset o of word w, MSB first).  The second OR may
+        // add zero to the word after the chunk's last; the stage has room.
+        uint32_t pos = a0 + (excl & 0x1FFFF);
+        auto place = [&](uint32_t cw, uint32_t l) {  // l <= 32; l = 0 places nothing (cw = 0)
+            const uint32_t o = pos & 31;
+            const uint64_t v = (uint64_t)cw << ((64 - o - l) & 63);
+            uint32_t* w = stage + (pos >> 5);
+            atomicOr(w, (uint32_t)(v >> 32));
+            atomicOr(w + 1, (uint32_t)v);
+            pos += l;
+        };
+        auto place_code = [&](const Code& c) {
+            if (c.len > 32) {
+                place(c.hi, c.len - 32);
+                place(c.lo, 32);
+            } else if (c.len) {
+                place(c.lo, c.len);
+            }
+        };
+        if constexpr (LDS_BOOK) {
+            if (!__any(long_mask != 0)) {
+#pragma unroll
+                for (int j = 0; j < kSymPerThread; j++) place(ent[j] >> 6, ent[j] & 63);  // absent symbols: ent 0
+            } else {
+#pragma unroll
+                for (int j = 0; j < kSymPerThread; j++) {
+                    const uint32_t l = ent[j] & 63;
+                    if (l == kLongCode)
+                        place_code(lookup_global((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF, cw_hi, cw_lo, len_tab));
+                    else if (l)
+                        place(ent[j] >> 6, l);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) place_code(c[j]);
+        }
+
+        // complete the last word with the next chunk's leading bits
+        const uint32_t e = (a0 + cbits) & 31;
+        if (e != 0 && k + 1 < nchunks) {
+            Code d{0, 0, 0};
+            if (cur.next != 0xFFFFFFFFu) d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
+            const uint32_t p = wave_incl_scan(d.len) - d.len;
+            uint32_t t = 0;
+            if (d.len && p < 32) {
+                const uint32_t kk = min(d.len, 32u - p);
+                const uint64_t cw = ((uint64_t)d.hi << 32) | d.lo;
+                t = (uint32_t)((cw >> (d.len - kk)) << (32 - p - kk));
+            }
+#pragma unroll
+            for (int m = 32; m > 0; m >>= 1) t |= __shfl_xor(t, m);
+            if (lane == 0) atomicOr(&stage[nwords - 1], t >> e);
+        }
+        wave_lds_sync();
+
+        for (uint32_t i = ((a0 != 0 && k != 0) ? 1u : 0u) + lane; i < nwords; i += kWave) {
+            const uint64_t Wg = wbase + i;
+            uint32_t wv = stage[i];
+            if (k == 0 && i == 0 && a0) {
+                // keep the tree bytes that precede the stream in this word
+                const uint32_t mask = ~0u << (32 - a0);
+                wv = (__builtin_bswap32(out32[Wg]) & mask) | (wv & ~mask);
+            }
+            if ((Wg + 1) * 4 > S.stream_end_byte) {
+                uint8_t* bp = archive + Wg * 4;
+                for (int j = 0; j < 4; j++)
+                    if (Wg * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(wv >> (24 - 8 * j));
+            } else {
+                out32[Wg] = __builtin_bswap32(wv);
+            }
+        }
+        wave_lds_sync();
     }
 }
 
@@ -983,14 +1058,18 @@ void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archiv
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st) {
-    (void)max_len;
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, 2048));
-    if (dict <= kMaxLdsBook)
-        hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), 0, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
-                           out_off, nchunks, sum, sparse, archive, err);
+    // words one chunk can touch: 1024 codewords of at most max_len bits, plus
+    // the partial words at either end
+    const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
+    const bool lds_book = dict <= kMaxLdsBook;
+    const size_t lds = ((lds_book ? (size_t)dict : 0) + 4 * (size_t)stage_words) * 4;
+    if (lds_book)
+        hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
+                           out_off, nchunks, sum, sparse, archive, err, stage_words);
     else
-        hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), 0, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
-                           out_off, nchunks, sum, sparse, archive, err);
+        hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, st, codes, n, cw_hi, cw_lo, len_tab, dict,
+                           bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words);
 }
 
 }  // namespace vsz

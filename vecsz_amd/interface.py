@@ -75,6 +75,32 @@ def archive_info(archive) -> dict:
             "huff_size": hs.value, "n_outlier": no.value}
 
 
+def huffman_codebook(freq, dict_size: Optional[int] = None):
+    """The reference's Huffman codebook for a quant-code histogram
+    (src/huffman.hh:239-262, :200-227, :425-502), built on the host by the
+    library: (code_bits u64[dict], code_len u8[dict], node_count, tree bytes)."""
+    lib = _lib.load()
+    f = np.ascontiguousarray(freq, dtype=np.uint64)
+    d = int(dict_size if dict_size is not None else f.size)
+    if f.size < d:
+        raise ValueError("histogram shorter than dict_size")
+    bits = np.zeros(d, dtype=np.uint64)
+    lens = np.zeros(d, dtype=np.uint8)
+    tb = ctypes.c_uint64()
+    nc = ctypes.c_int()
+    args = (f.ctypes.data_as(ctypes.c_void_p), d, bits.ctypes.data_as(ctypes.c_void_p),
+            lens.ctypes.data_as(ctypes.c_void_p))
+    rc = lib.vecsz_huffman_codebook(*args, None, 0, ctypes.byref(tb), ctypes.byref(nc))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    tree = np.zeros(tb.value, dtype=np.uint8)
+    rc = lib.vecsz_huffman_codebook(*args, tree.ctypes.data_as(ctypes.c_void_p), tb.value, ctypes.byref(tb),
+                                    ctypes.byref(nc))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return bits, lens, nc.value, tree.tobytes()
+
+
 def header_fields(head: bytes) -> dict:
     """ndim, dims (x first) and eb from the first bytes of an archive payload
     (datapack layout, src/utils/datapack.hh:33-47); no section-size checks."""
