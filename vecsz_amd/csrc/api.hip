@@ -94,8 +94,8 @@ struct vecsz_ctx {
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     Codebook cb;
     // decompress scratch
-    DevBuf lut, child, sub_start, sub_end, sub_end2, sub_cnt, sub_zeros, sub_bnd, sub_zmask, sub_nbnd, sub_dirty, sym_off,
-        zero_off, scan_tmp, scan_tot, dec_sum, flagged, nflagged;
+    DevBuf lut, subtab, lut16, sub16, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sym_off, zero_off, scan_tmp, scan_tot,
+        dec_sum, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
     HostBuf pin, pin2;
@@ -435,22 +435,24 @@ int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint
 // Serial decode on the host of a stream whose chunk-parallel decode did not
 // synchronise (e.g. fixed-length codes): the true start, count and zero count
 // of every subsequence.
+// Serial decode on the host: the last resort when the device rounds have
+// not converged.  Fills the true start, end and counts of every subsequence.
 int host_resync(vecsz_ctx* c, const DecodeTables& dt, const uint8_t* d_stream, uint64_t stream_bytes, uint64_t n,
                 uint64_t nsub) {
     std::vector<uint8_t> st(stream_bytes + 16, 0);
     HIPCHK(c, hipMemcpyAsync(st.data(), d_stream, stream_bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<uint64_t> start(nsub, 0);
+    std::vector<uint64_t> b(nsub, 0);
     std::vector<uint32_t> cnt(nsub, 0), zeros(nsub, 0);
     const uint64_t T = stream_bytes * 8;
-    uint64_t pos = 0, produced = 0;
-    auto bit = [&](uint64_t b) -> uint32_t { return (st[b >> 3] >> (7 - (b & 7))) & 1; };
+    uint64_t pos = 0;
+    auto bit = [&](uint64_t i) -> uint32_t { return (st[i >> 3] >> (7 - (i & 7))) & 1; };
     uint64_t cur = 0;
-    start[0] = 0;
-    while (pos < T && produced < n + 8) {
+    b[0] = 0;
+    while (pos < T) {
         while (cur + 1 < nsub && pos >= (cur + 1) * (uint64_t)kSubBits) {
             cur++;
-            start[cur] = pos;
+            b[cur] = pos;
         }
         uint32_t node = 0;
         uint32_t sym = 0;
@@ -467,10 +469,10 @@ int host_resync(vecsz_ctx* c, const DecodeTables& dt, const uint8_t* d_stream, u
         }
         cnt[cur]++;
         zeros[cur] += sym == 0;
-        produced++;
     }
-    for (uint64_t j = cur + 1; j < nsub; j++) start[j] = T;
-    HIPCHK(c, hipMemcpyAsync(c->sub_start.p, start.data(), nsub * 8, hipMemcpyHostToDevice, c->stream));
+    (void)n;
+    for (uint64_t j = cur + 1; j < nsub; j++) b[j] = pos;
+    HIPCHK(c, hipMemcpyAsync(c->sub_b.p, b.data(), nsub * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->sub_cnt.p, cnt.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->sub_zeros.p, zeros.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -554,40 +556,50 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         if (!tree_is_full(dt)) return fail(c, VECSZ_ECORRUPT, "Huffman tree is not full");
         const uint64_t T = stream_bytes * 8;
         const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
-        if (!c->lut.ensure(dt.lut.size() * 4) || !c->child.ensure(dt.child.size() * 4) ||
-            !c->sub_start.ensure(nsub * 8) || !c->sub_end.ensure(nsub * 8) || !c->sub_end2.ensure(nsub * 8) ||
-            !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) ||
-            !c->sub_bnd.ensure(nsub * kSyncRecord * 2) || !c->sub_zmask.ensure(nsub * 2) ||
-            !c->sub_nbnd.ensure(nsub) || !c->sub_dirty.ensure(nsub * 2) || !c->sym_off.ensure(nsub * 8) ||
-            !c->zero_off.ensure(nsub * 8) || !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) ||
-            !c->scan_tot.ensure(16) || !c->dec_sum.ensure(sizeof(DecSummary)))
+        if (!c->lut.ensure(dt.lut.size() * 4) || !c->subtab.ensure(std::max<size_t>(dt.sub.size(), 1) * 4) ||
+            !c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_e2.ensure(nsub * 8) ||
+            !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) || !c->sub_dirty.ensure(nsub * 2) ||
+            !c->sym_off.ensure(nsub * 8) || !c->zero_off.ensure(nsub * 8) ||
+            !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
+            !c->dec_sum.ensure(sizeof(DecSummary)))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
         HIPCHK(c, hipMemcpyAsync(c->lut.p, dt.lut.data(), dt.lut.size() * 4, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->child.p, dt.child.data(), dt.child.size() * 4, hipMemcpyHostToDevice, c->stream));
+        if (!dt.sub.empty())
+            HIPCHK(c, hipMemcpyAsync(c->subtab.p, dt.sub.data(), dt.sub.size() * 4, hipMemcpyHostToDevice, c->stream));
+        // compact tables when they exist and their deeper levels fit in LDS
+        const bool t16 = !dt.lut16.empty() && dt.sub16.size() <= kMaxSub16;
+        if (t16) {
+            if (!c->lut16.ensure(dt.lut16.size() * 2) || !c->sub16.ensure(std::max<size_t>(dt.sub16.size(), 1) * 2))
+                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+            HIPCHK(c, hipMemcpyAsync(c->lut16.p, dt.lut16.data(), dt.lut16.size() * 2, hipMemcpyHostToDevice, c->stream));
+            if (!dt.sub16.empty())
+                HIPCHK(c, hipMemcpyAsync(c->sub16.p, dt.sub16.data(), dt.sub16.size() * 2, hipMemcpyHostToDevice,
+                                         c->stream));
+        }
         HIPCHK(c, hipMemsetAsync(c->dec_sum.p, 0, sizeof(DecSummary), c->stream));
+        HIPCHK(c, hipMemsetAsync(c->sub_dirty.p, 0, nsub * 2, c->stream));
         DecodeArgs da;
         da.stream = d_stream;
         da.stream_bits = T;
         da.n = n;
         da.lut = c->lut.as<uint32_t>();
-        da.child = c->child.as<uint32_t>();
+        da.sub = c->subtab.as<uint32_t>();
+        da.lut16 = t16 ? c->lut16.as<uint16_t>() : nullptr;
+        da.sub16 = t16 ? c->sub16.as<uint16_t>() : nullptr;
+        da.n_sub16 = t16 ? (uint32_t)dt.sub16.size() : 0;
         da.nsub = nsub;
         da.outliers = d_outl;
         da.n_outlier = nout;
         SubState ss;
-        ss.start = c->sub_start.as<uint64_t>();
-        ss.end = c->sub_end.as<uint64_t>();
+        ss.b = c->sub_b.as<uint64_t>();
+        ss.e = c->sub_e.as<uint64_t>();
         ss.cnt = c->sub_cnt.as<uint32_t>();
-        ss.zeros = c->sub_zeros.as<uint32_t>();
-        ss.bnd = c->sub_bnd.as<uint16_t>();
-        ss.zmask = c->sub_zmask.as<uint16_t>();
-        ss.nbnd = c->sub_nbnd.as<uint8_t>();
+        ss.zer = c->sub_zeros.as<uint32_t>();
         ss.dirty = c->sub_dirty.as<uint8_t>();
         DecSummary* dsum = c->dec_sum.as<DecSummary>();
         {
             StageScope s(c, "decode_sync");
             launch_decode_sync(da, ss, dsum, c->stream);
-            launch_decode_check(da, ss, dsum, c->stream);
         }
         HIPCHK(c, hipGetLastError());
         DecSummary ds;
@@ -595,8 +607,8 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         HIPCHK(c, hipStreamSynchronize(c->stream));
         memcpy(&ds, h, sizeof(ds));
         if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
-        uint64_t* e_in = ss.end;
-        uint64_t* e_out = c->sub_end2.as<uint64_t>();
+        uint64_t* e_in = ss.e;
+        uint64_t* e_out = c->sub_e2.as<uint64_t>();
         uint8_t* dcur = ss.dirty;
         uint8_t* dnxt = ss.dirty + nsub;
         int rounds = 0;
@@ -620,7 +632,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         }
         {
             StageScope s(c, "decode_scan");
-            launch_scan_u32x2(ss.cnt, ss.zeros, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
+            launch_scan_u32x2(ss.cnt, ss.zer, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
                               c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream);
         }
         {

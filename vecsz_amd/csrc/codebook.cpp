@@ -224,25 +224,82 @@ int build_decode_tables(const uint8_t* huff, uint64_t huff_size, DecodeTables& d
             }
         }
     }
-    const uint32_t K = kLutBits;
-    dt.lut.assign((size_t)1 << K, 0);
-    for (uint32_t p = 0; p < (1u << K); p++) {
-        uint32_t node = 0, entry = 0;
-        for (uint32_t b = 0; b < K; b++) {
-            uint32_t bit = (p >> (K - 1 - b)) & 1;
-            uint32_t nx = dt.child[2 * node + bit];
-            if (nx == kNodeAbsent) {
-                entry = 0;
-                break;
+    // Multi-level tables.  fill() walks `bits` bits from `node` (`depth` bits
+    // already consumed) for every bit pattern: a leaf reached early is
+    // replicated over the unused low bits, a node still internal after all
+    // `bits` bits is queued and its entry patched once its own table exists.
+    // A full tree gives each internal node at most one table; the depth and
+    // size limits stop corrupt (cyclic or shared) node tables.
+    struct Pending {
+        uint32_t node, depth;
+        bool in_lut;
+        size_t slot;
+    };
+    std::vector<Pending> todo;
+    std::vector<uint32_t> tab_depth;  // bits consumed before each deeper table
+    auto fill = [&](uint32_t node, uint32_t depth, uint32_t bits, bool in_lut, size_t out_base) -> int {
+        for (uint32_t p = 0; p < (1u << bits); p++) {
+            uint32_t nd = node, entry = 0;
+            for (uint32_t b = 0; b < bits; b++) {
+                const uint32_t nx = dt.child[2 * nd + ((p >> (bits - 1 - b)) & 1)];
+                if (nx == kNodeAbsent) break;  // entry 0: invalid path
+                if (nx & kNodeLeaf) {
+                    const uint32_t len = depth + b + 1;
+                    if (len > 64) return VECSZ_ECORRUPT;
+                    entry = (len << 16) | (nx & 0xFFFF);
+                    break;
+                }
+                nd = nx;
+                if (b == bits - 1) {
+                    if (depth + bits >= 64) return VECSZ_ECORRUPT;
+                    entry = kLutSlow;
+                    todo.push_back({nd, depth + bits, in_lut, out_base + p});
+                }
             }
-            if (nx & kNodeLeaf) {
-                entry = ((b + 1) << 16) | (nx & 0xFFFF);
-                break;
-            }
-            node = nx;
-            if (b == K - 1) entry = kLutSlow | node;
+            (in_lut ? dt.lut : dt.sub)[out_base + p] = entry;
         }
-        dt.lut[p] = entry;
+        return VECSZ_OK;
+    };
+    dt.lut.assign((size_t)1 << kLutBits, 0);
+    dt.sub.clear();
+    int st = fill(0, 0, kLutBits, true, 0);
+    if (st) return st;
+    for (size_t i = 0; i < todo.size(); i++) {
+        const Pending pd = todo[i];
+        const size_t base = dt.sub.size();
+        if (base + (1u << kSubTabBits) > ((size_t)1 << 24)) return VECSZ_ECORRUPT;
+        dt.sub.resize(base + (1u << kSubTabBits));
+        (pd.in_lut ? dt.lut : dt.sub)[pd.slot] = kLutSlow | (uint32_t)base;
+        tab_depth.push_back(pd.depth);
+        st = fill(pd.node, pd.depth, kSubTabBits, false, base);
+        if (st) return st;
+    }
+
+    // compact 16-bit form
+    dt.lut16.clear();
+    dt.sub16.clear();
+    bool ok16 = tab_depth.size() <= 4096;
+    auto to16 = [&](uint32_t e, uint32_t depth, uint16_t& out) {
+        if (e == 0) return false;  // invalid path
+        if (e & kLutSlow) {
+            out = (uint16_t)((e & 0x7FFFFFFFu) >> kSubTabBits);
+            return true;
+        }
+        const uint32_t sym = e & 0xFFFF, used = ((e >> 16) & 0x7F) - depth;
+        if (sym >= 4096 || used == 0 || used > 15) return false;
+        out = (uint16_t)((used << 12) | sym);
+        return true;
+    };
+    if (ok16) {
+        dt.lut16.resize(dt.lut.size());
+        for (size_t i = 0; ok16 && i < dt.lut.size(); i++) ok16 = to16(dt.lut[i], 0, dt.lut16[i]);
+        dt.sub16.resize(dt.sub.size());
+        for (size_t i = 0; ok16 && i < dt.sub.size(); i++)
+            ok16 = to16(dt.sub[i], tab_depth[i >> kSubTabBits], dt.sub16[i]);
+    }
+    if (!ok16) {
+        dt.lut16.clear();
+        dt.sub16.clear();
     }
     return VECSZ_OK;
 }

@@ -26,7 +26,11 @@ struct DecodeTables {
     bool root_leaf = false;
     uint32_t root_sym = 0;
     std::vector<uint32_t> lut;    // 1 << kLutBits entries
-    std::vector<uint32_t> child;  // 2 * node_count entries
+    std::vector<uint32_t> sub;    // deeper tables, 1 << kSubTabBits entries each
+    std::vector<uint32_t> child;  // 2 * node_count entries (host-side walks)
+    // compact form of lut/sub (empty when a symbol or table number needs
+    // more than 12 bits or the tree has an invalid path)
+    std::vector<uint16_t> lut16, sub16;
 };
 
 // Parses the Huffman section of an archive (header + serialized tree) into

@@ -11,8 +11,10 @@ constexpr int kWave = 64;          // CDNA wavefront width
 constexpr int kChunk = 1024;       // symbols per encode chunk (one wavefront)
 constexpr int kSymPerThread = kChunk / kWave;  // 16 consecutive symbols per lane
 constexpr int kLutBits = 12;       // first-level decode table width
+constexpr int kSubTabBits = 4;     // width of each deeper decode table
 constexpr int kSubBits = 1024;     // bits per decode subsequence
-constexpr int kSyncRecord = 16;    // codeword boundaries recorded per subsequence
+constexpr int kCkptBits = 128;     // decoder checkpoint window
+constexpr int kCkpts = kSubBits / kCkptBits;
 
 // Error-bound family computed on the device from the value range
 // (ChangeToRelativeMode + InitializeErrorBoundFamily, src/types.cc:49-60,
@@ -62,11 +64,18 @@ struct DecSummary {
     uint64_t total_syms;  // symbols found by the sync pass
 };
 
-// Lookup-table entry of the decoder (kLutBits-bit window):
-//  bit 31 clear: leaf reached; bits 16..23 = code length (1..kLutBits), bits 0..15 = symbol
-//  bit 31 set  : bits 0..30 = index of the internal node reached after kLutBits bits
+// Decode table entry.  The first-level table is indexed by the next kLutBits
+// bits; an internal node reached after them points to a table of
+// 2^kSubTabBits entries indexed by the bits that follow, and so on.
+//  bit 31 clear: leaf; bits 16..22 = total code length (1..64), bits 0..15 = symbol
+//  bit 31 set  : bits 0..30 = first entry of the next-level table
 //  0           : invalid path (corrupt tree)
 constexpr uint32_t kLutSlow = 0x80000000u;
+// Compact tables for symbols below 4096: 16-bit entries, bits 0..11 a symbol
+// or a deeper table's number, bits 12..15 the bits consumed by a leaf in this
+// table (first level: total length 1..12; deeper: 1..kSubTabBits), 0 for a
+// pointer to the next table.
+constexpr uint32_t kMaxSub16 = 12288;  // deeper-table entries kept in LDS
 // Node table entry for the bit-serial slow path: child[2*i + bit]
 //  bit 31 set: leaf, bits 0..15 symbol; otherwise internal node index; 0xFFFFFFFF = absent.
 constexpr uint32_t kNodeLeaf = 0x80000000u;

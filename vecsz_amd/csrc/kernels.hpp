@@ -28,26 +28,25 @@ struct DecodeArgs {
     const uint8_t* stream;   // first byte of the bitstream (any alignment)
     uint64_t stream_bits;    // 8 * stream bytes
     uint64_t n;              // symbols to decode
-    const uint32_t* lut;     // 1 << kLutBits entries
-    const uint32_t* child;   // 2 * node_count entries
+    const uint32_t* lut;     // first-level table, 1 << kLutBits entries
+    const uint32_t* sub;     // deeper tables
+    const uint16_t* lut16;   // compact tables (nullptr: use lut / sub)
+    const uint16_t* sub16;
+    uint32_t n_sub16;        // entries of sub16
     uint64_t nsub;           // subsequences of kSubBits bits
     const uint8_t* outliers; // outlier section of the archive (any alignment)
     uint64_t n_outlier;
 };
 
 struct SubState {
-    uint64_t* start;    // true (or assumed) start bit of each subsequence
-    uint64_t* end;      // end bit reached by its decode
-    uint32_t* cnt;      // symbols whose codeword starts in [start, end)
-    uint32_t* zeros;    // of which are symbol 0
-    uint16_t* bnd;      // first kSyncRecord codeword starts (offset from start), per subsequence
-    uint16_t* zmask;    // symbol-0 flags of those codewords
-    uint8_t* nbnd;      // recorded boundary count
-    uint8_t* dirty;     // needs a re-decode from end[j-1]
+    uint64_t* b;      // true start bit of each subsequence (first codeword start >= j * kSubBits)
+    uint64_t* e;      // its end: first codeword start >= (j + 1) * kSubBits
+    uint32_t* cnt;    // codewords starting in [b, e)
+    uint32_t* zer;    // of which decode to symbol 0
+    uint8_t* dirty;   // needs a re-decode from e[j - 1] (2 * nsub: current and next round)
 };
 
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
-void launch_decode_check(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
 void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
                         uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st);
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,

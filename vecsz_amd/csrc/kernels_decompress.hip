@@ -2,15 +2,20 @@
 //
 //   Huffman decode (decode_withTree/decode, src/huffman.hh:343-372, :679-691)
 //   as a chunk-parallel, self-synchronising decode of the archive's single
-//   MSB-first bitstream:
-//     k_decode_sync   each thread decodes kSubBits bits from an assumed
-//                     codeword start and records its first codeword starts
-//     k_decode_check  a subsequence is synchronised when the previous one's end
-//                     is among its recorded starts (then every later codeword
-//                     agrees); counts are corrected, the rest marked dirty
-//     k_decode_redo   dirty subsequences re-decode from the previous end until
-//                     no end moves
-//     k_decode_write  final decode: quant codes + outlier scatter
+//   MSB-first bitstream, cut into subsequences of kSubBits bits:
+//     k_decode_sync   a workgroup stages 256 subsequences in LDS; each thread
+//                     decodes its own from the assumed start j*kSubBits and
+//                     records checkpoints (first codeword start in every
+//                     kCkptBits window, with the symbol and zero counts before
+//                     it); then it continues from its own end into the next
+//                     subsequence until a codeword start meets a checkpoint,
+//                     after which both decodes agree, and writes the next
+//                     subsequence's true start, end and counts
+//     k_decode_redo   subsequences whose predecessor's continuation found no
+//                     meeting point re-decode from the true start, in rounds,
+//                     until no end moves
+//     k_decode_write  final decode from the true starts: quant codes in
+//                     16-byte stores + outlier scatter
 //                     (src/vecsz_interface.hh:313-317)
 //   Lorenzo reconstruction (x_lorenzo_{1,2,3}d1l, src/dualquant.hh:915-1043):
 //     k_recon_*       integer segmented scans per Lorenzo block; exact, and
@@ -27,149 +32,383 @@ namespace vsz {
 
 __device__ __forceinline__ uint32_t lane_id2() { return __lane_id(); }
 
-struct BitReader {
-    const uint32_t* base4;
-    uint64_t shift0;
-    uint64_t last_word;
-    uint64_t buf;
-    int nb;
-    uint64_t next;
+// The bitstream in "aligned space": word w is the big-endian u32 at byte 4w
+// of the 16-byte aligned address at or before the stream start; stream bit p
+// is aligned-space bit p + shift0.  Bytes before the stream belong to the
+// archive header; bytes at or past the stream end are never read and count
+// as zero.
+struct StreamView {
+    const uint8_t* base;
+    uint32_t shift0;
+    uint64_t nbytes;  // readable bytes from base (through the stream end)
 
     __device__ __forceinline__ void setup(const uint8_t* stream, uint64_t stream_bits) {
-        uintptr_t p = reinterpret_cast<uintptr_t>(stream);
-        base4 = reinterpret_cast<const uint32_t*>(p & ~(uintptr_t)3);
-        shift0 = 8 * (p & 3);
-        last_word = stream_bits ? (shift0 + stream_bits - 1) >> 5 : 0;
+        // pointer arithmetic (not an integer round trip) keeps the global
+        // address space visible to the compiler: global_load, not flat_load
+        const uint32_t mis = (uint32_t)(reinterpret_cast<uintptr_t>(stream) & 15);
+        base = stream - mis;
+        shift0 = 8 * mis;
+        nbytes = mis + stream_bits / 8;
     }
-    __device__ __forceinline__ uint32_t load(uint64_t w) const {
-        return w <= last_word ? __builtin_bswap32(base4[w]) : 0u;
-    }
-    __device__ __forceinline__ void refill() {
-        if (nb <= 32) {
-            buf |= (uint64_t)load(next++) << (32 - nb);
-            nb += 32;
-        }
-    }
-    __device__ __forceinline__ void init(uint64_t pos) {
-        const uint64_t a = pos + shift0;
-        const uint64_t w = a >> 5;
-        const int o = (int)(a & 31);
-        buf = ((uint64_t)load(w) << 32) | load(w + 1);
-        buf <<= o;
-        nb = 64 - o;
-        next = w + 2;
-        refill();
-    }
-    __device__ __forceinline__ void consume(int l) {
-        buf <<= l;
-        nb -= l;
-        refill();
+    __device__ __forceinline__ uint32_t word(uint64_t w) const {
+        const uint64_t b = 4 * w;
+        if (b + 4 <= nbytes) return __builtin_bswap32(reinterpret_cast<const uint32_t*>(base)[w]);
+        uint32_t v = 0;
+        for (int i = 0; i < 4; i++) v = (v << 8) | (b + i < nbytes ? base[b + i] : 0u);
+        return v;
     }
 };
 
-// One codeword: LDS table for the first kLutBits bits, bit-serial walk of the
-// node table beyond.  The tree was checked to be full on the host, so every
-// walk ends at a leaf; `maxwalk` bounds it anyway.
-__device__ __forceinline__ uint32_t decode_one(BitReader& r, const uint32_t* __restrict__ lut,
-                                               const uint32_t* __restrict__ child, uint32_t& len, bool& bad) {
-    const uint32_t e = lut[r.buf >> (64 - kLutBits)];
-    if (!(e & kLutSlow)) {
-        len = (e >> 16) & 0xFF;
-        if (len == 0) {
-            bad = true;
-            len = 1;
-            r.consume(1);
-            return 0;
-        }
-        r.consume((int)len);
-        return e & 0xFFFF;
-    }
-    uint32_t node = e & 0x7FFFFFFFu;
-    r.consume(kLutBits);
-    len = kLutBits;
-    for (int guard = 0; guard < 65536; guard++) {
-        const uint32_t bit = (uint32_t)(r.buf >> 63);
-        r.consume(1);
-        len++;
-        const uint32_t nx = child[2 * node + bit];
-        if (nx == kNodeAbsent) break;
-        if (nx & kNodeLeaf) return nx & 0xFFFF;
-        node = nx;
-    }
-    bad = true;
-    return 0;
-}
+// Staged words are stored with one pad word after every 32, so the threads of
+// a wavefront, each at its own 1024-bit subsequence (32 words apart), read
+// different LDS banks.
+__device__ __forceinline__ uint32_t pad_idx(uint32_t w) { return w + (w >> 5); }
 
-__device__ __forceinline__ void load_lut(uint32_t* slut, const uint32_t* lut) {
-    for (int i = threadIdx.x; i < (1 << kLutBits); i += blockDim.x) slut[i] = lut[i];
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, DecSummary* sum) {
-    __shared__ uint32_t slut[1 << kLutBits];
-    __shared__ uint16_t sb[256 * kSyncRecord];
-    load_lut(slut, a.lut);
-    const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (j >= a.nsub) return;
-    BitReader r;
-    r.setup(a.stream, a.stream_bits);
-    const uint64_t start = j * kSubBits;
-    const uint64_t target = min(start + (uint64_t)kSubBits, a.stream_bits);
-    r.init(start);
-    uint64_t pos = start;
-    uint32_t cnt = 0, z = 0, zm = 0;
-    bool bad = false;
-    uint16_t* mine = sb + threadIdx.x * kSyncRecord;
-    while (pos < target) {
-        if (cnt < kSyncRecord) mine[cnt] = (uint16_t)(pos - start);
-        uint32_t len;
-        const uint32_t sym = decode_one(r, slut, a.child, len, bad);
-        if (sym == 0) {
-            z++;
-            if (cnt < kSyncRecord) zm |= 1u << cnt;
-        }
-        pos += len;
-        cnt++;
+// 64-bit window whose MSB is bit `lp` of the staged words (>= 33 valid bits).
+struct LdsWin {
+    const uint32_t* st;
+    __device__ __forceinline__ uint64_t operator()(uint32_t lp) const {
+        const uint32_t w = lp >> 5;
+        const uint32_t i0 = pad_idx(w);
+        const uint32_t i1 = i0 + 1 + ((w & 31) == 31);
+        return ((((uint64_t)st[i0]) << 32) | st[i1]) << (lp & 31);
     }
-    if (bad) atomicOr(&sum->error, 1u);
-    s.start[j] = start;
-    s.end[j] = pos;
-    s.cnt[j] = cnt;
-    s.zeros[j] = z;
-    s.zmask[j] = (uint16_t)zm;
-    const uint32_t nb = min(cnt, (uint32_t)kSyncRecord);
-    s.nbnd[j] = (uint8_t)nb;
-    for (uint32_t i = 0; i < nb; i++) s.bnd[j * kSyncRecord + i] = mine[i];
-}
+};
 
-__global__ void __launch_bounds__(256) k_decode_check(DecodeArgs a, SubState s, DecSummary* sum) {
-    const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (j >= a.nsub) return;
-    s.dirty[j] = 0;
-    s.dirty[a.nsub + j] = 0;
-    if (j == 0) return;
-    const uint64_t e = s.end[j - 1];
-    const uint64_t st = s.start[j];
-    if (e == st) return;
-    const uint64_t off = e - st;
-    const uint32_t nb = s.nbnd[j];
-    for (uint32_t i = 1; i < nb; i++) {
-        if ((uint64_t)s.bnd[j * kSyncRecord + i] == off) {
-            s.start[j] = e;
-            s.cnt[j] -= i;
-            s.zeros[j] -= __popc((uint32_t)s.zmask[j] & ((1u << i) - 1u));
+// Register bit buffer over the staged words: `buf` holds the next nb >= 33
+// bits MSB first and the following word is already loaded in `nxt`, so the
+// decode chain waits on the table lookup only.
+struct LdsBits {
+    const uint32_t* st;
+    uint64_t buf;
+    uint32_t nb, wi, nxt, pos;  // pos: local bit index of buf's MSB
+
+    __device__ __forceinline__ void init(const uint32_t* s, uint32_t lp) {
+        st = s;
+        pos = lp;
+        const uint32_t w = lp >> 5;
+        buf = ((((uint64_t)st[pad_idx(w)]) << 32) | st[pad_idx(w + 1)]) << (lp & 31);
+        nb = 64 - (lp & 31);
+        wi = w + 2;
+        nxt = st[pad_idx(wi)];
+    }
+    __device__ __forceinline__ void consume(uint32_t l) {
+        if (l >= nb) {  // a codeword longer than the buffer (> 32 bits): restart behind it
+            init(st, pos + l);
             return;
         }
+        pos += l;
+        buf <<= l;
+        nb -= l;
+        if (nb <= 32) {
+            buf |= (uint64_t)nxt << (32 - nb);
+            nb += 32;
+            wi++;
+            nxt = st[pad_idx(wi)];
+        }
     }
-    s.dirty[j] = 1;
-    atomicAdd(&sum->n_changed, 1u);
+};
+
+// The same from global memory, at stream bit p.
+struct GlobalWin {
+    StreamView sv;
+    __device__ __forceinline__ uint64_t operator()(uint64_t p) const {
+        const uint64_t a = p + sv.shift0;
+        const uint64_t w = a >> 5;
+        return ((((uint64_t)sv.word(w)) << 32) | sv.word(w + 1)) << (a & 31);
+    }
+};
+
+// One codeword at position p, 32-bit tables: first level in LDS, deeper
+// levels (codes longer than kLutBits) from the global sub-tables.  len = 0
+// marks an invalid path.
+template <typename Win, typename P>
+__device__ __forceinline__ uint32_t decode_sym(const Win& win, P p, const uint32_t* lut,
+                                               const uint32_t* __restrict__ sub, uint32_t& len) {
+    uint32_t e = lut[(uint32_t)(win(p) >> (64 - kLutBits))];
+    if (e & kLutSlow) {
+        P q = p + kLutBits;
+        for (int lv = 0; lv < 16 && (e & kLutSlow); lv++) {
+            e = sub[(e & 0x7FFFFFFFu) + (uint32_t)(win(q) >> (64 - kSubTabBits))];
+            q += kSubTabBits;
+        }
+    }
+    len = (e & kLutSlow) ? 0u : (e >> 16) & 0x7F;
+    return e & 0xFFFF;
 }
 
-// One round of re-decoding: dirty_cur -> dirty_nxt.
+// The same with the compact 16-bit tables, every level in LDS.
+template <typename Win, typename P>
+__device__ __forceinline__ uint32_t decode_sym16(const Win& win, P p, const uint16_t* lut, const uint16_t* sub,
+                                                 uint32_t& len) {
+    uint32_t e = lut[(uint32_t)(win(p) >> (64 - kLutBits))];
+    len = e >> 12;
+    if (len == 0) {
+        P q = p + kLutBits;
+        for (int lv = 0; lv < 16; lv++) {
+            e = sub[((e & 0xFFF) << kSubTabBits) + (uint32_t)(win(q) >> (64 - kSubTabBits))];
+            if (e >> 12) {
+                len = (uint32_t)(q - p) + (e >> 12);
+                break;
+            }
+            q += kSubTabBits;
+        }
+    }
+    return e & 0xFFF;
+}
+
+// Decode tables as a kernel sees them; load() copies what lives in LDS to
+// the front of the dynamic LDS area and returns the words it used.
+template <bool T16>
+struct Decoder;
+
+template <>
+struct Decoder<true> {
+    const uint16_t* lut;
+    const uint16_t* sub;
+    __device__ __forceinline__ uint32_t load(uint32_t* lds, const DecodeArgs& a) {
+        uint16_t* l = reinterpret_cast<uint16_t*>(lds);
+        const uint32_t nl = 1u << kLutBits, ns = a.n_sub16;
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) l[i] = a.lut16[i];
+        for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) l[nl + i] = a.sub16[i];
+        lut = l;
+        sub = l + nl;
+        return (nl + ns + 1) / 2;
+    }
+    template <typename Win, typename P>
+    __device__ __forceinline__ uint32_t operator()(const Win& w, P p, uint32_t& len) const {
+        return decode_sym16(w, p, lut, sub, len);
+    }
+    // Decodes and consumes the codeword at the head of bb; len = 0: invalid path.
+    __device__ __forceinline__ uint32_t take(LdsBits& bb, uint32_t& len) const {
+        uint32_t e = lut[(uint32_t)(bb.buf >> (64 - kLutBits))];
+        uint32_t l = e >> 12;
+        if (l == 0) {
+            uint32_t used = kLutBits;
+            for (int lv = 0; lv < 16; lv++) {
+                const uint32_t idx = used + kSubTabBits <= bb.nb
+                                         ? (uint32_t)((bb.buf << used) >> (64 - kSubTabBits))
+                                         : (uint32_t)(LdsWin{bb.st}(bb.pos + used) >> (64 - kSubTabBits));
+                e = sub[((e & 0xFFF) << kSubTabBits) + idx];
+                if (e >> 12) {
+                    l = used + (e >> 12);
+                    break;
+                }
+                used += kSubTabBits;
+            }
+        }
+        len = l;
+        bb.consume(l ? l : 1);
+        return e & 0xFFF;
+    }
+};
+
+template <>
+struct Decoder<false> {
+    const uint32_t* lut;
+    const uint32_t* sub;
+    __device__ __forceinline__ uint32_t load(uint32_t* lds, const DecodeArgs& a) {
+        const uint32_t nl = 1u << kLutBits;
+        for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) lds[i] = a.lut[i];
+        lut = lds;
+        sub = a.sub;
+        return nl;
+    }
+    template <typename Win, typename P>
+    __device__ __forceinline__ uint32_t operator()(const Win& w, P p, uint32_t& len) const {
+        return decode_sym(w, p, lut, sub, len);
+    }
+    __device__ __forceinline__ uint32_t take(LdsBits& bb, uint32_t& len) const {
+        uint32_t e = lut[(uint32_t)(bb.buf >> (64 - kLutBits))];
+        if (e & kLutSlow) {
+            uint32_t used = kLutBits;
+            for (int lv = 0; lv < 16 && (e & kLutSlow); lv++) {
+                const uint32_t idx = used + kSubTabBits <= bb.nb
+                                         ? (uint32_t)((bb.buf << used) >> (64 - kSubTabBits))
+                                         : (uint32_t)(LdsWin{bb.st}(bb.pos + used) >> (64 - kSubTabBits));
+                e = sub[(e & 0x7FFFFFFFu) + idx];
+                used += kSubTabBits;
+            }
+        }
+        len = (e & kLutSlow) ? 0u : (e >> 16) & 0x7F;
+        bb.consume(len ? len : 1);
+        return e & 0xFFFF;
+    }
+};
+
+constexpr int kSyncWg = 256;                                     // subsequences decoded per workgroup
+constexpr int kSyncStride = kSyncWg - 1;                         // ... whose results it writes
+constexpr uint32_t kStageWords = kSyncWg * (kSubBits / 32) + 12;  // + shift0 and the tail read by the last codewords
+constexpr uint32_t kStageLds = kStageWords + kStageWords / 32 + 1;
+constexpr int kCkStride = kCkpts + 1;                            // odd row stride: one bank per thread
+constexpr uint32_t kNoCkpt = 0xFF;                               // window without a codeword start
+
+// Stages NW aligned-space words starting at w0 (a multiple of 4) into the
+// padded layout.  Every thread issues all of its 16-byte loads before the
+// first LDS write, so the workgroup waits for one memory latency, not one per
+// word.
+template <uint32_t NW>
+__device__ __forceinline__ void stage_words(uint32_t* st, const StreamView& sv, uint64_t w0) {
+    constexpr uint32_t kVec = (NW + 4 * kSyncWg - 1) / (4 * kSyncWg);  // uint4 per thread
+    uint4 v[kVec];
+    const uint4* src = reinterpret_cast<const uint4*>(sv.base) + w0 / 4;
+    if (4 * (w0 + kVec * 4 * kSyncWg) <= sv.nbytes) {
+        // whole range inside the stream: unconditional loads, all in flight together
+#pragma unroll
+        for (uint32_t m = 0; m < kVec; m++) v[m] = src[threadIdx.x + m * kSyncWg];
+#pragma unroll
+        for (uint32_t m = 0; m < kVec; m++)
+            v[m] = make_uint4(__builtin_bswap32(v[m].x), __builtin_bswap32(v[m].y), __builtin_bswap32(v[m].z),
+                              __builtin_bswap32(v[m].w));
+    } else {
+#pragma unroll
+        for (uint32_t m = 0; m < kVec; m++) {
+            const uint64_t w = w0 + 4 * (threadIdx.x + m * kSyncWg);
+            v[m] = make_uint4(sv.word(w), sv.word(w + 1), sv.word(w + 2), sv.word(w + 3));
+        }
+    }
+#pragma unroll
+    for (uint32_t m = 0; m < kVec; m++) {
+        const uint32_t k = 4 * (threadIdx.x + m * kSyncWg);
+        if (k < NW) {
+            // four consecutive words never straddle a 32-word row
+            uint32_t* d = st + pad_idx(k);
+            d[0] = v[m].x;
+            d[1] = v[m].y;
+            d[2] = v[m].z;
+            d[3] = v[m].w;
+        }
+    }
+}
+
+// Checkpoint word: bits 0..7 offset of the first codeword start in the
+// window (kNoCkpt: none), 8..18 codewords before it, 19..29 zeros before it.
+__device__ __forceinline__ uint32_t ckpt(uint32_t off, uint32_t cnt, uint32_t z) {
+    return off | (cnt << 8) | (z << 19);
+}
+
+// Dynamic LDS words of the sync kernel after the tables.
+constexpr uint32_t kSyncLdsWords = kStageLds + kSyncWg * kCkStride + 3 * kSyncWg;
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, DecSummary* sum) {
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    uint32_t* st = lds + dec.load(lds, a);
+    uint32_t* ck = st + kStageLds;
+    uint32_t* own_cnt = ck + kSyncWg * kCkStride;
+    uint32_t* own_z = own_cnt + kSyncWg;
+    uint32_t* own_e = own_z + kSyncWg;
+    StreamView sv;
+    sv.setup(a.stream, a.stream_bits);
+    const uint64_t J0 = blockIdx.x * (uint64_t)kSyncStride;
+    stage_words<kStageWords>(st, sv, J0 * (kSubBits / 32));
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    const uint64_t j = J0 + t;
+    // local position of stream bit p: p - J0*kSubBits + shift0
+    auto local = [&](uint64_t p) -> uint32_t { return (uint32_t)(p - J0 * kSubBits) + sv.shift0; };
+    bool bad = false;
+
+    // 1. own decode from the assumed start, with checkpoints
+    if (j < a.nsub) {
+        const uint32_t s0 = t * kSubBits + sv.shift0;
+        const uint32_t e0 = local(min((j + 1) * (uint64_t)kSubBits, a.stream_bits));
+#pragma unroll
+        for (int k = 0; k < kCkpts; k++) ck[t * kCkStride + k] = kNoCkpt;
+        LdsBits bb;
+        bb.init(st, s0);
+        uint32_t cnt = 0, z = 0, last_k = 0xFFFFFFFFu;
+        while (bb.pos < e0) {
+            const uint32_t rel = bb.pos - s0;
+            if (rel / kCkptBits != last_k) {
+                last_k = rel / kCkptBits;
+                ck[t * kCkStride + last_k] = ckpt(rel % kCkptBits, cnt, z);
+            }
+            uint32_t len;
+            const uint32_t sym = dec.take(bb, len);
+            bad |= len == 0;
+            cnt++;
+            z += sym == 0;
+        }
+        const uint32_t lp = bb.pos;
+        own_cnt[t] = cnt;
+        own_z[t] = z;
+        own_e[t] = lp;
+        if (j == 0) {  // the stream's first subsequence starts at a true boundary
+            s.b[0] = 0;
+            s.e[0] = (uint64_t)lp - sv.shift0;
+            s.cnt[0] = cnt;
+            s.zer[0] = z;
+        }
+    }
+    __syncthreads();
+
+    // 2. continuation into subsequence j + 1 from this thread's own end
+    if (t < kSyncStride && j + 1 < a.nsub) {
+        const uint32_t u = t + 1;
+        const uint32_t su = u * kSubBits + sv.shift0;
+        const uint32_t eu = local(min((j + 2) * (uint64_t)kSubBits, a.stream_bits));
+        const uint32_t q0 = own_e[t];
+        uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0;
+        bool met = false;
+        LdsBits bb;
+        bb.init(st, q0);
+        for (;;) {
+            const uint32_t q = bb.pos;
+            if (q >= eu) {
+                if (q == own_e[u]) {
+                    met = true;
+                    before_c = own_cnt[u];
+                    before_z = own_z[u];
+                }
+                break;
+            }
+            const uint32_t rel = q - su;
+            const uint32_t e = ck[u * kCkStride + rel / kCkptBits];
+            if ((e & 0xFF) == rel % kCkptBits) {
+                met = true;
+                before_c = (e >> 8) & 0x7FF;
+                before_z = e >> 19;
+                break;
+            }
+            uint32_t len;
+            const uint32_t sym = dec.take(bb, len);
+            bad |= len == 0;
+            cc++;
+            cz += sym == 0;
+        }
+        const uint32_t q = bb.pos;
+        const uint64_t base = J0 * kSubBits - sv.shift0;
+        s.b[j + 1] = base + q0;
+        if (met) {
+            s.cnt[j + 1] = cc + own_cnt[u] - before_c;
+            s.zer[j + 1] = cz + own_z[u] - before_z;
+            s.e[j + 1] = base + own_e[u];
+        } else {
+            // the continuation decoded all of j + 1 from its true start; the
+            // check of j + 2 started from j + 1's own (different) end
+            s.cnt[j + 1] = cc;
+            s.zer[j + 1] = cz;
+            s.e[j + 1] = base + q;
+            if (j + 2 < a.nsub) {
+                s.dirty[j + 2] = 1;
+                atomicAdd(&sum->n_changed, 1u);
+            }
+        }
+    }
+    if (bad) atomicOr(&sum->error, 1u);
+}
+
+// One round of re-decoding from the previous subsequence's true end:
+// dirty_cur -> dirty_nxt, end_in -> end_out.
+template <bool T16>
 __global__ void __launch_bounds__(256) k_decode_redo(DecodeArgs a, SubState s, const uint64_t* end_in, uint64_t* end_out,
                                                      uint8_t* dirty_cur, uint8_t* dirty_nxt, DecSummary* sum) {
-    __shared__ uint32_t slut[1 << kLutBits];
-    load_lut(slut, a.lut);
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    dec.load(lds, a);
+    __syncthreads();
     const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (j >= a.nsub) return;
     if (!dirty_cur[j]) {
@@ -177,25 +416,28 @@ __global__ void __launch_bounds__(256) k_decode_redo(DecodeArgs a, SubState s, c
         return;
     }
     dirty_cur[j] = 0;
-    BitReader r;
-    r.setup(a.stream, a.stream_bits);
+    GlobalWin win;
+    win.sv.setup(a.stream, a.stream_bits);
     const uint64_t st = end_in[j - 1];
     const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
-    r.init(st);
     uint64_t pos = st;
     uint32_t cnt = 0, z = 0;
     bool bad = false;
     while (pos < target) {
         uint32_t len;
-        const uint32_t sym = decode_one(r, slut, a.child, len, bad);
+        const uint32_t sym = dec(win, pos, len);
+        if (len == 0) {
+            bad = true;
+            len = 1;
+        }
         z += sym == 0;
         pos += len;
         cnt++;
     }
     if (bad) atomicOr(&sum->error, 1u);
-    s.start[j] = st;
+    s.b[j] = st;
     s.cnt[j] = cnt;
-    s.zeros[j] = z;
+    s.zer[j] = z;
     end_out[j] = pos;
     if (pos != end_in[j] && j + 1 < a.nsub) {
         dirty_nxt[j + 1] = 1;
@@ -292,31 +534,64 @@ __device__ __forceinline__ float read_f32_bytes(const uint8_t* p) {
     return __uint_as_float(v);
 }
 
+// Final decode.  Thread j decodes cnt[j] codewords from its true start b[j]
+// out of the workgroup's staged bits; its symbols [O, O + cnt) go out as
+// single u16 stores up to an 8-symbol boundary, then 16-byte stores.
+template <bool T16>
 __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, const uint64_t* __restrict__ sym_off,
                                                       const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, DecSummary* sum) {
-    __shared__ uint32_t slut[1 << kLutBits];
-    load_lut(slut, a.lut);
-    const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    uint32_t* st = lds + dec.load(lds, a);
+    StreamView sv;
+    sv.setup(a.stream, a.stream_bits);
+    const uint64_t J0 = blockIdx.x * (uint64_t)kSyncWg;
+    stage_words<kStageWords>(st, sv, J0 * (kSubBits / 32));
+    __syncthreads();
+    const uint64_t j = J0 + threadIdx.x;
     if (j >= a.nsub) return;
-    BitReader r;
-    r.setup(a.stream, a.stream_bits);
-    r.init(s.start[j]);
-    const uint32_t c = s.cnt[j];
-    uint64_t idx = sym_off[j];
+    LdsBits bb;
+    bb.init(st, (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0);
+    const uint64_t O = sym_off[j];
+    const uint64_t iend = min(O + s.cnt[j], a.n);
     uint64_t rank = zero_off[j];
     bool bad = false;
-    for (uint32_t i = 0; i < c && idx < a.n; i++, idx++) {
+    auto next = [&]() -> uint32_t {
         uint32_t len;
-        const uint32_t sym = decode_one(r, slut, a.child, len, bad);
-        codes[idx] = (uint16_t)sym;
-        if (sym == 0) {
-            if (rank < a.n_outlier)
-                sparse[idx] = read_f32_bytes(a.outliers + 4 * rank);
-            else
-                bad = true;
-            rank++;
+        const uint32_t sym = dec.take(bb, len);
+        bad |= len == 0;
+        return sym;
+    };
+    auto outlier = [&](uint64_t i) {
+        if (rank < a.n_outlier)
+            sparse[i] = read_f32_bytes(a.outliers + 4 * rank);
+        else
+            bad = true;
+        rank++;
+    };
+    uint64_t i = O;
+    for (; i < iend && (i & 7); i++) {
+        const uint32_t sym = next();
+        codes[i] = (uint16_t)sym;
+        if (sym == 0) outlier(i);
+    }
+    for (; i + 8 <= iend; i += 8) {
+        uint32_t w[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+            const uint32_t lo = next();
+            if (lo == 0) outlier(i + 2 * m);
+            const uint32_t hi = next();
+            if (hi == 0) outlier(i + 2 * m + 1);
+            w[m] = lo | (hi << 16);
         }
+        *reinterpret_cast<uint4*>(codes + i) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    for (; i < iend; i++) {
+        const uint32_t sym = next();
+        codes[i] = (uint16_t)sym;
+        if (sym == 0) outlier(i);
     }
     if (bad) atomicOr(&sum->error, 1u);
 }
@@ -573,16 +848,45 @@ __global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const ui
 
 static unsigned grid_for(uint64_t threads, unsigned block) { return (unsigned)((threads + block - 1) / block); }
 
-void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
-    hipLaunchKernelGGL(k_decode_sync, dim3(grid_for(a.nsub, 256)), dim3(256), 0, st, a, s, sum);
+// Dynamic LDS bytes of the decode tables a kernel keeps in LDS.
+static size_t table_lds_bytes(const DecodeArgs& a, bool t16) {
+    return t16 ? (((size_t)(1u << kLutBits) + a.n_sub16 + 1) / 2) * 4 : (size_t)(1u << kLutBits) * 4;
 }
-void launch_decode_check(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
-    hipLaunchKernelGGL(k_decode_check, dim3(grid_for(a.nsub, 256)), dim3(256), 0, st, a, s, sum);
+
+// Lets a kernel take more than the default 64 KiB of dynamic LDS (gfx950
+// has 160 KiB per workgroup); set once per kernel.
+template <typename K>
+static void allow_big_lds(K kernel) {
+    static bool done = false;
+    if (!done) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
+        done = true;
+    }
+}
+
+void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
+    allow_big_lds(k_decode_sync<true>);
+    allow_big_lds(k_decode_sync<false>);
+    const bool t16 = a.lut16 != nullptr;
+    const size_t lds = table_lds_bytes(a, t16) + (size_t)kSyncLdsWords * 4;
+    const dim3 grid(grid_for(a.nsub, kSyncStride));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_sync<true>, grid, dim3(kSyncWg), lds, st, a, s, sum);
+    else
+        hipLaunchKernelGGL(k_decode_sync<false>, grid, dim3(kSyncWg), lds, st, a, s, sum);
 }
 void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
                         uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st) {
-    hipLaunchKernelGGL(k_decode_redo, dim3(grid_for(a.nsub, 256)), dim3(256), 0, st, a, s, end_in, end_out, dirty_cur,
-                       dirty_nxt, sum);
+    const bool t16 = a.lut16 != nullptr;
+    const size_t lds = table_lds_bytes(a, t16);
+    const dim3 grid(grid_for(a.nsub, 256));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_redo<true>, grid, dim3(256), lds, st, a, s, end_in, end_out, dirty_cur, dirty_nxt,
+                           sum);
+    else
+        hipLaunchKernelGGL(k_decode_redo<false>, grid, dim3(256), lds, st, a, s, end_in, end_out, dirty_cur, dirty_nxt,
+                           sum);
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {
@@ -593,8 +897,17 @@ void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_
 }
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
                          uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st) {
-    hipLaunchKernelGGL(k_decode_write, dim3(grid_for(a.nsub, 256)), dim3(256), 0, st, a, s, sym_off, zero_off, codes,
-                       sparse, sum);
+    allow_big_lds(k_decode_write<true>);
+    allow_big_lds(k_decode_write<false>);
+    const bool t16 = a.lut16 != nullptr;
+    const size_t lds = table_lds_bytes(a, t16) + (size_t)kStageLds * 4;
+    const dim3 grid(grid_for(a.nsub, kSyncWg));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_write<true>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
+                           sum);
+    else
+        hipLaunchKernelGGL(k_decode_write<false>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
+                           sum);
 }
 void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st) {
     unsigned g = (unsigned)std::min<uint64_t>(grid_for(n, 256), 8192);
