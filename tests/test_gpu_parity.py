@@ -137,6 +137,20 @@ def test_negative_zero_outliers(gpu_ctx, shape):
     _check_vs_oracle(gpu_ctx, x, 0.5, False, tag=f"-0 outliers {shape}")
 
 
+@pytest.mark.parametrize("shape,blocks", [((24, 36, 68), (8, 16)), ((50, 264), (8, 16, 32, 64)),
+                                          ((4100,), (8, 16, 32, 64))])
+def test_four_per_lane_reconstruction(gpu_ctx, shape, blocks):
+    """Rows that are multiples of 4 take the four-per-lane reconstruction;
+    spikes put outliers (scan restarts) at every position inside a lane."""
+    rng = np.random.default_rng(len(shape))
+    x = np.cumsum(rng.normal(0, 0.05, int(np.prod(shape)))).astype(np.float32)
+    x[rng.integers(0, x.size, x.size // 50)] += rng.choice([-1, 1], x.size // 50) * 500.0
+    x = x.reshape(shape)
+    for B in blocks:
+        _check_vs_oracle(gpu_ctx, x, 1e-3, False, block=B, tag=f"x4 B{B}{shape}")
+        _check_vs_oracle(gpu_ctx, x, 1e-4, True, block=B, tag=f"x4 r2r B{B}{shape}")
+
+
 def test_c1_sinusoids_full_size(gpu_ctx):
     """2D 1800x3600 abs 1e-4, Huffman only: bit-exact archive."""
     from vecsz_amd import datagen as dg

@@ -844,6 +844,123 @@ __global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const ui
     }
 }
 
+// Four consecutive x per lane (8-byte code loads, 16-byte stores).  A block
+// row of B elements spans B/4 lanes: each lane runs the Lorenzo prefix over
+// its four elements sequentially, then a segmented scan over the lanes of the
+// row carries the running value in, up to the lane's first restart (an
+// outlier or the block start).
+template <int B>
+__device__ __forceinline__ void row_scan4(int32_t (&D)[4], const int32_t (&g)[4], const bool (&rs)[4]) {
+    constexpr int L = B / 4;
+    D[0] = g[0];
+#pragma unroll
+    for (int i = 1; i < 4; i++) D[i] = rs[i] ? g[i] : D[i - 1] + g[i];
+    if constexpr (L > 1) {
+        const uint32_t lib = lane_id2() % L;
+        int32_t T = D[3];
+        int f = rs[0] | rs[1] | rs[2] | rs[3];
+#pragma unroll
+        for (int d = 1; d < L; d <<= 1) {
+            const int32_t t2 = __shfl_up(T, d, L);
+            const int f2 = __shfl_up(f, d, L);
+            if (lib >= (uint32_t)d) {
+                if (!f) T += t2;
+                f |= f2;
+            }
+        }
+        int32_t carry = __shfl_up(T, 1, L);
+        if (lib == 0) carry = 0;
+        bool open = true;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            open = open && !rs[i];
+            if (open) D[i] += carry;
+        }
+    }
+}
+
+template <int ND, int B>
+__global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, ReconArgs a, uint32_t* flagged,
+                                                  uint32_t* nflagged) {
+    constexpr int L = B / 4;
+    constexpr int YB = ND >= 2 ? B : 1;
+    constexpr int ZB = ND >= 3 ? B : 1;
+    const uint32_t lane = lane_id2();
+    const uint64_t ntx = (a.d0 + 255) / 256;
+    const uint64_t nty = ND >= 2 ? a.ntiles_y : 1, ntz = ND >= 3 ? a.ntiles_z : 1;
+    const uint64_t ntiles = ntx * nty * ntz;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    const uint64_t sy = a.d0, sz = a.d0 * a.d1;
+    const uint64_t nbx = (a.d0 + B - 1) / B, nby = ND >= 2 ? (a.d1 + B - 1) / B : 1;
+    const uint32_t R = (uint32_t)a.radius;
+    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+        const uint64_t tx = t % ntx;
+        const uint64_t ty = (t / ntx) % nty;
+        const uint64_t tz = t / (ntx * nty);
+        const uint64_t x0 = tx * 256 + 4 * lane;
+        const bool xin = x0 < a.d0;  // the dimension is a multiple of 4: all four or none
+        const bool rowstart = (lane % L) == 0;
+        bool bad = false;
+        int32_t prev[ZB > 1 ? B : 1][4];
+#pragma unroll
+        for (int j = 0; j < (ZB > 1 ? B : 1); j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) prev[j][i] = 0;
+        for (int zz = 0; zz < ZB; zz++) {
+            const uint64_t z = ND >= 3 ? tz * B + zz : 0;
+            if (z >= a.d2) break;
+            int32_t srow[4] = {0, 0, 0, 0}, zrow[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int yy = 0; yy < YB; yy++) {
+                const uint64_t y = ND >= 2 ? ty * B + yy : 0;
+                if (y < a.d1) {
+                    const uint64_t idx = x0 + y * sy + z * sz;
+                    uint2 cw = make_uint2(R | (R << 16), R | (R << 16));
+                    if (xin) cw = *reinterpret_cast<const uint2*>(codes + idx);
+                    const uint32_t c[4] = {cw.x & 0xFFFF, cw.x >> 16, cw.y & 0xFFFF, cw.y >> 16};
+                    int32_t g[4], base[4], Zv[4], D[4];
+                    bool rs[4], isout[4];
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int32_t Y = (ND >= 2 && yy > 0) ? srow[i] : 0;
+                        Zv[i] = (ND >= 3 && zz > 0) ? prev[ZB > 1 ? yy : 0][i] : 0;
+                        const int32_t YZ = (ND >= 3 && yy > 0 && zz > 0) ? zrow[i] : 0;
+                        base[i] = Y + Zv[i] - YZ;
+                        isout[i] = xin && c[i] == 0;
+                        v[i] = 0.f;
+                        if (isout[i]) {
+                            v[i] = sparse[idx + i];
+                            if (!exact_outlier(v[i])) bad = true;
+                            g[i] = (int32_t)v[i] - base[i];
+                        } else {
+                            g[i] = (int32_t)c[i] - a.radius;
+                        }
+                        rs[i] = isout[i] || (i == 0 && rowstart);
+                    }
+                    row_scan4<B>(D, g, rs);
+                    float o[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int32_t sv = D[i] + base[i];
+                        if (sv >= kExact || sv <= -kExact) bad = true;
+                        o[i] = isout[i] ? scale_outlier(v[i], a.twoeb) : scale_out(sv, a.twoeb);
+                        if (ND >= 3) {
+                            zrow[i] = Zv[i];
+                            prev[ZB > 1 ? yy : 0][i] = sv;
+                        }
+                        srow[i] = sv;
+                    }
+                    if (xin) *reinterpret_cast<float4*>(out + idx) = make_float4(o[0], o[1], o[2], o[3]);
+                }
+            }
+        }
+        flag_blocks(bad, L, tx * (256 / B), ty, tz, nbx, nby, flagged, nflagged);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 
 static unsigned grid_for(uint64_t threads, unsigned block) { return (unsigned)((threads + block - 1) / block); }
@@ -920,6 +1037,31 @@ void launch_scatter_all_outliers(const uint8_t* outliers, uint64_t n, float* spa
 
 void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* sparse, float* out, const ReconArgs& a,
                         uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st) {
+    // four x per lane when rows are whole multiples of 4 and the output is
+    // 16-byte aligned (3D keeps B x 4 running values per lane: B <= 16)
+    const bool x4 = a.d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(codes) & 7) == 0 && (ndim < 3 || B <= 16);
+    if (x4) {
+        const uint64_t ntx = (a.d0 + 255) / 256;
+        const uint64_t work = ntx * (ndim >= 2 ? a.ntiles_y : 1) * (ndim >= 3 ? a.ntiles_z : 1);
+        const dim3 g4((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 2048)));
+#define RX4(ND, BB)                                                                                       \
+    if (ndim == ND && B == BB) {                                                                          \
+        hipLaunchKernelGGL((k_recon_x4<ND, BB>), g4, dim3(256), 0, st, codes, sparse, out, a, flagged, nflagged); \
+        return;                                                                                           \
+    }
+        RX4(3, 8)
+        RX4(3, 16)
+        RX4(2, 8)
+        RX4(2, 16)
+        RX4(2, 32)
+        RX4(2, 64)
+        RX4(1, 8)
+        RX4(1, 16)
+        RX4(1, 32)
+        RX4(1, 64)
+#undef RX4
+    }
 #define RC_CASE(ND, KERN, BB)                                                                               \
     if (ndim == ND && B == BB) {                                                                            \
         hipLaunchKernelGGL(KERN<BB>, dim3(grid), dim3(256), 0, st, codes, sparse, out, a, flagged, nflagged); \
