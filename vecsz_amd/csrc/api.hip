@@ -93,12 +93,13 @@ struct vecsz_ctx {
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     Codebook cb;
+    DecodeTables dt;  // kept between calls: its vectors keep their capacity
     // decompress scratch
-    DevBuf lut, subtab, lut16, sub16, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sym_off, zero_off, scan_tmp, scan_tot,
+    DevBuf tabs, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sym_off, zero_off, scan_tmp, scan_tot,
         dec_sum, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
-    HostBuf pin, pin2;
+    HostBuf pin, pin2, pin_tab;
     std::vector<uint64_t> hist_host;
     // timing
     bool timing = false;
@@ -503,9 +504,12 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     timing_begin(c);
-    if (!c->pin.ensure(1 << 16)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    // One round trip for the metadata, the Huffman header and (for trees of
+    // up to ~14,000 nodes) the tree itself; a larger tree takes a second copy.
+    constexpr uint64_t kHeadRead = 1 << 17;
+    if (!c->pin.ensure(kHeadRead + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
     uint8_t* h = c->pin.as<uint8_t>();
-    const uint64_t head_avail = std::min<uint64_t>(in_bytes, 56);
+    const uint64_t head_avail = std::min<uint64_t>(in_bytes, kHeadRead);
     HIPCHK(c, hipMemcpyAsync(h, d_in, head_avail, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int ndim;
@@ -524,18 +528,26 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     const uint64_t mb = meta_bytes(ndim);
     const uint8_t* d_huff = d_in + mb + 4 * pad_len;
     if (huff_size < 8) return fail(c, VECSZ_ECORRUPT, "Huffman section too short");
-    HIPCHK(c, hipMemcpyAsync(h, d_huff, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int32_t ncs = (int32_t)(((uint32_t)h[0] << 24) | ((uint32_t)h[1] << 16) | ((uint32_t)h[2] << 8) | h[3]);
-    if (ncs <= 0) return fail(c, VECSZ_ECORRUPT, "Huffman node count");
-    const uint64_t tb = tree_byte_size((uint64_t)ncs);
-    if (8 + tb > huff_size) return fail(c, VECSZ_ECORRUPT, "Huffman tree truncated");
-    std::vector<uint8_t> hh(8 + tb);
-    HIPCHK(c, hipMemcpyAsync(hh.data(), d_huff, 8 + tb, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    DecodeTables dt;
-    rc = build_decode_tables(hh.data(), huff_size, dt);
-    if (rc) return fail(c, rc, "Huffman tree is corrupt");
+    DecodeTables& dt = c->dt;
+    uint64_t tb = 0;
+    {
+        HostTimer ht(c, "host_tables");
+        const uint64_t hoff = mb + 4 * pad_len;
+        const uint8_t* hp = h + hoff;
+        int32_t ncs = (int32_t)(((uint32_t)hp[0] << 24) | ((uint32_t)hp[1] << 16) | ((uint32_t)hp[2] << 8) | hp[3]);
+        if (ncs <= 0) return fail(c, VECSZ_ECORRUPT, "Huffman node count");
+        tb = tree_byte_size((uint64_t)ncs);
+        if (8 + tb > huff_size) return fail(c, VECSZ_ECORRUPT, "Huffman tree truncated");
+        if (hoff + 8 + tb > head_avail) {
+            if (!c->pin2.ensure(8 + tb)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+            hp = c->pin2.as<uint8_t>();
+            HIPCHK(c, hipMemcpyAsync(c->pin2.p, d_huff, 8 + tb, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        rc = build_decode_tables(hp, huff_size, dt);
+        if (rc) return fail(c, rc, "Huffman tree is corrupt");
+        if (!dt.root_leaf && !tree_is_full(dt)) return fail(c, VECSZ_ECORRUPT, "Huffman tree is not full");
+    }
 
     if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64))
         return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
@@ -553,39 +565,41 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             launch_scatter_all_outliers(d_outl, n, sparse, c->stream);
         }
     } else {
-        if (!tree_is_full(dt)) return fail(c, VECSZ_ECORRUPT, "Huffman tree is not full");
         const uint64_t T = stream_bytes * 8;
         const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
-        if (!c->lut.ensure(dt.lut.size() * 4) || !c->subtab.ensure(std::max<size_t>(dt.sub.size(), 1) * 4) ||
-            !c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_e2.ensure(nsub * 8) ||
+        if (!c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_e2.ensure(nsub * 8) ||
             !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) || !c->sub_dirty.ensure(nsub * 2) ||
             !c->sym_off.ensure(nsub * 8) || !c->zero_off.ensure(nsub * 8) ||
             !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
             !c->dec_sum.ensure(sizeof(DecSummary)))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-        HIPCHK(c, hipMemcpyAsync(c->lut.p, dt.lut.data(), dt.lut.size() * 4, hipMemcpyHostToDevice, c->stream));
-        if (!dt.sub.empty())
-            HIPCHK(c, hipMemcpyAsync(c->subtab.p, dt.sub.data(), dt.sub.size() * 4, hipMemcpyHostToDevice, c->stream));
-        // compact tables when they exist and their deeper levels fit in LDS
+        // Decode tables in one pinned staging copy: the compact form when the
+        // symbols fit 12 bits and the deeper tables fit in LDS, else 32-bit.
         const bool t16 = !dt.lut16.empty() && dt.sub16.size() <= kMaxSub16;
+        const size_t tab_bytes = t16 ? (dt.lut16.size() + dt.sub16.size()) * 2 : (dt.lut.size() + dt.sub.size()) * 4;
+        if (!c->tabs.ensure(tab_bytes + 16) || !c->pin_tab.ensure(tab_bytes + 16))
+            return fail(c, VECSZ_ENOMEM, "decode table allocation failed");
+        // (the previous call's copy out of pin_tab finished before that call's
+        // mid-call synchronisation)
+        uint8_t* tp = c->pin_tab.as<uint8_t>();
         if (t16) {
-            if (!c->lut16.ensure(dt.lut16.size() * 2) || !c->sub16.ensure(std::max<size_t>(dt.sub16.size(), 1) * 2))
-                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-            HIPCHK(c, hipMemcpyAsync(c->lut16.p, dt.lut16.data(), dt.lut16.size() * 2, hipMemcpyHostToDevice, c->stream));
-            if (!dt.sub16.empty())
-                HIPCHK(c, hipMemcpyAsync(c->sub16.p, dt.sub16.data(), dt.sub16.size() * 2, hipMemcpyHostToDevice,
-                                         c->stream));
+            memcpy(tp, dt.lut16.data(), dt.lut16.size() * 2);
+            memcpy(tp + dt.lut16.size() * 2, dt.sub16.data(), dt.sub16.size() * 2);
+        } else {
+            memcpy(tp, dt.lut.data(), dt.lut.size() * 4);
+            memcpy(tp + dt.lut.size() * 4, dt.sub.data(), dt.sub.size() * 4);
         }
+        HIPCHK(c, hipMemcpyAsync(c->tabs.p, tp, tab_bytes, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemsetAsync(c->dec_sum.p, 0, sizeof(DecSummary), c->stream));
         HIPCHK(c, hipMemsetAsync(c->sub_dirty.p, 0, nsub * 2, c->stream));
         DecodeArgs da;
         da.stream = d_stream;
         da.stream_bits = T;
         da.n = n;
-        da.lut = c->lut.as<uint32_t>();
-        da.sub = c->subtab.as<uint32_t>();
-        da.lut16 = t16 ? c->lut16.as<uint16_t>() : nullptr;
-        da.sub16 = t16 ? c->sub16.as<uint16_t>() : nullptr;
+        da.lut = t16 ? nullptr : c->tabs.as<uint32_t>();
+        da.sub = t16 ? nullptr : c->tabs.as<uint32_t>() + dt.lut.size();
+        da.lut16 = t16 ? c->tabs.as<uint16_t>() : nullptr;
+        da.sub16 = t16 ? c->tabs.as<uint16_t>() + dt.lut16.size() : nullptr;
         da.n_sub16 = t16 ? (uint32_t)dt.sub16.size() : 0;
         da.nsub = nsub;
         da.outliers = d_outl;

@@ -237,26 +237,40 @@ int build_decode_tables(const uint8_t* huff, uint64_t huff_size, DecodeTables& d
     };
     std::vector<Pending> todo;
     std::vector<uint32_t> tab_depth;  // bits consumed before each deeper table
+    // Depth-first over the subtree of `node`: a leaf b bits below it fills
+    // its 2^(bits - b) entries at once, so a table costs its size plus the
+    // nodes it covers.
+    struct Walk {
+        uint32_t node, b, prefix;
+    };
+    std::vector<Walk> stk;
     auto fill = [&](uint32_t node, uint32_t depth, uint32_t bits, bool in_lut, size_t out_base) -> int {
-        for (uint32_t p = 0; p < (1u << bits); p++) {
-            uint32_t nd = node, entry = 0;
-            for (uint32_t b = 0; b < bits; b++) {
-                const uint32_t nx = dt.child[2 * nd + ((p >> (bits - 1 - b)) & 1)];
-                if (nx == kNodeAbsent) break;  // entry 0: invalid path
+        uint32_t* out = (in_lut ? dt.lut : dt.sub).data() + out_base;
+        for (uint32_t p = 0; p < (1u << bits); p++) out[p] = 0;  // invalid unless reached
+        stk.clear();
+        stk.push_back({node, 0, 0});
+        while (!stk.empty()) {
+            const Walk w = stk.back();
+            stk.pop_back();
+            for (uint32_t bit = 0; bit < 2; bit++) {
+                const uint32_t nx = dt.child[2 * w.node + bit];
+                if (nx == kNodeAbsent) continue;
+                const uint32_t b = w.b + 1, prefix = (w.prefix << 1) | bit;
                 if (nx & kNodeLeaf) {
-                    const uint32_t len = depth + b + 1;
+                    const uint32_t len = depth + b;
                     if (len > 64) return VECSZ_ECORRUPT;
-                    entry = (len << 16) | (nx & 0xFFFF);
-                    break;
-                }
-                nd = nx;
-                if (b == bits - 1) {
+                    const uint32_t e = (len << 16) | (nx & 0xFFFF);
+                    const uint32_t lo = prefix << (bits - b), hi = (prefix + 1) << (bits - b);
+                    for (uint32_t q = lo; q < hi; q++) out[q] = e;
+                } else if (b == bits) {
                     if (depth + bits >= 64) return VECSZ_ECORRUPT;
-                    entry = kLutSlow;
-                    todo.push_back({nd, depth + bits, in_lut, out_base + p});
+                    out[prefix] = kLutSlow;  // patched with the table index below
+                    todo.push_back({nx, depth + bits, in_lut, out_base + prefix});
+                } else {
+                    if (stk.size() > 4096) return VECSZ_ECORRUPT;
+                    stk.push_back({nx, b, prefix});
                 }
             }
-            (in_lut ? dt.lut : dt.sub)[out_base + p] = entry;
         }
         return VECSZ_OK;
     };
