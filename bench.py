@@ -108,6 +108,126 @@ def cpu_baseline(x_host, eb, range_, dims, dict_size, block):
     }
 
 
+def bench_batch(args):
+    """96 fields of uneven shape (2D and 3D GRFs, per-field r2r eb): one
+    batched call against the same fields compressed by one call each, on one
+    context; archives must be identical."""
+    import torch
+
+    from vecsz_amd import Context, datagen, make_params
+
+    torch.cuda.set_device(0)
+    mix = datagen.batch_mix_shapes(96, seed=0)
+    fields, params, outs = [], [], []
+    for k, (shape, eb) in enumerate(mix):
+        x = datagen.grf(shape, seed=100 + k, device="cuda", pow2=True)
+        fields.append(x)
+        if k % 16 == 15:
+            torch.cuda.synchronize()
+            log(f"generated {k + 1} fields")
+        p = make_params(tuple(reversed(shape)), eb, "r2r", args.block, args.dict)
+        params.append(p)
+    ctx = Context(0)
+    caps = [ctx.compress_bound(p) for p in params]
+    outs = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    outs1 = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    torch.cuda.synchronize()
+    total = sum(4 * f.numel() for f in fields)
+    log(f"c4 mix: {len(fields)} fields, {total / 2**30:.3f} GiB")
+
+    def plain():
+        return [ctx.compress_into(p, f.data_ptr(), o.data_ptr(), o.numel()) for p, f, o in zip(params, fields, outs1)]
+
+    def batch():
+        return ctx.compress_batch_into(params, [f.data_ptr() for f in fields], [o.data_ptr() for o in outs], caps)
+
+    for _ in range(args.warmup):
+        log("warmup: per-field calls")
+        plain()
+        log("warmup: batched call")
+        batch()
+    torch.cuda.synchronize()
+    log("timing compress")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r1 = plain()
+    torch.cuda.synchronize()
+    tp = (time.perf_counter() - t0) / args.steps
+    sweep = {}
+    for nw in reversed([int(w) for w in args.workers.split(",")]):
+        ctx.set_workers(nw)
+        batch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rb = batch()
+        torch.cuda.synchronize()
+        sweep[nw] = (time.perf_counter() - t0) / args.steps
+        log(f"batch with {nw} workers: {sweep[nw] * 1e3:.3f} ms")
+    tb = sweep[int(args.workers.split(",")[0])]
+    same = r1 == rb and all(torch.equal(a[:s], b[:s]) for a, b, (s, _) in zip(outs, outs1, rb))
+
+    # decompress both ways
+    vr = [float((f.max() - f.min()).item()) for f in fields]
+    dparams = [make_params((1,), 1e-4, "r2r", args.block, args.dict, v) for v in vr]
+    ys = [torch.empty_like(f) for f in fields]
+    ys1 = [torch.empty_like(f) for f in fields]
+
+    def dplain():
+        for p, a, (s, _), y in zip(dparams, outs, rb, ys1):
+            ctx.decompress_into(p, a.data_ptr(), s, y.data_ptr(), y.numel())
+
+    def dbatch():
+        ctx.decompress_batch_into(dparams, [a.data_ptr() for a in outs], [s for s, _ in rb],
+                                  [y.data_ptr() for y in ys], [y.numel() for y in ys])
+
+    log("timing decompress")
+    for _ in range(args.warmup):
+        dplain()
+        dbatch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dplain()
+    torch.cuda.synchronize()
+    tdp = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dbatch()
+    torch.cuda.synchronize()
+    tdb = (time.perf_counter() - t0) / args.steps
+    dsame = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
+    arch = sum(s for s, _ in rb)
+    out = {
+        "metric": METRIC + " [batched form, 96-field mix]",
+        "value": round(total / tb / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(tb * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+            "data": "synthetic (96 GRFs E(k)~k^-5/3 cropped from power-of-two grids, seeds 100..195, shapes from "
+                "datagen.batch_mix_shapes(96, 0))",
+        "config": {"workload": f"batched mix: {len(fields)} fields, {total / 2**30:.3f} GiB, r2r eb per field in "
+                               "{1e-2,1e-3,1e-4}, B 8, dict 4096", "parallelism": "single-gpu"},
+        "per_field_calls_ms": round(tp * 1e3, 3),
+        "batch_ms_by_workers": {str(k): round(v * 1e3, 3) for k, v in sorted(sweep.items())},
+        "batch_speedup": round(tp / tb, 3),
+        "identical_archives": bool(same),
+        "decompress_gbps": round(total / tdb / 1e9, 2),
+        "decompress_per_field_calls_ms": round(tdp * 1e3, 3),
+        "decompress_batch_ms": round(tdb * 1e3, 3),
+        "decompress_batch_speedup": round(tdp / tdb, 3),
+        "identical_outputs": bool(dsame),
+        "archive_bytes": arch,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +238,9 @@ def main():
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--dict", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="grf512", choices=["grf512", "c4"],
+                    help="grf512: one 3D field (default line); c4: 96-field batched mix vs per-field calls")
+    ap.add_argument("--workers", default="8", help="c4: comma-separated worker counts to time (first is reported)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain device copy of the field (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -129,6 +252,9 @@ def main():
     import torch
 
     from vecsz_amd import Context, datagen, make_params
+
+    if args.workload == "c4":
+        return bench_batch(args)
 
     shape = tuple(int(s) for s in args.shape.split(","))
     dims = tuple(reversed(shape))

@@ -99,6 +99,22 @@ int vecsz_decompress_host(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* 
 int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint64_t dims[3], double* eb,
                        uint64_t* huff_size, uint64_t* n_outlier);
 
+/* Batched form: many independent fields per call, each compressed exactly as
+ * vecsz_compress would (identical archives).  No counterpart in the
+ * reference, whose main compresses one file per run (src/vecsz.cc:37).
+ * Fields are spread, largest first, over worker contexts (own HIP stream and
+ * scratch each, set_workers of them), so small fields overlap on the GPU and
+ * host codebook work overlaps kernels.  status[k] receives each field's
+ * VECSZ_* status; the call returns the first failure (or VECSZ_OK) after all
+ * fields are done.  Inputs must be complete on the context's stream. */
+int vecsz_compress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* params, const float* const* d_in,
+                         uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes, uint64_t* n_outlier,
+                         int* status);
+int vecsz_decompress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* params, const uint8_t* const* d_in,
+                           const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status);
+/* Worker contexts used by the batched calls (1..16, default 8). */
+int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
+
 /* Huffman codebook of a quant-code histogram, built on the host exactly as the
  * reference builds it: HuffmanTree init / build_tree / build_code
  * (src/huffman.hh:239-262, :200-227) and the tree serialization written into

@@ -23,7 +23,8 @@ MODE_R2R = 1
 EXPORTS = (
     "vecsz_ctx_create", "vecsz_ctx_destroy", "vecsz_strerror", "vecsz_ctx_last_error", "vecsz_params_default",
     "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
-    "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
+    "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_compress_batch", "vecsz_decompress_batch",
+    "vecsz_ctx_set_workers", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
 )
 
 
@@ -74,6 +75,9 @@ def load():
     lib.vecsz_archive_info.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int), u64p, ctypes.POINTER(ctypes.c_double),
                                        u64p, u64p]
     lib.vecsz_huffman_codebook.argtypes = [vp, ctypes.c_int, vp, vp, vp, u64, u64p, ctypes.POINTER(ctypes.c_int)]
+    lib.vecsz_compress_batch.argtypes = [vp, ctypes.c_int, P, vp, vp, u64p, u64p, u64p, ctypes.POINTER(ctypes.c_int)]
+    lib.vecsz_decompress_batch.argtypes = [vp, ctypes.c_int, P, vp, u64p, vp, u64p, ctypes.POINTER(ctypes.c_int)]
+    lib.vecsz_ctx_set_workers.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_enable_timing.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int]

@@ -10,11 +10,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vecsz_hip.h"
@@ -32,12 +34,15 @@ struct DevBuf {
     ~DevBuf() {
         if (p) (void)hipFree(p);
     }
+    // Grows by at least half again (and to 64 KiB at least) so that buffers
+    // sized by varying quantities (trees, tables) settle after a few calls:
+    // hipFree synchronises the whole device, which would stall other streams.
     bool ensure(size_t bytes) {
         if (bytes <= cap && p) return true;
+        const size_t want = std::max<size_t>({bytes, cap + cap / 2, (size_t)64 << 10});
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
-        size_t want = std::max<size_t>(bytes, 256);
         if (hipMalloc(&p, want) != hipSuccess) {
             p = nullptr;
             (void)hipGetLastError();
@@ -60,10 +65,10 @@ struct HostBuf {
     }
     bool ensure(size_t bytes) {
         if (bytes <= cap && p) return true;
+        const size_t want = std::max<size_t>({bytes, 2 * cap, (size_t)1 << 20});
         if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
-        size_t want = std::max<size_t>(bytes, 4096);
         if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
             p = nullptr;
             (void)hipGetLastError();
@@ -90,6 +95,9 @@ struct vecsz_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string last_error;
+    // batch workers: each a context of its own (stream + scratch), created on first use
+    std::vector<vecsz_ctx*> workers;
+    int nworkers = 8;
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     Codebook cb;
@@ -754,6 +762,8 @@ int vecsz_ctx_create(vecsz_ctx** out, int device, void* stream) {
 
 int vecsz_ctx_destroy(vecsz_ctx* c) {
     if (!c) return VECSZ_OK;
+    for (vecsz_ctx* w : c->workers) vecsz_ctx_destroy(w);
+    c->workers.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
@@ -853,6 +863,106 @@ int vecsz_huffman_codebook(const uint64_t* freq, int dict, uint64_t* code_bits, 
         memcpy(tree, cb.tree.data(), cb.tree.size());
     }
     return VECSZ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+
+// Runs job(worker, k) for every field k, largest first, on up to
+// c->nworkers host threads.  Each worker is a context with its own stream and
+// scratch, so small fields overlap on the GPU and one field's host codebook
+// overlaps the others' kernels.  Returns after every worker's stream is idle.
+template <typename Prep, typename Job>
+static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, Prep&& prep, Job&& job) {
+    if (nf <= 0) return VECSZ_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // inputs produced on the caller's stream
+    const int nw = std::max(1, std::min(c->nworkers, nf));
+    while ((int)c->workers.size() < nw) {
+        vecsz_ctx* w = nullptr;
+        const int rc = vecsz_ctx_create(&w, c->device, nullptr);
+        if (rc) return fail(c, rc, "batch worker context");
+        c->workers.push_back(w);
+    }
+    // size every worker's field-proportional scratch for the largest field
+    // up front: growing it later means hipFree, which stalls all streams
+    for (int t = 0; t < nw; t++)
+        if (!prep(c->workers[t])) return fail(c, VECSZ_ENOMEM, "batch worker scratch");
+    std::vector<int> order(nf);
+    for (int k = 0; k < nf; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
+    std::atomic<int> next{0};
+    auto body = [&](vecsz_ctx* w) {
+        (void)hipSetDevice(w->device);
+        for (int i; (i = next.fetch_add(1)) < nf;) {
+            const int k = order[i];
+            status[k] = job(w, k);
+        }
+        (void)hipStreamSynchronize(w->stream);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nw; t++) th.emplace_back(body, c->workers[t]);
+    body(c->workers[0]);
+    for (auto& t : th) t.join();
+    for (int k = 0; k < nf; k++)
+        if (status[k]) return fail(c, status[k], "batch field " + std::to_string(k) + " failed");
+    return VECSZ_OK;
+}
+
+extern "C" {
+
+int vecsz_ctx_set_workers(vecsz_ctx* c, int nworkers) {
+    if (!c || nworkers < 1 || nworkers > 16) return VECSZ_EINVAL;
+    c->nworkers = nworkers;
+    return VECSZ_OK;
+}
+
+int vecsz_compress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, const float* const* d_in,
+                         uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes, uint64_t* n_outlier,
+                         int* status) {
+    if (!c || nfields < 0 || (nfields && (!p || !d_in || !d_out || !out_cap || !out_bytes || !n_outlier || !status)))
+        return fail(c, VECSZ_EINVAL, "NULL argument");
+    std::vector<uint64_t> weight(nfields);
+    for (int k = 0; k < nfields; k++) {
+        uint64_t n = 1;
+        for (int i = 0; i < std::max(1, std::min(3, p[k].ndim)); i++) n *= p[k].dims[i];
+        weight[k] = n;
+    }
+    const uint64_t nmax = nfields ? *std::max_element(weight.begin(), weight.end()) : 0;
+    auto prep = [&](vecsz_ctx* w) {
+        (void)hipSetDevice(w->device);
+        const uint64_t nch = (nmax + kChunk - 1) / kChunk;
+        return w->codes.ensure(nmax * 2 + 64) && w->sparse.ensure(nmax * 4 + 64) && w->chunk_bits.ensure(nch * 4) &&
+               w->chunk_zeros.ensure(nch * 4) && w->bit_off.ensure(nch * 8) && w->out_off.ensure(nch * 8);
+    };
+    return run_batch(c, nfields, weight.data(), status, prep, [&](vecsz_ctx* w, int k) {
+        if (!d_in[k] || !d_out[k]) return fail(w, VECSZ_EINVAL, "NULL field");
+        return compress_device(w, &p[k], d_in[k], d_out[k], out_cap[k], &out_bytes[k], &n_outlier[k]);
+    });
+}
+
+int vecsz_decompress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, const uint8_t* const* d_in,
+                           const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status) {
+    if (!c || nfields < 0 || (nfields && (!p || !d_in || !in_bytes || !d_out || !out_cap || !status)))
+        return fail(c, VECSZ_EINVAL, "NULL argument");
+    std::vector<uint64_t> weight(nfields);
+    for (int k = 0; k < nfields; k++) weight[k] = out_cap[k];
+    const uint64_t nmax = nfields ? *std::max_element(weight.begin(), weight.end()) : 0;
+    uint64_t bmax = 0;
+    for (int k = 0; k < nfields; k++) bmax = std::max(bmax, in_bytes[k]);
+    auto prep = [&](vecsz_ctx* w) {
+        (void)hipSetDevice(w->device);
+        const uint64_t nsub = bmax * 8 / kSubBits + 2;
+        return w->codes.ensure(nmax * 2 + 64) && w->sparse.ensure(nmax * 4 + 64) && w->sub_b.ensure(nsub * 8) &&
+               w->sub_e.ensure(nsub * 8) && w->sub_e2.ensure(nsub * 8) && w->sub_cnt.ensure(nsub * 4) &&
+               w->sub_zeros.ensure(nsub * 4) && w->sub_dirty.ensure(nsub * 2) && w->sym_off.ensure(nsub * 8) &&
+               w->zero_off.ensure(nsub * 8);
+    };
+    return run_batch(c, nfields, weight.data(), status, prep, [&](vecsz_ctx* w, int k) {
+        if (!d_in[k] || !d_out[k]) return fail(w, VECSZ_EINVAL, "NULL field");
+        return decompress_device(w, &p[k], d_in[k], in_bytes[k], d_out[k], out_cap[k]);
+    });
 }
 
 int vecsz_ctx_enable_timing(vecsz_ctx* c, int on) {

@@ -90,12 +90,21 @@ def _grf_torch(shape, seed, lo, hi):
     return f.contiguous()
 
 
-def grf(shape, seed: int = 0, lo: float = -50.0, hi: float = 50.0, device: str = "cpu"):
+def grf(shape, seed: int = 0, lo: float = -50.0, hi: float = 50.0, device: str = "cpu", pow2: bool = False):
     """Gaussian random field, E(k) ~ k^-5/3, scaled to [lo, hi].
 
     ``device="cpu"`` returns a numpy array; ``device="cuda"`` returns a torch
-    tensor on the GPU."""
+    tensor on the GPU.  ``pow2=True`` synthesises on the next power-of-two grid
+    and crops (a window of a GRF is a GRF): FFT sizes then come from a small
+    set, which keeps FFT plan creation out of the way for many odd shapes."""
     shape = tuple(int(s) for s in shape)
+    if pow2:
+        big = tuple(1 << max(0, (s - 1).bit_length()) for s in shape)
+        f = grf(big, seed, lo, hi, device)
+        f = f[tuple(slice(0, s) for s in shape)]
+        fmin, fmax = f.min(), f.max()
+        f = (f - fmin) * ((hi - lo) / (fmax - fmin)) + lo
+        return f.contiguous() if device != "cpu" else np.ascontiguousarray(f.astype(np.float32))
     if device == "cpu":
         return _grf_numpy(shape, seed, lo, hi)
     return _grf_torch(shape, seed, lo, hi)
@@ -157,3 +166,27 @@ def batch_mix(n_fields: int = 96, seed: int = 0, target_bytes: float = 8 * 2**30
         if abs(total - target_bytes) <= tol * target_bytes:
             return fields, seed + attempt
     raise RuntimeError("no draw within tolerance")
+
+
+def batch_mix_shapes(nfields: int = 96, seed: int = 0):
+    """Shapes (C order) of the batched-form mix: half 2D, half 3D (seeded);
+    element count log-uniform over 64x64 .. 4096x4096 (2D) and
+    32^3 .. 256x384x384 (3D), split into axes with log-uniform aspect and
+    clamped to those per-axis ranges.  Returns [(shape, eb)] with the r2r
+    error bound drawn from {1e-2, 1e-3, 1e-4}."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nfields):
+        eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        if rng.random() < 0.5:
+            n = np.exp(rng.uniform(np.log(64 * 64), np.log(4096 * 4096)))
+            y = int(np.clip(round(np.sqrt(n) * np.exp(rng.uniform(np.log(0.5), np.log(2.0)))), 64, 4096))
+            x = int(np.clip(round(n / y), 64, 4096))
+            out.append(((y, x), eb))
+        else:
+            n = np.exp(rng.uniform(np.log(32 ** 3), np.log(256 * 384 * 384)))
+            z = int(np.clip(round(np.cbrt(n) * np.exp(rng.uniform(np.log(0.5), np.log(2.0)))), 32, 256))
+            y = int(np.clip(round(np.sqrt(n / z) * np.exp(rng.uniform(np.log(0.7), np.log(1.4)))), 32, 384))
+            x = int(np.clip(round(n / (z * y)), 32, 384))
+            out.append(((z, y, x), eb))
+    return out

@@ -159,6 +159,10 @@ class Context:
 
     # -------------------------------------------------------------- device API
 
+    def set_workers(self, n: int):
+        """Worker contexts of the batched calls (1..16)."""
+        self._check(self._lib.vecsz_ctx_set_workers(self._ctx, int(n)))
+
     def compress_bound(self, params: Params) -> int:
         b = ctypes.c_uint64()
         self._check(self._lib.vecsz_compress_bound(ctypes.byref(params), ctypes.byref(b)))
@@ -177,6 +181,64 @@ class Context:
         rc = self._lib.vecsz_decompress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr), in_bytes,
                                         ctypes.c_void_p(d_out_ptr), out_cap)
         self._check(rc)
+
+    def compress_batch_into(self, params, d_in, d_out, caps):
+        """Raw device pointers, one entry per field; returns [(bytes, n_outlier)]."""
+        nf = len(params)
+        P = (Params * nf)(*params)
+        vp = ctypes.c_void_p
+        ins = (vp * nf)(*[vp(int(x)) for x in d_in])
+        outs = (vp * nf)(*[vp(int(x)) for x in d_out])
+        cap = (ctypes.c_uint64 * nf)(*[int(c) for c in caps])
+        sizes = (ctypes.c_uint64 * nf)()
+        nout = (ctypes.c_uint64 * nf)()
+        st = (ctypes.c_int * nf)()
+        self._check(self._lib.vecsz_compress_batch(self._ctx, nf, P, ins, outs, cap, sizes, nout, st))
+        return [(sizes[k], nout[k]) for k in range(nf)]
+
+    def decompress_batch_into(self, params, d_in, in_bytes, d_out, caps):
+        nf = len(params)
+        P = (Params * nf)(*params)
+        vp = ctypes.c_void_p
+        ins = (vp * nf)(*[vp(int(x)) for x in d_in])
+        nb = (ctypes.c_uint64 * nf)(*[int(b) for b in in_bytes])
+        outs = (vp * nf)(*[vp(int(x)) for x in d_out])
+        cap = (ctypes.c_uint64 * nf)(*[int(c) for c in caps])
+        st = (ctypes.c_int * nf)()
+        self._check(self._lib.vecsz_decompress_batch(self._ctx, nf, P, ins, nb, outs, cap, st))
+
+    def compress_batch(self, fields, ebs, modes="abs", block: int = 8, dict_size: int = 4096):
+        """Compress a list of float32 CUDA tensors in one batched call;
+        returns [(uint8 CUDA tensor, n_outlier)], each identical to compress()."""
+        nf = len(fields)
+        ebs = list(ebs) if isinstance(ebs, (list, tuple)) else [ebs] * nf
+        modes = list(modes) if isinstance(modes, (list, tuple)) else [modes] * nf
+        fields = [f.contiguous() for f in fields]
+        params, outs = [], []
+        for f, eb, m in zip(fields, ebs, modes):
+            if not f.is_cuda or f.dtype != torch.float32:
+                raise TypeError("expected float32 CUDA tensors")
+            p = make_params(dims_of(f.shape), eb, m, block, dict_size)
+            params.append(p)
+            outs.append(torch.empty(self.compress_bound(p), dtype=torch.uint8, device=f.device))
+        res = self.compress_batch_into(params, [f.data_ptr() for f in fields], [o.data_ptr() for o in outs],
+                                       [o.numel() for o in outs])
+        return [(o[:size], nout) for o, (size, nout) in zip(outs, res)]
+
+    def decompress_batch(self, archives, modes="abs", value_ranges=0.0, block: int = 8, dict_size: int = 4096):
+        """Decompress a list of uint8 CUDA archives in one batched call."""
+        nf = len(archives)
+        modes = list(modes) if isinstance(modes, (list, tuple)) else [modes] * nf
+        vrs = list(value_ranges) if isinstance(value_ranges, (list, tuple)) else [value_ranges] * nf
+        params, outs = [], []
+        for a, m, vr in zip(archives, modes, vrs):
+            info = header_fields(a[:64].cpu().numpy().tobytes())
+            shape = tuple(reversed(info["dims"]))
+            params.append(make_params((1,), 1e-4, m, block, dict_size, vr))
+            outs.append(torch.empty(shape, dtype=torch.float32, device=a.device))
+        self.decompress_batch_into(params, [a.data_ptr() for a in archives], [a.numel() for a in archives],
+                                   [o.data_ptr() for o in outs], [o.numel() for o in outs])
+        return outs
 
     def compress(self, x, eb: float = 1e-4, mode="abs", block: int = 8, dict_size: int = 4096,
                  value_range: float = 0.0, out=None):
