@@ -228,6 +228,115 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """Served form: a Poisson stream of compress and decompress requests (50/50,
+    sizes log-uniform 64 KiB..512 MiB, 1D/2D/3D, abs eb 1e-4) offered at a
+    fraction of the measured peak to the request server (200 GB budget);
+    p50/p99 latency and sustained GB/s.  Peak = the same requests submitted
+    all at once."""
+    import random
+
+    import numpy as np
+    import torch
+
+    from vecsz_amd import Context, Server, datagen, make_params
+
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(5)
+    ctx = Context(0)
+    reqs = []
+    for i in range(args.requests):
+        nbytes = float(np.exp(rng.uniform(np.log(64 << 10), np.log(512 << 20))))
+        n = max(1, int(nbytes // 4))
+        nd = int(rng.integers(1, 4))
+        if nd == 1:
+            shape = (n,)
+        elif nd == 2:
+            y = max(8, int(round(np.sqrt(n))))
+            shape = (y, max(8, n // y))
+        else:
+            z = max(8, int(round(np.cbrt(n))))
+            y = max(8, int(round(np.sqrt(n / z))))
+            shape = (z, y, max(8, n // (z * y)))
+        x = datagen.grf(shape, seed=300 + i, device="cuda", pow2=True) if nd > 1 else \
+            datagen.walk_spikes_1d(shape[0], seed=300 + i, device="cuda")
+        kind = "c" if rng.random() < 0.5 else "d"
+        p = make_params(tuple(reversed(shape)), 1e-4, "abs")
+        cap = ctx.compress_bound(p)
+        r = {"kind": kind, "x": x, "p": p, "bytes": 4 * x.numel()}
+        if kind == "c":
+            r["out"] = torch.empty(cap, dtype=torch.uint8, device="cuda")
+        else:
+            a = torch.empty(cap, dtype=torch.uint8, device="cuda")
+            size, _ = ctx.compress_into(p, x.data_ptr(), a.data_ptr(), cap)
+            r["a"], r["size"] = a, size
+            r["out"] = torch.empty_like(x)
+        reqs.append(r)
+        if i % 16 == 15:
+            torch.cuda.synchronize()
+            log(f"prepared {i + 1} requests")
+    torch.cuda.synchronize()
+    total = sum(r["bytes"] for r in reqs)
+    pd = make_params((1,), 1e-4, "abs")
+    srv = Server(0, slots=8, hbm_budget=200 * 10**9)
+
+    def submit(r):
+        if r["kind"] == "c":
+            return srv.submit_compress(r["p"], r["x"].data_ptr(), r["out"].data_ptr(), r["out"].numel())
+        return srv.submit_decompress(pd, r["a"].data_ptr(), r["size"], r["out"].data_ptr(), r["out"].numel())
+
+    # warm the slots, then the peak: everything submitted at once
+    for t in [submit(r) for r in reqs]:
+        srv.wait(t)
+    t0 = time.perf_counter()
+    for t in [submit(r) for r in reqs]:
+        srv.wait(t)
+    peak = total / (time.perf_counter() - t0)
+    log(f"peak (closed loop): {peak / 1e9:.2f} GB/s over {total / 2**30:.2f} GiB")
+
+    # open loop: Poisson arrivals at load * peak (bytes/s); submission order shuffled
+    lam = args.load * peak / (total / len(reqs))
+    order = list(range(len(reqs)))
+    random.Random(7).shuffle(order)
+    gaps = np.random.default_rng(11).exponential(1.0 / lam, size=len(reqs))
+    arrivals = np.cumsum(gaps)
+    tickets = []
+    start = time.perf_counter()
+    for k, i in enumerate(order):
+        delay = arrivals[k] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        tickets.append(submit(reqs[i]))
+    res = [srv.wait(t) for t in tickets]
+    span = time.perf_counter() - start
+    lat = np.array([r["latency_ms"] for r in res])
+    qms = np.array([r["queue_ms"] for r in res])
+    srv.close()
+    out = {
+        "metric": METRIC + " [served form, Poisson 50/50 compress/decompress]",
+        "value": round(total / span / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": len(reqs),
+        "warmup": 1,
+        "ms_per_step": round(span * 1e3 / len(reqs), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (GRFs cropped from power-of-two grids for 2D/3D, random walk + spikes for 1D)",
+        "config": {"workload": f"served: {len(reqs)} requests, {total / 2**30:.2f} GiB, sizes log-uniform 64 KiB..512 MiB, "
+                               f"abs eb 1e-4, offered load {args.load} x peak, 8 slots, 200 GB budget",
+                   "parallelism": "single-gpu"},
+        "peak_gbps": round(peak / 1e9, 2),
+        "offered_gbps": round(args.load * peak / 1e9, 2),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3)},
+        "queue_ms": {"p50": round(float(np.percentile(qms, 50)), 3), "p99": round(float(np.percentile(qms, 99)), 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,8 +347,11 @@ def main():
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--dict", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="grf512", choices=["grf512", "c4"],
-                    help="grf512: one 3D field (default line); c4: 96-field batched mix vs per-field calls")
+    ap.add_argument("--workload", default="grf512", choices=["grf512", "c4", "c5"],
+                    help="grf512: one 3D field (default line); c4: 96-field batched mix vs per-field calls; "
+                         "c5: served Poisson request stream")
+    ap.add_argument("--requests", type=int, default=96, help="c5: requests in the stream")
+    ap.add_argument("--load", type=float, default=0.7, help="c5: offered load as a fraction of measured peak")
     ap.add_argument("--workers", default="8", help="c4: comma-separated worker counts to time (first is reported)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain device copy of the field (known bytes) to scale HBM counters")
@@ -255,6 +367,8 @@ def main():
 
     if args.workload == "c4":
         return bench_batch(args)
+    if args.workload == "c5":
+        return bench_serve(args)
 
     shape = tuple(int(s) for s in args.shape.split(","))
     dims = tuple(reversed(shape))

@@ -115,6 +115,34 @@ int vecsz_decompress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* para
 /* Worker contexts used by the batched calls (1..16, default 8). */
 int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
 
+/* Served form: a request server on one GPU (no counterpart in the
+ * reference).  nslots slot threads, each with its own context (stream and
+ * scratch kept resident between requests), take requests from one FIFO
+ * queue; a request is admitted when a slot is free and the device bytes it
+ * holds while running (its buffers plus its scratch) fit in what is left of
+ * hbm_budget, so requests that arrive while others run start as soon as room
+ * frees up and overlap on the GPU.  Results are those of vecsz_compress /
+ * vecsz_decompress on the same request.  Buffers are the caller's device
+ * memory and must stay valid until the request is waited for. */
+typedef struct vecsz_server vecsz_server;
+typedef struct vecsz_server_result {
+    int status;          /* VECSZ_* of the request */
+    uint64_t out_bytes;  /* compress: archive size */
+    uint64_t n_outlier;  /* compress: outlier count */
+    double queue_ms;     /* submit -> start */
+    double run_ms;       /* start -> done */
+    double latency_ms;   /* submit -> done */
+} vecsz_server_result;
+int vecsz_server_create(vecsz_server** server, int device, int nslots, uint64_t hbm_budget);
+int vecsz_server_destroy(vecsz_server* server);
+/* VECSZ_ENOMEM when the request could never fit the budget. */
+int vecsz_server_submit_compress(vecsz_server* server, const vecsz_params* p, const float* d_in, uint8_t* d_out,
+                                 uint64_t out_cap, uint64_t* ticket);
+int vecsz_server_submit_decompress(vecsz_server* server, const vecsz_params* p, const uint8_t* d_in,
+                                   uint64_t in_bytes, float* d_out, uint64_t out_cap, uint64_t* ticket);
+/* Blocks until the request is done; each ticket is waited for once. */
+int vecsz_server_wait(vecsz_server* server, uint64_t ticket, vecsz_server_result* result);
+
 /* Huffman codebook of a quant-code histogram, built on the host exactly as the
  * reference builds it: HuffmanTree init / build_tree / build_code
  * (src/huffman.hh:239-262, :200-227) and the tree serialization written into

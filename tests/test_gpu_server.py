@@ -1,0 +1,76 @@
+"""The request server: compress and decompress requests submitted together
+(overlapping on the GPU) give exactly what each gives alone; admission keeps
+to the HBM budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _requests(ctx, seed=0):
+    from vecsz_amd import datagen as dg, make_params
+
+    rng = np.random.default_rng(seed)
+    shapes = [(100_000,), (3000,), (96, 130), (256, 256), (20, 33, 70), (64, 64, 64), (1 << 18,), (40, 48, 56)]
+    reqs = []
+    for i, s in enumerate(shapes):
+        x = torch.from_numpy(dg.grf(s, seed=i) if len(s) > 1 else
+                             np.cumsum(rng.normal(0, 0.05, s)).astype(np.float32)).cuda()
+        eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        p = make_params(tuple(reversed(s)), eb, "abs")
+        a, nout = ctx.compress(x, eb, "abs")  # alone
+        y = ctx.decompress(a, "abs")
+        reqs.append({"x": x, "p": p, "a": a.clone(), "nout": nout, "y": y.clone()})
+    return reqs
+
+
+def _run_mixed(server, reqs):
+    from vecsz_amd import make_params
+
+    tickets = []
+    for r in reqs:
+        cap = int(r["x"].numel() * 12 + 4096)
+        r["out_c"] = torch.empty(cap, dtype=torch.uint8, device="cuda")
+        r["out_d"] = torch.empty_like(r["x"])
+        torch.cuda.synchronize()
+        tickets.append(("c", r, server.submit_compress(r["p"], r["x"].data_ptr(), r["out_c"].data_ptr(), cap)))
+        pd = make_params((1,), 1e-4, "abs")
+        tickets.append(("d", r, server.submit_decompress(pd, r["a"].data_ptr(), r["a"].numel(), r["out_d"].data_ptr(),
+                                                         r["out_d"].numel())))
+    for kind, r, t in tickets:
+        res = server.wait(t)
+        if kind == "c":
+            assert res["out_bytes"] == r["a"].numel() and res["n_outlier"] == r["nout"]
+            assert torch.equal(r["out_c"][: res["out_bytes"]], r["a"])
+        else:
+            assert torch.equal(r["out_d"].view(torch.int32), r["y"].view(torch.int32))
+
+
+def test_server_results_match_standalone(gpu_ctx):
+    from vecsz_amd import Server
+
+    reqs = _requests(gpu_ctx)
+    srv = Server(0, slots=4)
+    print("mixed round 1", flush=True)
+    _run_mixed(srv, reqs)
+    print("mixed round 2", flush=True)
+    _run_mixed(srv, reqs)  # resident slot scratch reused
+    srv.close()
+
+
+def test_server_budget_admission(gpu_ctx):
+    from vecsz_amd import Server, VecszError, make_params
+
+    reqs = _requests(gpu_ctx, seed=3)[:4]
+    # room for about one request at a time: everything still completes, correctly
+    srv = Server(0, slots=4, hbm_budget=64 << 20)
+    print("tight budget", flush=True)
+    _run_mixed(srv, reqs)
+    # a request that can never fit is refused at submit
+    x = torch.zeros(1 << 24, device="cuda")
+    out = torch.empty(1 << 20, dtype=torch.uint8, device="cuda")
+    with pytest.raises(VecszError):
+        srv.submit_compress(make_params((1 << 24,), 1e-3, "abs"), x.data_ptr(), out.data_ptr(), out.numel())
+    srv.close()

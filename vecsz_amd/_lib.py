@@ -24,7 +24,8 @@ EXPORTS = (
     "vecsz_ctx_create", "vecsz_ctx_destroy", "vecsz_strerror", "vecsz_ctx_last_error", "vecsz_params_default",
     "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
     "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_compress_batch", "vecsz_decompress_batch",
-    "vecsz_ctx_set_workers", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
+    "vecsz_ctx_set_workers", "vecsz_server_create", "vecsz_server_destroy", "vecsz_server_submit_compress",
+    "vecsz_server_submit_decompress", "vecsz_server_wait", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
 )
 
 
@@ -37,6 +38,17 @@ class Params(ctypes.Structure):
         ("value_range", ctypes.c_double),
         ("block", ctypes.c_int),
         ("dict_size", ctypes.c_int),
+    ]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("out_bytes", ctypes.c_uint64),
+        ("n_outlier", ctypes.c_uint64),
+        ("queue_ms", ctypes.c_double),
+        ("run_ms", ctypes.c_double),
+        ("latency_ms", ctypes.c_double),
     ]
 
 
@@ -78,6 +90,11 @@ def load():
     lib.vecsz_compress_batch.argtypes = [vp, ctypes.c_int, P, vp, vp, u64p, u64p, u64p, ctypes.POINTER(ctypes.c_int)]
     lib.vecsz_decompress_batch.argtypes = [vp, ctypes.c_int, P, vp, u64p, vp, u64p, ctypes.POINTER(ctypes.c_int)]
     lib.vecsz_ctx_set_workers.argtypes = [vp, ctypes.c_int]
+    lib.vecsz_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, u64]
+    lib.vecsz_server_destroy.argtypes = [vp]
+    lib.vecsz_server_submit_compress.argtypes = [vp, P, vp, vp, u64, u64p]
+    lib.vecsz_server_submit_decompress.argtypes = [vp, P, vp, u64, vp, u64, u64p]
+    lib.vecsz_server_wait.argtypes = [vp, u64, ctypes.POINTER(ServerResult)]
     lib.vecsz_ctx_enable_timing.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int]

@@ -13,6 +13,10 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -963,6 +967,184 @@ int vecsz_decompress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, con
         if (!d_in[k] || !d_out[k]) return fail(w, VECSZ_EINVAL, "NULL field");
         return decompress_device(w, &p[k], d_in[k], in_bytes[k], d_out[k], out_cap[k]);
     });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- server
+
+namespace {
+
+// Device bytes a request holds while it runs: its input and output buffers
+// plus the scratch its slot's context needs for it.
+uint64_t request_bytes(int kind, const vecsz_params* p, uint64_t in_bytes, uint64_t out_cap) {
+    uint64_t n = 1;
+    for (int i = 0; i < std::max(1, std::min(3, p->ndim)); i++) n *= p->dims[i];
+    if (kind == 0) {
+        const uint64_t nch = (n + kChunk - 1) / kChunk;
+        return 4 * n + out_cap + 6 * n + 24 * nch + (uint64_t)768 * p->dict_size * 4 + (1 << 20);
+    }
+    const uint64_t nsub = in_bytes * 8 / kSubBits + 2;
+    return in_bytes + 4 * out_cap + 6 * out_cap + 50 * nsub + (1 << 20);
+}
+
+struct Request {
+    int kind = 0;  // 0 compress, 1 decompress
+    vecsz_params p{};
+    const void* in = nullptr;
+    void* out = nullptr;
+    uint64_t in_bytes = 0, out_cap = 0, need = 0;
+    int status = 0;
+    uint64_t out_bytes = 0, n_outlier = 0;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    bool done = false;
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct vecsz_server {
+    int device = 0;
+    uint64_t budget = 0, in_use = 0, next_ticket = 1;
+    bool stop = false;
+    std::vector<vecsz_ctx*> slots;
+    std::vector<std::thread> threads;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Request> reqs;
+
+    void slot_loop(vecsz_ctx* c) {
+        (void)hipSetDevice(c->device);
+        for (;;) {
+            uint64_t t;
+            Request r;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                // FIFO admission: the oldest request starts once a slot is
+                // free and its bytes fit in what the budget has left
+                cv_work.wait(lk, [&] { return stop || (!queue.empty() && reqs[queue.front()].need <= budget - in_use); });
+                if (stop && queue.empty()) return;
+                if (queue.empty() || reqs[queue.front()].need > budget - in_use) continue;
+                t = queue.front();
+                queue.pop_front();
+                Request& q = reqs[t];
+                in_use += q.need;
+                q.t_start = now_ms();
+                r = q;
+            }
+            int rc;
+            if (r.kind == 0) {
+                rc = compress_device(c, &r.p, static_cast<const float*>(r.in), static_cast<uint8_t*>(r.out), r.out_cap,
+                                     &r.out_bytes, &r.n_outlier);
+            } else {
+                rc = decompress_device(c, &r.p, static_cast<const uint8_t*>(r.in), r.in_bytes, static_cast<float*>(r.out),
+                                       r.out_cap);
+                if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VECSZ_EHIP;
+            }
+            {
+                std::lock_guard<std::mutex> lk(m);
+                Request& q = reqs[t];
+                q.status = rc;
+                q.out_bytes = r.out_bytes;
+                q.n_outlier = r.n_outlier;
+                q.t_end = now_ms();
+                q.done = true;
+                in_use -= q.need;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int vecsz_server_create(vecsz_server** out, int device, int nslots, uint64_t hbm_budget) {
+    if (!out || nslots < 1 || nslots > 16 || hbm_budget == 0) return VECSZ_EINVAL;
+    *out = nullptr;
+    vecsz_server* s = new vecsz_server();
+    s->device = device;
+    s->budget = hbm_budget;
+    for (int i = 0; i < nslots; i++) {
+        vecsz_ctx* c = nullptr;
+        const int rc = vecsz_ctx_create(&c, device, nullptr);
+        if (rc) {
+            for (vecsz_ctx* x : s->slots) vecsz_ctx_destroy(x);
+            delete s;
+            return rc;
+        }
+        s->slots.push_back(c);
+    }
+    for (vecsz_ctx* c : s->slots) s->threads.emplace_back([s, c] { s->slot_loop(c); });
+    *out = s;
+    return VECSZ_OK;
+}
+
+int vecsz_server_destroy(vecsz_server* s) {
+    if (!s) return VECSZ_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->threads) t.join();
+    for (vecsz_ctx* c : s->slots) vecsz_ctx_destroy(c);
+    delete s;
+    return VECSZ_OK;
+}
+
+static int server_submit(vecsz_server* s, int kind, const vecsz_params* p, const void* in, uint64_t in_bytes, void* out,
+                         uint64_t out_cap, uint64_t* ticket) {
+    if (!s || !p || !in || !out || !ticket) return VECSZ_EINVAL;
+    Request r;
+    r.kind = kind;
+    r.p = *p;
+    r.in = in;
+    r.out = out;
+    r.in_bytes = in_bytes;
+    r.out_cap = out_cap;
+    r.need = request_bytes(kind, p, in_bytes, out_cap);
+    if (r.need > s->budget) return VECSZ_ENOMEM;  // could never be admitted
+    r.t_submit = now_ms();
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        *ticket = s->next_ticket++;
+        s->reqs[*ticket] = r;
+        s->queue.push_back(*ticket);
+    }
+    s->cv_work.notify_one();
+    return VECSZ_OK;
+}
+
+int vecsz_server_submit_compress(vecsz_server* s, const vecsz_params* p, const float* d_in, uint8_t* d_out,
+                                 uint64_t out_cap, uint64_t* ticket) {
+    return server_submit(s, 0, p, d_in, 0, d_out, out_cap, ticket);
+}
+
+int vecsz_server_submit_decompress(vecsz_server* s, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
+                                   float* d_out, uint64_t out_cap, uint64_t* ticket) {
+    return server_submit(s, 1, p, d_in, in_bytes, d_out, out_cap, ticket);
+}
+
+int vecsz_server_wait(vecsz_server* s, uint64_t ticket, vecsz_server_result* res) {
+    if (!s || !res) return VECSZ_EINVAL;
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return VECSZ_EINVAL;
+    s->cv_done.wait(lk, [&] { return s->reqs[ticket].done; });
+    const Request& r = s->reqs[ticket];
+    res->status = r.status;
+    res->out_bytes = r.out_bytes;
+    res->n_outlier = r.n_outlier;
+    res->queue_ms = r.t_start - r.t_submit;
+    res->run_ms = r.t_end - r.t_start;
+    res->latency_ms = r.t_end - r.t_submit;
+    s->reqs.erase(ticket);
+    return VECSZ_OK;
 }
 
 int vecsz_ctx_enable_timing(vecsz_ctx* c, int on) {

@@ -296,6 +296,59 @@ class Context:
         return y.reshape(shape)
 
 
+class Server:
+    """Request server on one GPU (vecsz_server_*): submit compress and
+    decompress requests on device buffers, wait for each by ticket.  Requests
+    overlap on the GPU across `slots` slot contexts; admission keeps the bytes
+    held by running requests under `hbm_budget`."""
+
+    def __init__(self, device: int = 0, slots: int = 8, hbm_budget: int = 200 * 10**9):
+        self._lib = _lib.load()
+        s = ctypes.c_void_p()
+        rc = self._lib.vecsz_server_create(ctypes.byref(s), device, slots, int(hbm_budget))
+        if rc:
+            raise VecszError(rc, self._lib.vecsz_strerror(rc).decode())
+        self._s = s
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._lib.vecsz_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ticket(self, rc, t):
+        if rc:
+            raise VecszError(rc, self._lib.vecsz_strerror(rc).decode())
+        return t.value
+
+    def submit_compress(self, params: Params, d_in: int, d_out: int, out_cap: int) -> int:
+        t = ctypes.c_uint64()
+        rc = self._lib.vecsz_server_submit_compress(self._s, ctypes.byref(params), ctypes.c_void_p(d_in),
+                                                    ctypes.c_void_p(d_out), out_cap, ctypes.byref(t))
+        return self._ticket(rc, t)
+
+    def submit_decompress(self, params: Params, d_in: int, in_bytes: int, d_out: int, out_cap: int) -> int:
+        t = ctypes.c_uint64()
+        rc = self._lib.vecsz_server_submit_decompress(self._s, ctypes.byref(params), ctypes.c_void_p(d_in), in_bytes,
+                                                      ctypes.c_void_p(d_out), out_cap, ctypes.byref(t))
+        return self._ticket(rc, t)
+
+    def wait(self, ticket: int) -> dict:
+        r = _lib.ServerResult()
+        rc = self._lib.vecsz_server_wait(self._s, ticket, ctypes.byref(r))
+        if rc:
+            raise VecszError(rc, self._lib.vecsz_strerror(rc).decode())
+        if r.status:
+            raise VecszError(r.status, self._lib.vecsz_strerror(r.status).decode())
+        return {"out_bytes": r.out_bytes, "n_outlier": r.n_outlier, "queue_ms": r.queue_ms, "run_ms": r.run_ms,
+                "latency_ms": r.latency_ms}
+
+
 # ---------------------------------------------------------------- reference-shaped API
 
 @dataclass
