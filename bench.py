@@ -68,7 +68,7 @@ def measured_traffic(kernel, dims, eb):
     return int(round((f * fetch_scale + w * write_scale) * 1024))
 
 
-def cpu_baseline(x_host, eb, range_, dims, dict_size, block):
+def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
     """The reference's own Compress/Decompress (oracle/_ref) when it was built,
     else the C restatement, timed on this host's cores."""
     import numpy as np
@@ -76,7 +76,8 @@ def cpu_baseline(x_host, eb, range_, dims, dict_size, block):
     from oracle import pyoracle as po
 
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    eb_abs = eb * range_  # same eb_final as r2r; avoids the reference's file re-read
+    # abs mode with eb_final: the same archive bits as r2r without the
+    # reference's re-read of the input file
     if po.ref_available():
         po.ref().ref_set_threads(threads)
         t0 = time.perf_counter()
@@ -100,8 +101,8 @@ def cpu_baseline(x_host, eb, range_, dims, dict_size, block):
         "unit": "GB/s",
         "cores": used,
         "kind": kind,
-        "sample": f"one compress of the full {'x'.join(map(str, dims))} field (abs eb = eb*range, same eb_final, "
-                  f"no file IO) on {used} OpenMP threads; decompress {4 * n / td / 1e9:.4f} GB/s",
+        "sample": f"one compress of the full {'x'.join(map(str, dims))} field in abs mode at the same eb_final "
+                  f"(no file IO) on {used} OpenMP threads; decompress {4 * n / td / 1e9:.4f} GB/s",
         "compress_s": round(tc, 3),
         "decompress_s": round(td, 3),
         "decompress_gbps": round(4 * n / td / 1e9, 4),
@@ -347,9 +348,10 @@ def main():
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--dict", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="grf512", choices=["grf512", "c4", "c5"],
-                    help="grf512: one 3D field (default line); c4: 96-field batched mix vs per-field calls; "
-                         "c5: served Poisson request stream")
+    ap.add_argument("--workload", default="grf512", choices=["grf512", "c1", "c3", "target", "c4", "c5"],
+                    help="grf512: 3D GRF of --shape at r2r --eb (default line); c1: 2D 1800x3600 sinusoids, abs "
+                         "1e-4; c3: 1D 2^28 walk + spikes, r2r 1e-4; target: 1024x512x512 GRF, r2r 1e-4; "
+                         "c4: 96-field batched mix vs per-field calls; c5: served Poisson request stream")
     ap.add_argument("--requests", type=int, default=96, help="c5: requests in the stream")
     ap.add_argument("--load", type=float, default=0.7, help="c5: offered load as a fraction of measured peak")
     ap.add_argument("--workers", default="8", help="c4: comma-separated worker counts to time (first is reported)")
@@ -370,21 +372,36 @@ def main():
     if args.workload == "c5":
         return bench_serve(args)
 
-    shape = tuple(int(s) for s in args.shape.split(","))
-    dims = tuple(reversed(shape))
     torch.cuda.set_device(0)
-    log(f"generating GRF {shape} on the GPU")
-    x = datagen.grf(shape, seed=0, device="cuda")
+    mode = "r2r"
+    if args.workload == "c1":
+        shape, mode, args.eb = (1800, 3600), "abs", 1e-4
+        log("generating the 2D sinusoid field on the host")
+        x = torch.from_numpy(datagen.sinusoids_2d(1800, 3600, seed=0)).cuda()
+        what = "2D f32 3600x1800 sinusoids + noise, abs eb=1e-4"
+    elif args.workload == "c3":
+        shape, args.eb = (1 << 28,), 1e-4
+        log("generating the 1D walk + spikes on the GPU")
+        x = datagen.walk_spikes_1d(1 << 28, seed=0, device="cuda")
+        what = "1D f32 2^28 random walk + 0.1% spikes, r2r eb=1e-4"
+    else:
+        if args.workload == "target":
+            args.shape, args.eb = "1024,512,512", 1e-4
+        shape = tuple(int(s) for s in args.shape.split(","))
+        log(f"generating GRF {shape} on the GPU")
+        x = datagen.grf(shape, seed=0, device="cuda")
+        what = f"{len(shape)}D f32 {'x'.join(map(str, reversed(shape)))} GRF r2r eb={args.eb}"
+    dims = tuple(reversed(shape))
     torch.cuda.synchronize()
     n = x.numel()
-    value_range = float((x.max() - x.min()).item())
+    value_range = float((x.max() - x.min()).item()) if mode == "r2r" else 0.0
 
     ctx = Context(0)
-    p = make_params(dims, args.eb, "r2r", args.block, args.dict)
+    p = make_params(dims, args.eb, mode, args.block, args.dict)
     cap = ctx.compress_bound(p)
     arch = torch.empty(cap, dtype=torch.uint8, device="cuda")
     y = torch.empty_like(x)
-    pd = make_params((1,), 1e-4, "r2r", args.block, args.dict, value_range)
+    pd = make_params((1,), 1e-4, mode, args.block, args.dict, value_range)
 
     def comp():
         return ctx.compress_into(p, x.data_ptr(), arch.data_ptr(), cap)
@@ -432,7 +449,7 @@ def main():
     # reference's fp32 slack (bit-exact parity is tested in tests/).
     err = (x.double() - y.double()).abs()
     slack = x.double().abs() * 2.0 ** -22
-    eb_final = args.eb * value_range
+    eb_final = args.eb * value_range if mode == "r2r" else args.eb
     bound_ok = bool((err <= eb_final + slack).all().item())
 
     # per-stage device time: HIP events on the library's stream, averaged over
@@ -480,9 +497,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (GRF E(k)~k^-5/3, seed 0, generated on the GPU)",
-        "config": {"workload": f"3D f32 {'x'.join(map(str, dims))} GRF r2r eb={args.eb} compress",
-                   "dims_xyz": list(dims), "eb": args.eb, "mode": "r2r", "block": args.block,
+        "data": "synthetic, seed 0 (see vecsz_amd/datagen.py)",
+        "config": {"workload": what + " compress",
+                   "dims_xyz": list(dims), "eb": args.eb, "mode": mode, "block": args.block,
                    "dict_size": args.dict, "parallelism": "single-gpu"},
         "decompress_gbps": round(dec_gbps, 2),
         "decompress_ms": round(td * 1e3, 4),
@@ -500,7 +517,7 @@ def main():
     if not args.no_cpu:
         log("timing the CPU baseline")
         x_host = x.cpu().numpy()
-        out["cpu_baseline"] = cpu_baseline(x_host, args.eb, value_range, dims, args.dict, args.block)
+        out["cpu_baseline"] = cpu_baseline(x_host, eb_final, dims, args.dict, args.block)
         out["speedup_vs_cpu"] = round(comp_gbps / out["cpu_baseline"]["value"], 2)
     else:
         out["cpu_baseline"] = None
