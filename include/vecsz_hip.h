@@ -154,6 +154,71 @@ int vecsz_server_wait(vecsz_server* server, uint64_t ticket, vecsz_server_result
 int vecsz_huffman_codebook(const uint64_t* h_freq, int dict, uint64_t* h_code_bits, uint8_t* h_code_len,
                            uint8_t* h_tree, uint64_t tree_cap, uint64_t* tree_bytes, int* node_count);
 
+/* Metadata the archive does not carry (mode, B, dict, the r2r range and
+ * eb_final: reference defect D7), as a small fixed-size record to keep beside
+ * an archive so that decompress needs nothing else.  No counterpart in the
+ * reference (it takes these from its command line).  The archive itself is
+ * unchanged. */
+#define VECSZ_SIDECAR_MAGIC 0x4D5A5356u /* "VSZM" */
+typedef struct vecsz_sidecar {
+    uint32_t magic;      /* VECSZ_SIDECAR_MAGIC */
+    uint32_t version;    /* 1 */
+    int32_t ndim;
+    int32_t mode;        /* VECSZ_MODE_ABS or VECSZ_MODE_R2R */
+    int32_t block;
+    int32_t dict_size;
+    double eb;           /* user eb, as stored in the archive */
+    double value_range;  /* r2r: the range compress used; abs: 0 */
+    double eb_final;
+} vecsz_sidecar;
+/* The record of this context's last successful compress. */
+int vecsz_ctx_last_sidecar(vecsz_ctx* ctx, vecsz_sidecar* out);
+/* Decompress parameters from a record (EINVAL on a bad magic or version). */
+int vecsz_params_from_sidecar(const vecsz_sidecar* s, vecsz_params* p);
+
+/* Quality of a reconstruction, computed on the device: VerifyData
+ * (src/utils/verify.hh:30-74) into the reference's stat_t fields
+ * (src/types.hh:52-63).  Per-element arithmetic is the reference's; its sums
+ * are taken in parallel order, so they match the reference's serial sums to
+ * rounding; min, max, max_abserr and its first index are exact. */
+typedef struct vecsz_stat {
+    double minimum, maximum, range;
+    double PSNR, MSE, NRMSE;
+    double coeff;
+    double max_abserr_vs_range, max_pwr_rel_abserr;
+    uint64_t len, max_abserr_index;
+    double max_abserr;
+} vecsz_stat;
+int vecsz_verify(vecsz_ctx* ctx, const float* d_orig, const float* d_recon, uint64_t n, vecsz_stat* out);
+
+/* Lossless pass over an archive payload, on the host (src/lossless.hh:179-219,
+ * called at level 3 from src/vecsz_interface.hh:206-217).  GZIP uses zlib
+ * 1.2.11, the reference's release, so the bytes match the reference's; ZSTD
+ * uses the system zstd runtime, whose frames decode both ways but whose
+ * compressed bytes differ from the reference's 1.3.5.  VECSZ_EINVAL when the
+ * zstd runtime is absent. */
+#define VECSZ_LOSSLESS_NONE 0
+#define VECSZ_LOSSLESS_GZIP 1
+#define VECSZ_LOSSLESS_ZSTD 2
+int vecsz_lossless_bound(int kind, uint64_t bytes, uint64_t* bound);
+int vecsz_lossless_compress(int kind, const uint8_t* h_in, uint64_t bytes, uint8_t* h_out, uint64_t out_cap,
+                            uint64_t* out_bytes);
+/* Kind of a lossless stream by its first bytes, as
+ * is_lossless_compressed_data (src/lossless.hh:60-75); 0 when neither. */
+int vecsz_lossless_kind(const uint8_t* h_in, uint64_t bytes);
+/* Inverse of the pass: exactly `target` bytes (the payload size) or ECORRUPT. */
+int vecsz_lossless_decompress(const uint8_t* h_in, uint64_t bytes, uint8_t* h_out, uint64_t target,
+                              uint64_t* out_bytes);
+
+/* .sz file container (src/vecsz.cc:43-56): [u64 lossless_size][u64 data_size]
+ * then the lossless stream (lossless_size > 0) or the payload.  The
+ * reference writes data_size bytes after the header even when the stream is
+ * lossless_size long (D8); pack writes the exact body, unpack accepts both. */
+int vecsz_sz_pack(const uint8_t* h_body, uint64_t body_bytes, uint64_t lossless_size, uint64_t data_size,
+                  uint8_t* h_out, uint64_t out_cap, uint64_t* out_bytes);
+int vecsz_sz_unpack(const uint8_t* h_file, uint64_t bytes, uint64_t* lossless_size, uint64_t* data_size,
+                    uint64_t* body_offset, uint64_t* body_bytes);
+
 /* Per-stage device timing of the last compress/decompress call, measured
  * with HIP events on the context's stream when enabled (off by default).
  * names/ms receive up to `cap` entries; returns the entry count. */

@@ -19,14 +19,19 @@
 //  - r2r: Compress takes the value range from data_in itself (the reference
 //    re-reads ap->files.input_file, which holds the same floats, :49-50);
 //    Decompress re-reads ap->files.input_file like the reference (:280-284).
-// Differences: errors throw std::runtime_error instead of exit(1); the
-// zstd/gzip pass (ap->szwf.lossless_pass) is not performed here and makes
-// Compress throw; only T = float is supported (the only type the reference
-// instantiates, src/vecsz.cc:37).
+//  - with ap->szwf.lossless_pass, Compress runs gzip (szwf.lossless_gzip) or
+//    zstd at level 3 over the payload and returns that malloc'ed stream with
+//    *lossless_size set (:206-217); Decompress inflates a lossless_size > 0
+//    stream to the target_size payload first (:237-244).
+// Differences: errors throw std::runtime_error instead of exit(1); only
+// T = float is supported (the only type the reference instantiates,
+// src/vecsz.cc:37); the zstd pass uses the system zstd runtime (its frames
+// decode both ways, its bytes differ from the reference's zstd 1.3.5).
 #ifndef VECSZ_AMD_INTERFACE_HH
 #define VECSZ_AMD_INTERFACE_HH
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -95,7 +100,6 @@ vecsz_params params_of(const AP* ap) {
 template <typename T, typename Q, typename AP>
 void* Compress(AP* ap, T* data_in, size_t* num_outlier, size_t* data_size, unsigned long* lossless_size) {
     static_assert(std::is_same<T, float>::value, "vecSZ's path is float32");
-    if (ap->szwf.lossless_pass) throw std::runtime_error("vecsz: the zstd/gzip pass is not part of this library");
     vecsz_params p = detail::params_of(ap);
     size_t len = 1;
     for (int i = 0; i < p.ndim; i++) len *= (size_t)p.dims[i];
@@ -105,11 +109,25 @@ void* Compress(AP* ap, T* data_in, size_t* num_outlier, size_t* data_size, unsig
     std::vector<uint8_t> tmp(bound);
     uint64_t size = 0, nout = 0;
     detail::check(vecsz_compress_host(detail::context(), &p, data_in, tmp.data(), bound, &size, &nout));
-    unsigned char* out = new unsigned char[size];
-    std::memcpy(out, tmp.data(), size);
     *num_outlier += (size_t)nout;
     *data_size = (size_t)size;
     if (lossless_size) *lossless_size = 0;
+    if (ap->szwf.lossless_pass) {
+        const int kind = ap->szwf.lossless_gzip ? VECSZ_LOSSLESS_GZIP : VECSZ_LOSSLESS_ZSTD;
+        uint64_t lbound = 0, lsize = 0;
+        detail::check(vecsz_lossless_bound(kind, size, &lbound));
+        unsigned char* lout = static_cast<unsigned char*>(std::malloc(lbound));
+        if (!lout) throw std::bad_alloc();
+        const int rc = vecsz_lossless_compress(kind, tmp.data(), size, lout, lbound, &lsize);
+        if (rc != VECSZ_OK) {
+            std::free(lout);
+            detail::check(rc);
+        }
+        if (lossless_size) *lossless_size = (unsigned long)lsize;
+        return lout;  // the reference keeps data_in in this branch (:217)
+    }
+    unsigned char* out = new unsigned char[size];
+    std::memcpy(out, tmp.data(), size);
     delete[] data_in;
     return out;
 }
@@ -117,7 +135,13 @@ void* Compress(AP* ap, T* data_in, size_t* num_outlier, size_t* data_size, unsig
 template <typename T, typename Q, typename AP>
 void* Decompress(AP* ap, unsigned char* tmp_data, size_t target_size, unsigned long lossless_size) {
     static_assert(std::is_same<T, float>::value, "vecSZ's path is float32");
-    if (lossless_size) throw std::runtime_error("vecsz: the zstd/gzip pass is not part of this library");
+    std::vector<uint8_t> inflated;
+    if (lossless_size) {
+        inflated.resize(target_size);
+        uint64_t got = 0;
+        detail::check(vecsz_lossless_decompress(tmp_data, lossless_size, inflated.data(), target_size, &got));
+        tmp_data = inflated.data();
+    }
     int ndim = 0;
     uint64_t dims[3];
     double eb = 0;

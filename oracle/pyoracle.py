@@ -79,6 +79,11 @@ def ref():
         lib.ref_free_f32.argtypes = [_c_vp]
         lib.ref_set_threads.argtypes = [ctypes.c_int]
         lib.ref_set_threads.restype = ctypes.c_int
+        lib.ref_lossless_compress.argtypes = [ctypes.c_int, _c_vp, ctypes.c_ulong, ctypes.POINTER(_c_vp)]
+        lib.ref_lossless_compress.restype = ctypes.c_ulong
+        lib.ref_lossless_decompress.argtypes = [_c_vp, ctypes.c_ulong, ctypes.POINTER(_c_vp), ctypes.c_ulong]
+        lib.ref_lossless_decompress.restype = ctypes.c_long
+        lib.ref_free_malloc.argtypes = [_c_vp]
         _ref = lib
     return _ref
 
@@ -263,3 +268,27 @@ def ref_decompress(archive: bytes, original: np.ndarray | None = None, r2r: bool
         if tmp is not None:
             os.unlink(tmp.name)
     return y
+
+
+def ref_lossless_compress(kind: str, data: bytes) -> bytes:
+    """The reference's own lossless pass (src/lossless.hh:179-198, level 3)."""
+    lib = ref()
+    buf = np.frombuffer(data, dtype=np.uint8)
+    out = _c_vp()
+    n = lib.ref_lossless_compress(1 if kind == "zstd" else 0, buf.ctypes.data, len(data), ctypes.byref(out))
+    res = ctypes.string_at(out.value, n)
+    lib.ref_free_malloc(out)
+    return res
+
+
+def ref_lossless_decompress(data: bytes, target: int) -> bytes:
+    """The reference's detection + inverse (src/lossless.hh:60-75, :200-218)."""
+    lib = ref()
+    buf = np.frombuffer(data, dtype=np.uint8)
+    out = _c_vp()
+    n = lib.ref_lossless_decompress(buf.ctypes.data, len(data), ctypes.byref(out), target)
+    if n < 0:
+        raise OracleError("not a lossless stream")
+    res = ctypes.string_at(out.value, n)
+    lib.ref_free_malloc(out)
+    return res

@@ -200,4 +200,20 @@ int ref_decompress_pad0(int r2r, int dict, int block, const char* path, const ui
 void ref_free(void* p) { delete[] (uint8_t*)p; }
 void ref_free_f32(float* p) { delete[] p; }
 
+// The reference's lossless pass (src/lossless.hh:179-198) at its level 3;
+// the result is malloc'ed (free with ref_free_malloc).
+unsigned long ref_lossless_compress(int zstd, const uint8_t* in, unsigned long n, uint8_t** out) {
+    return vecsz::lossless::sz_lossless_compress(zstd ? ZSTD_COMPRESSOR : GZIP_COMPRESSOR, 3,
+                                                 const_cast<unsigned char*>(in), n, out);
+}
+
+// Detection + inverse (src/lossless.hh:60-75, :200-218); -1 when undetected.
+long ref_lossless_decompress(const uint8_t* in, unsigned long n, uint8_t** out, unsigned long target) {
+    const int kind = vecsz::lossless::is_lossless_compressed_data(const_cast<unsigned char*>(in), n);
+    if (kind < 0) return -1;
+    return (long)vecsz::lossless::sz_lossless_decompress(kind, const_cast<unsigned char*>(in), n, out, target);
+}
+
+void ref_free_malloc(void* p) { free(p); }
+
 }  // extern "C"

@@ -1,0 +1,75 @@
+// Drives include/vecsz_interface.hh the way the reference's main does
+// (src/vecsz.cc:37, 59): Compress<float,int>, then Decompress<float,int> on
+// the payload, with an ArgParse carrying the fields the interface reads.
+// Prints one line: "nodevice" when no GPU is usable, else
+// "ok <data_size> <lossless_size> <max_abs_err> <eb_final>".
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "vecsz_interface.hh"
+
+struct Dim4 {
+    int _0 = 1, _1 = 1, _2 = 1, _3 = 1;
+};
+struct Files {
+    std::string input_file;
+};
+struct Szwf {
+    bool lossless_pass = false, lossless_gzip = false;
+};
+struct ArgParse {
+    int ndim = 3;
+    Dim4 dim4;
+    std::string mode = "abs";
+    double eb = 1e-4;
+    int block_size = 8;
+    int dict_size = 4096;
+    Szwf szwf;
+    Files files;
+};
+
+int main(int argc, char** argv) {
+    const std::string lossless = argc > 1 ? argv[1] : "none";
+    ArgParse ap;
+    ap.dim4 = {40, 30, 20, 1};
+    ap.eb = 1e-3;
+    ap.szwf.lossless_pass = lossless != "none";
+    ap.szwf.lossless_gzip = lossless == "gzip";
+    const size_t n = 40 * 30 * 20;
+    float* x = new float[n];
+    float* orig = new float[n];
+    for (size_t i = 0; i < n; i++) orig[i] = x[i] = (float)(std::sin(0.01 * (double)i) * 3.0 + 0.001 * (double)(i % 7));
+    size_t nout = 0, size = 0;
+    unsigned long lsize = 0;
+    void* arch = nullptr;
+    try {
+        arch = vecsz::interface::Compress<float, int>(&ap, x, &nout, &size, &lsize);
+    } catch (const std::runtime_error& e) {
+        if (std::strstr(e.what(), "no usable GPU")) {
+            std::printf("nodevice\n");
+            return 0;
+        }
+        std::printf("error %s\n", e.what());
+        return 1;
+    }
+    if (!lsize) x = nullptr;  // Compress deleted it, like the reference
+    ArgParse ap2;
+    ap2.mode = "abs";
+    ap2.szwf = ap.szwf;
+    float* y = (float*)vecsz::interface::Decompress<float, int>(&ap2, (unsigned char*)arch, size, lsize);
+    double maxerr = 0;
+    for (size_t i = 0; i < n; i++) maxerr = std::fmax(maxerr, std::fabs((double)y[i] - (double)orig[i]));
+    std::printf("ok %zu %lu %.9g %.9g %d %d %d\n", size, lsize, maxerr, ap.eb, ap2.ndim, ap2.dim4._0, ap2.dim4._2);
+    if (lsize) {
+        std::free(arch);
+        delete[] x;
+    } else {
+        delete[] (unsigned char*)arch;
+    }
+    delete[] y;
+    delete[] orig;
+    return 0;
+}

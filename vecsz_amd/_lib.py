@@ -25,7 +25,9 @@ EXPORTS = (
     "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
     "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_compress_batch", "vecsz_decompress_batch",
     "vecsz_ctx_set_workers", "vecsz_server_create", "vecsz_server_destroy", "vecsz_server_submit_compress",
-    "vecsz_server_submit_decompress", "vecsz_server_wait", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
+    "vecsz_server_submit_decompress", "vecsz_server_wait", "vecsz_lossless_bound", "vecsz_lossless_compress",
+    "vecsz_lossless_kind", "vecsz_lossless_decompress", "vecsz_sz_pack", "vecsz_sz_unpack", "vecsz_ctx_last_sidecar",
+    "vecsz_params_from_sidecar", "vecsz_verify", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
 )
 
 
@@ -39,6 +41,30 @@ class Params(ctypes.Structure):
         ("block", ctypes.c_int),
         ("dict_size", ctypes.c_int),
     ]
+
+
+VECSZ_LOSSLESS_GZIP = 1
+VECSZ_LOSSLESS_ZSTD = 2
+
+
+class Sidecar(ctypes.Structure):
+    _fields_ = [
+        ("magic", ctypes.c_uint32),
+        ("version", ctypes.c_uint32),
+        ("ndim", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("dict_size", ctypes.c_int32),
+        ("eb", ctypes.c_double),
+        ("value_range", ctypes.c_double),
+        ("eb_final", ctypes.c_double),
+    ]
+
+
+class Stat(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("minimum", "maximum", "range", "PSNR", "MSE", "NRMSE", "coeff",
+                                               "max_abserr_vs_range", "max_pwr_rel_abserr")] + [
+        ("len", ctypes.c_uint64), ("max_abserr_index", ctypes.c_uint64), ("max_abserr", ctypes.c_double)]
 
 
 class ServerResult(ctypes.Structure):
@@ -95,6 +121,15 @@ def load():
     lib.vecsz_server_submit_compress.argtypes = [vp, P, vp, vp, u64, u64p]
     lib.vecsz_server_submit_decompress.argtypes = [vp, P, vp, u64, vp, u64, u64p]
     lib.vecsz_server_wait.argtypes = [vp, u64, ctypes.POINTER(ServerResult)]
+    lib.vecsz_lossless_bound.argtypes = [ctypes.c_int, u64, u64p]
+    lib.vecsz_lossless_compress.argtypes = [ctypes.c_int, vp, u64, vp, u64, u64p]
+    lib.vecsz_lossless_kind.argtypes = [vp, u64]
+    lib.vecsz_lossless_decompress.argtypes = [vp, u64, vp, u64, u64p]
+    lib.vecsz_sz_pack.argtypes = [vp, u64, u64, u64, vp, u64, u64p]
+    lib.vecsz_sz_unpack.argtypes = [vp, u64, u64p, u64p, u64p, u64p]
+    lib.vecsz_ctx_last_sidecar.argtypes = [vp, ctypes.POINTER(Sidecar)]
+    lib.vecsz_params_from_sidecar.argtypes = [ctypes.POINTER(Sidecar), P]
+    lib.vecsz_verify.argtypes = [vp, vp, vp, u64, ctypes.POINTER(Stat)]
     lib.vecsz_ctx_enable_timing.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_timings.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                       ctypes.c_int]

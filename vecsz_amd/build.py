@@ -14,7 +14,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libvecsz_hip.so")
 ARCH = os.environ.get("VECSZ_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "kernels_compress.hip", "kernels_decompress.hip", "codebook.cpp"]
+SOURCES = ["api.hip", "kernels_compress.hip", "kernels_decompress.hip", "codebook.cpp", "lossless.cpp", "kernels_verify.hip"]
+# zlib (the same 1.2.11 release the reference vendors) for the gzip pass;
+# libdl to load the system zstd runtime on first use
+LIBS = ["-lz", "-ldl"]
 
 # -ffp-contract=off: no fused multiply-add may change a rounding the
 #   reference performs separately (prequant product, reconstruction scale).
@@ -43,7 +46,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *FLAGS, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc(), *FLAGS, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES] + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, cwd=CSRC, capture_output=True, text=True)

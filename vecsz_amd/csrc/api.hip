@@ -106,6 +106,13 @@ struct vecsz_ctx {
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     Codebook cb;
     DecodeTables dt;  // kept between calls: its vectors keep their capacity
+    // metadata of the last successful compress
+    vecsz_sidecar sidecar{};
+    bool have_sidecar = false;
+    double sc_range = 0, sc_eb_final = 0;
+    // verify partials
+    DevBuf vparts;
+    HostBuf vpin;
     // decompress scratch
     DevBuf tabs, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sym_off, zero_off, scan_tmp, scan_tot,
         dec_sum, flagged, nflagged;
@@ -336,6 +343,11 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         EbDev e;
         memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
         if (e.zero_range) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
+        c->sc_range = e.range;
+        c->sc_eb_final = e.eb_final;
+    } else {
+        c->sc_range = r2r ? p->value_range : 0.0;
+        c->sc_eb_final = eb.eb_final;
     }
 
     Codebook& cb = c->cb;
@@ -415,6 +427,17 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     *out_bytes = S.archive_size;
     if (n_outlier) *n_outlier = S.n_outlier;
     if (e) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    vecsz_sidecar& sc = c->sidecar;
+    sc.magic = VECSZ_SIDECAR_MAGIC;
+    sc.version = 1;
+    sc.ndim = ndim;
+    sc.mode = p->mode;
+    sc.block = B;
+    sc.dict_size = dict;
+    sc.eb = p->eb;
+    sc.value_range = c->sc_range;
+    sc.eb_final = c->sc_eb_final;
+    c->have_sidecar = true;
     return VECSZ_OK;
 }
 
@@ -1144,6 +1167,78 @@ int vecsz_server_wait(vecsz_server* s, uint64_t ticket, vecsz_server_result* res
     res->run_ms = r.t_end - r.t_start;
     res->latency_ms = r.t_end - r.t_submit;
     s->reqs.erase(ticket);
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_last_sidecar(vecsz_ctx* c, vecsz_sidecar* out) {
+    if (!c || !out) return VECSZ_EINVAL;
+    if (!c->have_sidecar) return fail(c, VECSZ_EINVAL, "no compress has completed on this context");
+    *out = c->sidecar;
+    return VECSZ_OK;
+}
+
+int vecsz_params_from_sidecar(const vecsz_sidecar* sc, vecsz_params* p) {
+    if (!sc || !p || sc->magic != VECSZ_SIDECAR_MAGIC || sc->version != 1) return VECSZ_EINVAL;
+    vecsz_params_default(p);
+    p->ndim = sc->ndim;
+    p->mode = sc->mode;
+    p->eb = sc->eb;
+    p->value_range = sc->value_range;
+    p->block = sc->block;
+    p->dict_size = sc->dict_size;
+    return VECSZ_OK;
+}
+
+int vecsz_verify(vecsz_ctx* c, const float* d_orig, const float* d_recon, uint64_t n, vecsz_stat* out) {
+    if (!c || !d_orig || !d_recon || !out || n == 0) return fail(c, VECSZ_EINVAL, "bad verify arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int grid = verify_grid(n);
+    const size_t bytes = (size_t)grid * std::max(sizeof(VerifyPart1), sizeof(VerifyPart2));
+    if (!c->vparts.ensure(bytes) || !c->vpin.ensure(bytes)) return fail(c, VECSZ_ENOMEM, "verify scratch");
+    launch_verify_pass1(d_orig, d_recon, n, c->vparts.as<VerifyPart1>(), grid, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->vpin.p, c->vparts.p, grid * sizeof(VerifyPart1), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const VerifyPart1* p1 = c->vpin.as<VerifyPart1>();
+    double so = 0, sx = 0, sq = 0;
+    float mn = p1[0].min_o, mx = p1[0].max_o, err = p1[0].max_err, rel = 0;
+    uint64_t idx = p1[0].err_idx;
+    for (int g = 0; g < grid; g++) {
+        so += p1[g].sum_o;
+        sx += p1[g].sum_x;
+        sq += p1[g].sum_sq;
+        mn = std::min(mn, p1[g].min_o);
+        mx = std::max(mx, p1[g].max_o);
+        rel = std::max(rel, p1[g].max_rel);
+        if (p1[g].max_err > err || (p1[g].max_err == err && p1[g].err_idx < idx)) {
+            err = p1[g].max_err;
+            idx = p1[g].err_idx;
+        }
+    }
+    const double mean_o = so / (double)n, mean_x = sx / (double)n;
+    launch_verify_pass2(d_orig, d_recon, n, mean_o, mean_x, c->vparts.as<VerifyPart2>(), grid, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->vpin.p, c->vparts.p, grid * sizeof(VerifyPart2), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const VerifyPart2* p2 = c->vpin.as<VerifyPart2>();
+    double corr = 0, vo = 0, vx = 0;
+    for (int g = 0; g < grid; g++) {
+        corr += p2[g].corr;
+        vo += p2[g].var_o;
+        vx += p2[g].var_x;
+    }
+    // the closing formulas of VerifyData (src/utils/verify.hh:60-73)
+    const double std_o = std::sqrt(vo / (double)n), std_x = std::sqrt(vx / (double)n);
+    out->len = n;
+    out->coeff = corr / (double)n / std_o / std_x;
+    out->maximum = mx;
+    out->minimum = mn;
+    out->range = (double)mx - (double)mn;
+    out->max_abserr_index = idx;
+    out->max_abserr = err;
+    out->max_abserr_vs_range = err / out->range;
+    out->max_pwr_rel_abserr = rel;
+    out->MSE = sq / (double)n;
+    out->NRMSE = std::sqrt(out->MSE) / out->range;
+    out->PSNR = 20 * std::log10(out->range) - 10 * std::log10(out->MSE);
     return VECSZ_OK;
 }
 

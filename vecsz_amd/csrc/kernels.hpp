@@ -67,4 +67,18 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
 void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
                                  const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st);
 
+// Reconstruction quality (kernels_verify.hip)
+struct VerifyPart1 {
+    double sum_o, sum_x, sum_sq;
+    float min_o, max_o, max_err, max_rel;
+    uint64_t err_idx;
+};
+struct VerifyPart2 {
+    double corr, var_o, var_x;
+};
+int verify_grid(uint64_t n);
+void launch_verify_pass1(const float* o, const float* x, uint64_t n, VerifyPart1* part, int grid, hipStream_t st);
+void launch_verify_pass2(const float* o, const float* x, uint64_t n, double mean_o, double mean_x, VerifyPart2* part,
+                         int grid, hipStream_t st);
+
 }  // namespace vsz

@@ -101,6 +101,88 @@ def huffman_codebook(freq, dict_size: Optional[int] = None):
     return bits, lens, nc.value, tree.tobytes()
 
 
+LOSSLESS = {"gzip": _lib.VECSZ_LOSSLESS_GZIP if hasattr(_lib, "VECSZ_LOSSLESS_GZIP") else 1,
+            "zstd": _lib.VECSZ_LOSSLESS_ZSTD if hasattr(_lib, "VECSZ_LOSSLESS_ZSTD") else 2}
+
+
+def _buf(b) -> np.ndarray:
+    return np.frombuffer(bytes(b), dtype=np.uint8)
+
+
+def lossless_compress(kind: str, payload) -> bytes:
+    """The reference's lossless pass (gzip or zstd, level 3) over a payload
+    (src/lossless.hh:179-198)."""
+    lib = _lib.load()
+    k = LOSSLESS[kind]
+    src = _buf(payload)
+    bound = ctypes.c_uint64()
+    rc = lib.vecsz_lossless_bound(k, src.size, ctypes.byref(bound))
+    if rc:
+        raise VecszError(rc, f"{kind} pass unavailable")
+    out = np.empty(bound.value, dtype=np.uint8)
+    n = ctypes.c_uint64()
+    rc = lib.vecsz_lossless_compress(k, src.ctypes.data_as(ctypes.c_void_p), src.size,
+                                     out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(n))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return out[: n.value].tobytes()
+
+
+def lossless_kind(data) -> Optional[str]:
+    src = _buf(data[:64])
+    k = _lib.load().vecsz_lossless_kind(src.ctypes.data_as(ctypes.c_void_p), src.size)
+    return {1: "gzip", 2: "zstd"}.get(k)
+
+
+def lossless_decompress(data, target: int) -> bytes:
+    """Inverse of the lossless pass: the `target`-byte payload
+    (src/lossless.hh:200-218)."""
+    lib = _lib.load()
+    src = _buf(data)
+    out = np.empty(int(target), dtype=np.uint8)
+    n = ctypes.c_uint64()
+    rc = lib.vecsz_lossless_decompress(src.ctypes.data_as(ctypes.c_void_p), src.size,
+                                       out.ctypes.data_as(ctypes.c_void_p), int(target), ctypes.byref(n))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return out.tobytes()
+
+
+def sz_pack(body, lossless_size: int, data_size: int) -> bytes:
+    """.sz file bytes (src/vecsz.cc:43-56): header + body."""
+    lib = _lib.load()
+    src = _buf(body)
+    out = np.empty(src.size + 16, dtype=np.uint8)
+    n = ctypes.c_uint64()
+    rc = lib.vecsz_sz_pack(src.ctypes.data_as(ctypes.c_void_p), src.size, int(lossless_size), int(data_size),
+                           out.ctypes.data_as(ctypes.c_void_p), out.size, ctypes.byref(n))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return out[: n.value].tobytes()
+
+
+def sz_unpack(file_bytes) -> tuple:
+    """(body, lossless_size, data_size) of .sz file bytes."""
+    lib = _lib.load()
+    src = _buf(file_bytes)
+    ls, ds, off, nb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    rc = lib.vecsz_sz_unpack(src.ctypes.data_as(ctypes.c_void_p), src.size, ctypes.byref(ls), ctypes.byref(ds),
+                             ctypes.byref(off), ctypes.byref(nb))
+    if rc:
+        raise VecszError(rc, lib.vecsz_strerror(rc).decode())
+    return src[off.value: off.value + nb.value].tobytes(), ls.value, ds.value
+
+
+def params_from_sidecar(record: bytes) -> Params:
+    """Decompress parameters from a metadata record (Context.last_sidecar)."""
+    sc = _lib.Sidecar.from_buffer_copy(bytes(record))
+    p = Params()
+    rc = _lib.load().vecsz_params_from_sidecar(ctypes.byref(sc), ctypes.byref(p))
+    if rc:
+        raise VecszError(rc, "not a vecsz metadata record")
+    return p
+
+
 def header_fields(head: bytes) -> dict:
     """ndim, dims (x first) and eb from the first bytes of an archive payload
     (datapack layout, src/utils/datapack.hh:33-47); no section-size checks."""
@@ -158,6 +240,25 @@ class Context:
         return [(names[i].decode(), ms[i]) for i in range(n)]
 
     # -------------------------------------------------------------- device API
+
+    def last_sidecar(self) -> bytes:
+        """The 48-byte metadata record (mode, B, dict, r2r range, eb_final) of
+        this context's last compress: keep it beside the archive and
+        decompress needs nothing else."""
+        sc = _lib.Sidecar()
+        self._check(self._lib.vecsz_ctx_last_sidecar(self._ctx, ctypes.byref(sc)))
+        return bytes(sc)
+
+    def verify(self, orig, recon) -> dict:
+        """VerifyData (src/utils/verify.hh:30-74) on the device for two float32
+        CUDA tensors of the same size."""
+        if orig.numel() != recon.numel():
+            raise ValueError("size mismatch")
+        o, x = orig.contiguous(), recon.contiguous()
+        st = _lib.Stat()
+        self._check(self._lib.vecsz_verify(self._ctx, ctypes.c_void_p(o.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                           o.numel(), ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
 
     def set_workers(self, n: int):
         """Worker contexts of the batched calls (1..16)."""
@@ -373,6 +474,7 @@ class ArgParse:
     block_size: int = 8
     value_range: float = 0.0
     device: int = 0
+    lossless: Optional[str] = None  # szwf.lossless_pass: None, "gzip" or "zstd" (argparse.hh:32-35)
 
 
 _default_ctx: Optional[Context] = None
@@ -386,8 +488,10 @@ def _ctx(ap: ArgParse) -> Context:
 
 
 def Compress(ap: ArgParse, data_in):
-    """vecsz::interface::Compress<float,int> (src/vecsz_interface.hh:31-229) without
-    the lossless pass.  Returns (archive, num_outlier, data_size, lossless_size=0)."""
+    """vecsz::interface::Compress<float,int> (src/vecsz_interface.hh:31-229).
+    Returns (archive, num_outlier, data_size, lossless_size): with
+    ap.lossless set, the archive is the lossless stream of the data_size-byte
+    payload (:206-217), else lossless_size is 0."""
     dims = [ap.dim4._0, ap.dim4._1, ap.dim4._2][: ap.ndim]
     if torch is not None and isinstance(data_in, torch.Tensor):
         x = data_in.reshape(tuple(reversed(dims)))
@@ -395,6 +499,10 @@ def Compress(ap: ArgParse, data_in):
         x = np.asarray(data_in, dtype=np.float32).reshape(tuple(reversed(dims)))
     arch, nout = _ctx(ap).compress(x, ap.eb, ap.mode, ap.block_size, ap.dict_size, ap.value_range)
     size = arch.numel() if hasattr(arch, "numel") else len(arch)
+    if ap.lossless:
+        payload = arch.cpu().numpy().tobytes() if hasattr(arch, "numel") else arch
+        packed = lossless_compress(ap.lossless, payload)
+        return packed, nout, size, len(packed)
     return arch, nout, size, 0
 
 
@@ -403,7 +511,9 @@ def Decompress(ap: ArgParse, tmp_data, target_size: int = 0, lossless_size: int 
     Sets ap.ndim / ap.dim4 / ap.eb from the archive like the reference does
     (:262-268) and returns the reconstructed field."""
     if lossless_size:
-        raise NotImplementedError("the zstd/gzip pass is not part of this library")
+        # lossless pass first (:237-244): the stream inflates to the target_size payload
+        raw = tmp_data.cpu().numpy().tobytes() if hasattr(tmp_data, "numel") else bytes(tmp_data)
+        tmp_data = lossless_decompress(raw[:lossless_size], target_size)
     y = _ctx(ap).decompress(tmp_data, ap.mode, ap.value_range, ap.block_size, ap.dict_size)
     dims = tuple(reversed(tuple(y.shape)))
     ap.ndim = len(dims)
