@@ -59,55 +59,17 @@ struct StreamView {
     }
 };
 
-// Staged words are stored with one pad word after every 32, so the threads of
-// a wavefront, each at its own 1024-bit subsequence (32 words apart), read
-// different LDS banks.
+// Staged words are stored in rows of 33: the 32 words of one 1024-bit
+// subsequence followed by a copy of the next row's first word.  Threads of a
+// wavefront, each at its own subsequence, then read different LDS banks, and
+// any two consecutive words sit side by side (one ds_read2).
 __device__ __forceinline__ uint32_t pad_idx(uint32_t w) { return w + (w >> 5); }
 
-// 64-bit window whose MSB is bit `lp` of the staged words (>= 33 valid bits).
-struct LdsWin {
-    const uint32_t* st;
-    __device__ __forceinline__ uint64_t operator()(uint32_t lp) const {
-        const uint32_t w = lp >> 5;
-        const uint32_t i0 = pad_idx(w);
-        const uint32_t i1 = i0 + 1 + ((w & 31) == 31);
-        return ((((uint64_t)st[i0]) << 32) | st[i1]) << (lp & 31);
-    }
-};
-
-// Register bit buffer over the staged words: `buf` holds the next nb >= 33
-// bits MSB first and the following word is already loaded in `nxt`, so the
-// decode chain waits on the table lookup only.
-struct LdsBits {
-    const uint32_t* st;
-    uint64_t buf;
-    uint32_t nb, wi, nxt, pos;  // pos: local bit index of buf's MSB
-
-    __device__ __forceinline__ void init(const uint32_t* s, uint32_t lp) {
-        st = s;
-        pos = lp;
-        const uint32_t w = lp >> 5;
-        buf = ((((uint64_t)st[pad_idx(w)]) << 32) | st[pad_idx(w + 1)]) << (lp & 31);
-        nb = 64 - (lp & 31);
-        wi = w + 2;
-        nxt = st[pad_idx(wi)];
-    }
-    __device__ __forceinline__ void consume(uint32_t l) {
-        if (l >= nb) {  // a codeword longer than the buffer (> 32 bits): restart behind it
-            init(st, pos + l);
-            return;
-        }
-        pos += l;
-        buf <<= l;
-        nb -= l;
-        if (nb <= 32) {
-            buf |= (uint64_t)nxt << (32 - nb);
-            nb += 32;
-            wi++;
-            nxt = st[pad_idx(wi)];
-        }
-    }
-};
+// 32-bit window whose MSB is bit `lp` of the staged words.
+__device__ __forceinline__ uint32_t win32(const uint32_t* st, uint32_t lp) {
+    const uint32_t i0 = pad_idx(lp >> 5);
+    return __builtin_elementwise_fshl(st[i0], st[i0 + 1], lp & 31);
+}
 
 // The same from global memory, at stream bit p.
 struct GlobalWin {
@@ -179,16 +141,17 @@ struct Decoder<true> {
     __device__ __forceinline__ uint32_t operator()(const Win& w, P p, uint32_t& len) const {
         return decode_sym16(w, p, lut, sub, len);
     }
-    // Decodes and consumes the codeword at the head of bb; len = 0: invalid path.
-    __device__ __forceinline__ uint32_t take(LdsBits& bb, uint32_t& len) const {
-        uint32_t e = lut[(uint32_t)(bb.buf >> (64 - kLutBits))];
+    // Decodes the codeword at staged bit lp; len = 0: invalid path.  The
+    // window covers codewords of up to 32 bits; longer ones re-read.
+    __device__ __forceinline__ uint32_t take_at(const uint32_t* st, uint32_t lp, uint32_t& len) const {
+        const uint32_t w = win32(st, lp);
+        uint32_t e = lut[w >> (32 - kLutBits)];
         uint32_t l = e >> 12;
         if (l == 0) {
             uint32_t used = kLutBits;
             for (int lv = 0; lv < 16; lv++) {
-                const uint32_t idx = used + kSubTabBits <= bb.nb
-                                         ? (uint32_t)((bb.buf << used) >> (64 - kSubTabBits))
-                                         : (uint32_t)(LdsWin{bb.st}(bb.pos + used) >> (64 - kSubTabBits));
+                const uint32_t idx = used + kSubTabBits <= 32 ? (w << used) >> (32 - kSubTabBits)
+                                                              : win32(st, lp + used) >> (32 - kSubTabBits);
                 e = sub[((e & 0xFFF) << kSubTabBits) + idx];
                 if (e >> 12) {
                     l = used + (e >> 12);
@@ -198,7 +161,6 @@ struct Decoder<true> {
             }
         }
         len = l;
-        bb.consume(l ? l : 1);
         return e & 0xFFF;
     }
 };
@@ -218,20 +180,19 @@ struct Decoder<false> {
     __device__ __forceinline__ uint32_t operator()(const Win& w, P p, uint32_t& len) const {
         return decode_sym(w, p, lut, sub, len);
     }
-    __device__ __forceinline__ uint32_t take(LdsBits& bb, uint32_t& len) const {
-        uint32_t e = lut[(uint32_t)(bb.buf >> (64 - kLutBits))];
+    __device__ __forceinline__ uint32_t take_at(const uint32_t* st, uint32_t lp, uint32_t& len) const {
+        const uint32_t w = win32(st, lp);
+        uint32_t e = lut[w >> (32 - kLutBits)];
         if (e & kLutSlow) {
             uint32_t used = kLutBits;
             for (int lv = 0; lv < 16 && (e & kLutSlow); lv++) {
-                const uint32_t idx = used + kSubTabBits <= bb.nb
-                                         ? (uint32_t)((bb.buf << used) >> (64 - kSubTabBits))
-                                         : (uint32_t)(LdsWin{bb.st}(bb.pos + used) >> (64 - kSubTabBits));
+                const uint32_t idx = used + kSubTabBits <= 32 ? (w << used) >> (32 - kSubTabBits)
+                                                              : win32(st, lp + used) >> (32 - kSubTabBits);
                 e = sub[(e & 0x7FFFFFFFu) + idx];
                 used += kSubTabBits;
             }
         }
         len = (e & kLutSlow) ? 0u : (e >> 16) & 0x7F;
-        bb.consume(len ? len : 1);
         return e & 0xFFFF;
     }
 };
@@ -277,6 +238,7 @@ __device__ __forceinline__ void stage_words(uint32_t* st, const StreamView& sv, 
             d[1] = v[m].y;
             d[2] = v[m].z;
             d[3] = v[m].w;
+            if ((k & 31) == 0 && k > 0) d[-1] = v[m].x;  // the previous row's copy of its successor
         }
     }
 }
@@ -316,22 +278,20 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         const uint32_t e0 = local(min((j + 1) * (uint64_t)kSubBits, a.stream_bits));
 #pragma unroll
         for (int k = 0; k < kCkpts; k++) ck[t * kCkStride + k] = kNoCkpt;
-        LdsBits bb;
-        bb.init(st, s0);
-        uint32_t cnt = 0, z = 0, last_k = 0xFFFFFFFFu;
-        while (bb.pos < e0) {
-            const uint32_t rel = bb.pos - s0;
+        uint32_t lp = s0, cnt = 0, z = 0, last_k = 0xFFFFFFFFu;
+        while (lp < e0) {
+            const uint32_t rel = lp - s0;
             if (rel / kCkptBits != last_k) {
                 last_k = rel / kCkptBits;
                 ck[t * kCkStride + last_k] = ckpt(rel % kCkptBits, cnt, z);
             }
             uint32_t len;
-            const uint32_t sym = dec.take(bb, len);
+            const uint32_t sym = dec.take_at(st, lp, len);
             bad |= len == 0;
+            lp += len ? len : 1;
             cnt++;
             z += sym == 0;
         }
-        const uint32_t lp = bb.pos;
         own_cnt[t] = cnt;
         own_z[t] = z;
         own_e[t] = lp;
@@ -352,10 +312,8 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         const uint32_t q0 = own_e[t];
         uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0;
         bool met = false;
-        LdsBits bb;
-        bb.init(st, q0);
+        uint32_t q = q0;
         for (;;) {
-            const uint32_t q = bb.pos;
             if (q >= eu) {
                 if (q == own_e[u]) {
                     met = true;
@@ -373,12 +331,12 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
                 break;
             }
             uint32_t len;
-            const uint32_t sym = dec.take(bb, len);
+            const uint32_t sym = dec.take_at(st, q, len);
             bad |= len == 0;
+            q += len ? len : 1;
             cc++;
             cz += sym == 0;
         }
-        const uint32_t q = bb.pos;
         const uint64_t base = J0 * kSubBits - sv.shift0;
         s.b[j + 1] = base + q0;
         if (met) {
@@ -551,16 +509,16 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
     __syncthreads();
     const uint64_t j = J0 + threadIdx.x;
     if (j >= a.nsub) return;
-    LdsBits bb;
-    bb.init(st, (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0);
+    uint32_t lp = (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0;
     const uint64_t O = sym_off[j];
     const uint64_t iend = min(O + s.cnt[j], a.n);
     uint64_t rank = zero_off[j];
     bool bad = false;
     auto next = [&]() -> uint32_t {
         uint32_t len;
-        const uint32_t sym = dec.take(bb, len);
+        const uint32_t sym = dec.take_at(st, lp, len);
         bad |= len == 0;
+        lp += len ? len : 1;
         return sym;
     };
     auto outlier = [&](uint64_t i) {
