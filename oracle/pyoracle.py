@@ -43,6 +43,16 @@ def _load_oracle():
     lib.orc_reconstruct.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                     _c_vp, _c_vp, _c_vp]
     lib.orc_eb_family.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _c_vp]
+    lib.orc_compress_pad.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_vp,
+                                     ctypes.POINTER(_c_vp), _c_size_p, _c_size_p, ctypes.POINTER(ctypes.c_double)]
+    lib.orc_pad_values.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, _c_vp, _c_vp, _c_vp, _c_size_p, _c_vp]
+    lib.orc_pad_halo_from_archive.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, _c_vp, ctypes.c_uint64, _c_vp]
+    lib.orc_dualquant_halo.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, ctypes.c_int, _c_vp, _c_vp, _c_vp,
+                                       _c_vp, _c_vp]
+    lib.orc_nblk.argtypes = [ctypes.c_size_t, ctypes.c_int]
+    lib.orc_nblk.restype = ctypes.c_size_t
     lib.orc_free.argtypes = [_c_vp]
     return lib
 
@@ -75,6 +85,9 @@ def ref():
                                        ctypes.c_size_t, ctypes.POINTER(_c_vp), _c_size_p, ctypes.c_int]
         lib.ref_decompress_pad0.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, _c_vp,
                                             ctypes.POINTER(_c_vp), _c_size_p, ctypes.c_int]
+        lib.ref_compress_pad.argtypes = [ctypes.c_int, _c_size_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_char_p,
+                                         _c_vp, ctypes.POINTER(_c_vp), _c_size_p, _c_size_p, ctypes.c_int]
         lib.ref_free.argtypes = [_c_vp]
         lib.ref_free_f32.argtypes = [_c_vp]
         lib.ref_set_threads.argtypes = [ctypes.c_int]
@@ -120,7 +133,12 @@ def eb_family(r2r: bool, eb: float, rng: float = 0.0) -> EbFamily:
     return f
 
 
-def compress(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4096, block: int = 8):
+PAD_MODES = {"none": 0, "global": 1, "block": 2, "edge": 3}
+PAD_TYPES = {"max": 0, "min": 1, "avg": 2, "const": 3}
+
+
+def compress(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4096, block: int = 8,
+             pad_mode: str = "none", pad_type: str = "const", pad_constant: float = 0.0):
     """Oracle compress. Returns (archive bytes, n_outlier, eb_final)."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     dims = dims_xfirst(x)
@@ -128,8 +146,9 @@ def compress(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4096,
     size = ctypes.c_size_t()
     nout = ctypes.c_size_t()
     ebf = ctypes.c_double()
-    rc = orc().orc_compress(len(dims), _dims_arr(dims), int(r2r), eb, dict_size, block, x.ctypes.data,
-                            ctypes.byref(out), ctypes.byref(size), ctypes.byref(nout), ctypes.byref(ebf))
+    rc = orc().orc_compress_pad(len(dims), _dims_arr(dims), int(r2r), eb, dict_size, block, PAD_MODES[pad_mode],
+                                PAD_TYPES[pad_type], pad_constant, x.ctypes.data, ctypes.byref(out),
+                                ctypes.byref(size), ctypes.byref(nout), ctypes.byref(ebf))
     if rc:
         raise OracleError(f"orc_compress rc={rc}")
     data = ctypes.string_at(out.value, size.value)
@@ -162,6 +181,42 @@ def dualquant(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4096
     if rc:
         raise OracleError(f"orc_dualquant rc={rc}")
     return code, outl
+
+
+def nblk(d: int, block: int) -> int:
+    """Blocks along a dimension as the reference counts them: (d + B + 1) // B."""
+    return int(orc().orc_nblk(d, block))
+
+
+def pad_values(x: np.ndarray, eb: float, r2r: bool = False, block: int = 8, pad_mode: str = "block",
+               pad_type: str = "max", pad_constant: float = 0.0):
+    """Oracle pad values: (archive pad_vals float32[pad_len], halo float32[nblocks, 3] (z, y, x faces))."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    dims = dims_xfirst(x)
+    f = eb_family(r2r, eb, value_range(x) if r2r else 0.0)
+    plen = ctypes.c_size_t()
+    args = (len(dims), _dims_arr(dims), block, PAD_MODES[pad_mode], PAD_TYPES[pad_type], pad_constant,
+            ctypes.byref(f), x.ctypes.data)
+    rc = orc().orc_pad_values(*args, None, ctypes.byref(plen), None)
+    if rc:
+        raise OracleError(f"orc_pad_values rc={rc}")
+    nb = int(np.prod([nblk(d, block) for d in dims]))
+    pv = np.zeros(max(plen.value, 1), dtype=np.float32)
+    halo = np.zeros((nb, 3), dtype=np.float32)
+    rc = orc().orc_pad_values(*args, pv.ctypes.data, ctypes.byref(plen), halo.ctypes.data)
+    if rc:
+        raise OracleError(f"orc_pad_values rc={rc}")
+    return pv[:plen.value], halo
+
+
+def halo_from_archive(dims_xfirst_: tuple, block: int, pad: np.ndarray) -> np.ndarray:
+    """Halo values decompress derives from an archive's pad section."""
+    pad = np.ascontiguousarray(pad, dtype=np.float32)
+    nb = int(np.prod([nblk(d, block) for d in dims_xfirst_]))
+    halo = np.zeros((nb, 3), dtype=np.float32)
+    orc().orc_pad_halo_from_archive(len(dims_xfirst_), _dims_arr(dims_xfirst_), block,
+                                    pad.ctypes.data if pad.size else None, pad.size, halo.ctypes.data)
+    return halo
 
 
 def codebook(freq: np.ndarray, state_num: int):
@@ -212,7 +267,7 @@ def archive_sections(archive: bytes):
 # ------------------------------------------------------------------ reference
 
 def ref_compress(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4096, block: int = 8,
-                 threads: int = 0):
+                 threads: int = 0, pad_mode: str = "none", pad_type: str = "const", pad_constant: float = 0.0):
     """Run the reference's own Compress<float,int> in-process (test only)."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     dims = dims_xfirst(x)
@@ -230,8 +285,13 @@ def ref_compress(x: np.ndarray, eb: float, r2r: bool = False, dict_size: int = 4
         out = _c_vp()
         size = ctypes.c_size_t()
         nout = ctypes.c_size_t()
-        lib.ref_compress(len(dims), _dims_arr(dims), int(r2r), eb, dict_size, block, path, x.ctypes.data,
-                         ctypes.byref(out), ctypes.byref(size), ctypes.byref(nout), 1)
+        if pad_mode == "none":
+            lib.ref_compress(len(dims), _dims_arr(dims), int(r2r), eb, dict_size, block, path, x.ctypes.data,
+                             ctypes.byref(out), ctypes.byref(size), ctypes.byref(nout), 1)
+        else:
+            lib.ref_compress_pad(len(dims), _dims_arr(dims), int(r2r), eb, dict_size, block, PAD_MODES[pad_mode],
+                                 PAD_TYPES[pad_type], pad_constant, path, x.ctypes.data, ctypes.byref(out),
+                                 ctypes.byref(size), ctypes.byref(nout), 1)
         data = ctypes.string_at(out.value, size.value)
         lib.ref_free(out)
     finally:

@@ -106,6 +106,39 @@ int ref_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
     return 0;
 }
 
+// Compress with padding: pad_mode 1 global, 2 block, 3 edge (the flags
+// --padding sets, src/argument_parser/argparse.cc:399-415), pad_type 0 max,
+// 1 min, 2 avg, 3 constant.  The reference appends pad values through a
+// counter shared by its OpenMP threads (src/dualquant.hh:41, :232, :243), so
+// callers run it with one thread (ref_set_threads(1)) for the serial order.
+int ref_compress_pad(int ndim, const size_t* dims, int r2r, double eb, int dict, int block, int pad_mode,
+                     int pad_type, float pad_constant, const char* path, const float* data, uint8_t** out,
+                     size_t* out_size, size_t* n_outlier, int quiet) {
+    ArgParse* ap = new ArgParse();
+    fill_args(ap, ndim, dims, r2r, eb, dict, block, path);
+    ap->szwf.global_padding = pad_mode == 1;
+    ap->szwf.block_padding = pad_mode == 2;
+    ap->szwf.edge_padding = pad_mode == 3;
+    ap->pad_type = pad_type;
+    ap->pad_constant = pad_constant;
+    size_t len = 1;
+    for (int i = 0; i < ndim; i++) len *= dims[i];
+    float* copy = new float[len + 64]();
+    memcpy(copy, data, len * sizeof(float));
+    size_t data_size = 0, nnz = 0;
+    unsigned long lossless_size = 0;
+    void* res;
+    {
+        Mute m(quiet != 0);
+        res = vecsz::interface::Compress<float, int>(ap, copy, &nnz, &data_size, &lossless_size);
+    }
+    *out = (uint8_t*)res;
+    *out_size = data_size;
+    *n_outlier = nnz;
+    delete ap;
+    return 0;
+}
+
 // Runs vecsz::interface::Decompress<float,int> (src/vecsz_interface.hh:232).
 int ref_decompress(int r2r, int dict, int block, const char* path, const uint8_t* archive,
                    size_t archive_size, float** out, size_t* out_len, int quiet) {

@@ -80,8 +80,37 @@ static inline float prequant_scalar(float x, double r_d) { return (float)round((
  *  Neighbours in another Lorenzo block read the zero halo (:224, :246, :551,
  *  :573).  code = quantizable ? (int)(e + R) : 0; outlier = q where code == 0.
  */
+/* Value of a neighbour at offsets (dz, dy, dx) in {0, 1} of an element at
+ * in-block position (iz, iy, ix), given the element's own block halo
+ * h = {z face, y face, x face}.  A neighbour inside the block reads v.
+ * Halo cells take the face that the reference's padded block gives them
+ * first: 3D (k == 0) ? z : (j == 0) ? y : x (src/utils/padding.hh:245, :290,
+ * :330, :365); 2D (i == 0) ? x : y (:226, :275, :316, :353). */
+static inline float halo_nb(int ndim, const float* h, int dz, int dy, int dx, size_t iz, size_t iy, size_t ix,
+                            const float* v, size_t off) {
+    static const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    if (!h) h = zero3;
+    int oz = dz && iz == 0, oy = dy && iy == 0, ox = dx && ix == 0;
+    if (ndim == 3) {
+        if (oz) return h[0];
+        if (oy) return h[1];
+        if (ox) return h[2];
+    } else if (ndim == 2) {
+        if (ox) return h[2];
+        if (oy) return h[1];
+    } else if (ox) {
+        return h[2];
+    }
+    return v[off];
+}
+
 int orc_dualquant(int ndim, const size_t* dims, int block, int radius, const orc_ebs* ebs,
                   const float* x, int32_t* code, float* outlier) {
+    return orc_dualquant_halo(ndim, dims, block, radius, ebs, x, NULL, code, outlier);
+}
+
+int orc_dualquant_halo(int ndim, const size_t* dims, int block, int radius, const orc_ebs* ebs,
+                       const float* x, const float* halo, int32_t* code, float* outlier) {
     if (ndim < 1 || ndim > 3 || block <= 0 || (block % 8) != 0) return ORC_EINVAL;
     size_t d0 = dims[0], d1 = ndim >= 2 ? dims[1] : 1, d2 = ndim >= 3 ? dims[2] : 1;
     size_t n = d0 * d1 * d2;
@@ -90,6 +119,7 @@ int orc_dualquant(int ndim, const size_t* dims, int block, int radius, const orc
     const float R = (float)radius;
     const float rf = ebs->r_f;
     const double rd = ebs->r_d;
+    const size_t nb0 = orc_nblk(d0, block), nb1 = ndim >= 2 ? orc_nblk(d1, block) : 1;
     float* q = (float*)malloc(n * sizeof(float) + 4);
     if (!q) return ORC_ENOMEM;
 
@@ -103,33 +133,35 @@ int orc_dualquant(int ndim, const size_t* dims, int block, int radius, const orc
     memset(code, 0, n * sizeof(int32_t));
     memset(outlier, 0, n * sizeof(float));
 
+    const size_t sy = d0, sz = d0 * d1;
     for (size_t z = 0; z < d2; z++)
         for (size_t y = 0; y < d1; y++)
             for (size_t i = 0; i < d0; i++) {
                 size_t id = i + d0 * (y + d1 * z);
-                int hx = (i % B) != 0, hy = (y % B) != 0, hz = (z % B) != 0;
+                size_t ix = i % B, iy = y % B, iz = z % B;
+                const float* h = halo ? halo + 3 * ((i / B) + nb0 * ((y / B) + nb1 * (z / B))) : NULL;
                 float cur = q[id], pred;
                 int vec_test;
+#define NB(dz, dy, dx) halo_nb(ndim, h, dz, dy, dx, iz, iy, ix, q, id - (dx) - (dy) * sy - (dz) * sz)
                 if (ndim == 1) {
-                    pred = hx ? q[id - 1] : 0.0f;
-                    size_t bstart = i - (i % B);
+                    pred = NB(0, 0, 1);
+                    size_t bstart = i - ix;
                     vec_test = (bstart + B) < d0;
                 } else if (ndim == 2) {
                     if (i >= x8) continue; /* D2: lost partial x-group */
-                    float W = hx ? q[id - 1] : 0.0f;
-                    float N = hy ? q[id - d0] : 0.0f;
-                    float NW = (hx && hy) ? q[id - 1 - d0] : 0.0f;
+                    float W = NB(0, 0, 1);
+                    float N = NB(0, 1, 0);
+                    float NW = NB(0, 1, 1);
                     pred = W + (N - NW);
                     vec_test = 1;
                 } else {
-                    size_t sy = d0, sz = d0 * d1;
-                    float X = hx ? q[id - 1] : 0.0f;
-                    float Y = hy ? q[id - sy] : 0.0f;
-                    float Z = hz ? q[id - sz] : 0.0f;
-                    float XY = (hx && hy) ? q[id - 1 - sy] : 0.0f;
-                    float XZ = (hx && hz) ? q[id - 1 - sz] : 0.0f;
-                    float YZ = (hy && hz) ? q[id - sy - sz] : 0.0f;
-                    float XYZ = (hx && hy && hz) ? q[id - 1 - sy - sz] : 0.0f;
+                    float X = NB(0, 0, 1);
+                    float Y = NB(0, 1, 0);
+                    float Z = NB(1, 0, 0);
+                    float XY = NB(0, 1, 1);
+                    float XZ = NB(1, 0, 1);
+                    float YZ = NB(1, 1, 0);
+                    float XYZ = NB(1, 1, 1);
                     if (i < x8) {
                         float dist1 = X + (Y + Z);
                         float dist2 = XY + (XZ + YZ);
@@ -140,16 +172,205 @@ int orc_dualquant(int ndim, const size_t* dims, int block, int radius, const orc
                         vec_test = 0;
                     }
                 }
+#undef NB
                 float e = cur - pred;
                 int quant = vec_test ? (sqrtf(e * e) < R) : (fabsf(e) < R);
                 if (quant) {
-                    code[id] = (int32_t)(e + R);
+                    /* the AVX forms convert with _mm256_cvtps_epi32 (round to
+                     * nearest even, src/dualquant.hh:172, :362, :506, :708, :885),
+                     * the scalar forms with static_cast (truncation, :200, :378,
+                     * :524, :726, :905); they differ only for a non-integral e, which
+                     * arises in 1D block/edge padding (raw pad value) */
+                    code[id] = vec_test ? (int32_t)rintf(e + R) : (int32_t)(e + R);
                 } else {
                     code[id] = 0;
                     outlier[id] = cur;
                 }
             }
     free(q);
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------- padding */
+
+size_t orc_nblk(size_t d, int block) { return (d + (size_t)block + 1) / (size_t)block; }
+
+/* find_pad_value (src/utils/padding.hh:121-143) over a linear array in
+ * index order: find_max / find_min keep the first value and replace it by a
+ * strictly larger / smaller one (:65-99); find_avg sums in float and divides
+ * by the float element count (:101-119). */
+static float pad_find(const float* a, size_t count, int type, float constant) {
+    if (type == ORC_PADV_CONST) return constant;
+    if (type == ORC_PADV_AVG) {
+        float sum = 0.0f;
+        for (size_t i = 0; i < count; i++) sum += a[i];
+        return sum / (float)count;
+    }
+    float m = a[0];
+    for (size_t i = 0; i < count; i++) {
+        if (type == ORC_PADV_MAX) m = (a[i] > m) ? a[i] : m;
+        else m = (a[i] < m) ? a[i] : m;
+    }
+    return m;
+}
+
+/* block_pad / edge_pad prequantization of a pad value (src/utils/padding.hh:
+ * 219-220, :236-237, :259-263): the float is scaled in double, stored back to
+ * float, then rounded half away from zero. */
+static float pad_prequant_block(float p, double r_d) {
+    float t = (float)((double)p * r_d);
+    return (float)round((double)t);
+}
+
+/* Global pad value as Compress stores it: round(pad_constant * ebx2_r) in
+ * double, stored to float (src/vecsz_interface.hh:79). */
+static float pad_prequant_global(float c, double r_d) { return (float)round((double)c * r_d); }
+
+/* fill_2d_block / fill_3d_block (src/utils/padding.hh:17-63): the block's
+ * elements in x-fastest order; elements outside the field read +0.0. */
+static void pad_fill(int ndim, const float* x, size_t d0, size_t d1, size_t d2, size_t B, size_t x0, size_t y0,
+                     size_t z0, float* blk) {
+    size_t nz = ndim >= 3 ? B : 1, ny = ndim >= 2 ? B : 1;
+    for (size_t k = 0; k < nz; k++)
+        for (size_t j = 0; j < ny; j++)
+            for (size_t i = 0; i < B; i++) {
+                size_t gx = x0 + i, gy = y0 + j, gz = z0 + k;
+                int in = gx < d0 && gy < d1 && gz < d2;
+                blk[i + B * (j + ny * k)] = in ? x[gx + d0 * (gy + d1 * gz)] : 0.0f;
+            }
+}
+
+int orc_pad_values(int ndim, const size_t* dims, int block, int mode, int type, float constant,
+                   const orc_ebs* ebs, const float* x, float* pad_vals, size_t* pad_len, float* halo) {
+    if (ndim < 1 || ndim > 3 || block <= 0 || mode < ORC_PAD_NONE || mode > ORC_PAD_EDGE || type < 0 ||
+        type > ORC_PADV_CONST)
+        return ORC_EINVAL;
+    size_t d0 = dims[0], d1 = ndim >= 2 ? dims[1] : 1, d2 = ndim >= 3 ? dims[2] : 1;
+    size_t B = (size_t)block;
+    size_t nb0 = orc_nblk(d0, block), nb1 = ndim >= 2 ? orc_nblk(d1, block) : 1, nb2 = ndim >= 3 ? orc_nblk(d2, block) : 1;
+    size_t nblocks = nb0 * nb1 * nb2;
+    size_t len = mode == ORC_PAD_NONE ? 0 : mode == ORC_PAD_GLOBAL ? 1 : mode == ORC_PAD_BLOCK ? nblocks : (size_t)ndim * nblocks;
+    *pad_len = len;
+    if (!pad_vals && !halo) return ORC_OK;
+    const double rd = ebs->r_d;
+    if (halo) memset(halo, 0, 3 * nblocks * sizeof(float));
+    if (mode == ORC_PAD_NONE) return ORC_OK;
+    if (mode == ORC_PAD_GLOBAL) {
+        /* the whole field (src/vecsz_interface.hh:73-80), then every block pads
+         * with that constant (block_pad with CONST_VAL, :77) */
+        float c = pad_find(x, d0 * d1 * d2, type, constant);
+        float stored = pad_prequant_global(c, rd);
+        float used = ndim == 1 ? stored : pad_prequant_block(c, rd); /* 1D reads pad_vals[0] (src/dualquant.hh:43-46) */
+        if (pad_vals) pad_vals[0] = stored;
+        if (halo)
+            for (size_t b = 0; b < nblocks; b++) halo[3 * b] = halo[3 * b + 1] = halo[3 * b + 2] = used;
+        return ORC_OK;
+    }
+    size_t bsz = B * (ndim >= 2 ? B : 1) * (ndim >= 3 ? B : 1);
+    float* blk = (float*)malloc(bsz * sizeof(float));
+    float* edge = (float*)malloc(B * B * sizeof(float));
+    if (!blk || !edge) { free(blk); free(edge); return ORC_ENOMEM; }
+    size_t pi = 0;
+    for (size_t b2 = 0; b2 < nb2; b2++)
+        for (size_t b1 = 0; b1 < nb1; b1++)
+            for (size_t b0 = 0; b0 < nb0; b0++) {
+                size_t id = b0 + nb0 * (b1 + nb1 * b2);
+                float hz, hy, hx;
+                pad_fill(ndim, x, d0, d1, d2, B, b0 * B, b1 * B, b2 * B, blk);
+                if (ndim == 1) {
+                    /* 1D block and edge padding: the raw pad value of the block's
+                     * data, not prequantized (src/dualquant.hh:37-42) */
+                    float p = pad_find(blk, B, type, constant);
+                    if (pad_vals) pad_vals[pi] = p;
+                    pi++;
+                    hz = hy = hx = p;
+                } else if (mode == ORC_PAD_BLOCK) {
+                    float p = pad_prequant_block(pad_find(blk, bsz, type, constant), rd);
+                    if (pad_vals) pad_vals[pi] = p;
+                    pi++;
+                    hz = hy = hx = p;
+                } else if (ndim == 2) {
+                    /* find_edge_pad_value (src/utils/padding.hh:145-206): pv0 from
+                     * the block's first row, pv1 from its first column */
+                    float pv[2];
+                    for (size_t i = 0; i < B; i++) edge[i] = blk[i];
+                    pv[0] = pad_find(edge, B, type, constant);
+                    for (size_t i = 0; i < B; i++) edge[i] = blk[i * B];
+                    pv[1] = pad_find(edge, B, type, constant);
+                    for (int f = 0; f < 2; f++) {
+                        pv[f] = pad_prequant_block(pv[f], rd);
+                        if (pad_vals) pad_vals[pi] = pv[f];
+                        pi++;
+                    }
+                    hz = 0.0f;
+                    hx = pv[0];
+                    hy = pv[1];
+                } else {
+                    /* 3D: pv0 from the z = 0 plane, pv1 from the x = 0 plane,
+                     * pv2 from the y = 0 plane */
+                    float pv[3];
+                    for (size_t j = 0; j < B; j++)
+                        for (size_t i = 0; i < B; i++) edge[i + j * B] = blk[i + j * B];
+                    pv[0] = pad_find(edge, B * B, type, constant);
+                    for (size_t j = 0; j < B; j++)
+                        for (size_t i = 0; i < B; i++) edge[i + j * B] = blk[i * B + j * B * B];
+                    pv[1] = pad_find(edge, B * B, type, constant);
+                    for (size_t j = 0; j < B; j++)
+                        for (size_t i = 0; i < B; i++) edge[i + j * B] = blk[i + j * B * B];
+                    pv[2] = pad_find(edge, B * B, type, constant);
+                    for (int f = 0; f < 3; f++) {
+                        pv[f] = pad_prequant_block(pv[f], rd);
+                        if (pad_vals) pad_vals[pi] = pv[f];
+                        pi++;
+                    }
+                    hz = pv[0];
+                    hy = pv[1];
+                    hx = pv[2];
+                }
+                if (halo) {
+                    halo[3 * id] = hz;
+                    halo[3 * id + 1] = hy;
+                    halo[3 * id + 2] = hx;
+                }
+            }
+    free(blk);
+    free(edge);
+    return ORC_OK;
+}
+
+int orc_pad_halo_from_archive(int ndim, const size_t* dims, int block, const float* pad_vals, uint64_t pad_len,
+                              float* halo) {
+    size_t d0 = dims[0], d1 = ndim >= 2 ? dims[1] : 1, d2 = ndim >= 3 ? dims[2] : 1;
+    size_t nb0 = orc_nblk(d0, block), nb1 = ndim >= 2 ? orc_nblk(d1, block) : 1, nb2 = ndim >= 3 ? orc_nblk(d2, block) : 1;
+    size_t nblocks = nb0 * nb1 * nb2;
+    int mode = pad_len == 1 ? ORC_PAD_GLOBAL
+             : pad_len == nblocks ? ORC_PAD_BLOCK
+             : pad_len == (uint64_t)ndim * nblocks ? ORC_PAD_EDGE : ORC_PAD_NONE;
+    for (size_t b = 0; b < nblocks; b++) {
+        float hz = 0.0f, hy = 0.0f, hx = 0.0f;
+        if (mode == ORC_PAD_GLOBAL) {
+            hz = hy = hx = pad_vals[0];
+        } else if (mode == ORC_PAD_BLOCK) {
+            hz = hy = hx = pad_vals[b];
+        } else if (mode == ORC_PAD_EDGE) {
+            if (ndim == 2) {
+                hx = pad_vals[2 * b];
+                hy = pad_vals[2 * b + 1];
+            } else {
+                hz = pad_vals[3 * b];
+                hy = pad_vals[3 * b + 1];
+                hx = pad_vals[3 * b + 2];
+            }
+        } else if (ndim == 1) {
+            /* x_lorenzo_1d1l reads pad_vals[0] without a padding mode
+             * (src/dualquant.hh:925-927); with no pad section that is D3's
+             * read of new float[0], taken as +0.0 */
+            hz = hy = hx = pad_len > 0 ? pad_vals[0] : 0.0f;
+        }
+        halo[3 * b] = hz;
+        halo[3 * b + 1] = hy;
+        halo[3 * b + 2] = hx;
+    }
     return ORC_OK;
 }
 
@@ -363,6 +584,13 @@ static void put_u32be(uint8_t* b, uint32_t v) {
 int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int block,
                  const float* x, uint8_t** archive, size_t* archive_size, size_t* n_outlier,
                  double* eb_final_out) {
+    return orc_compress_pad(ndim, dims, r2r, eb, dict, block, ORC_PAD_NONE, ORC_PADV_CONST, 0.0f, x, archive,
+                            archive_size, n_outlier, eb_final_out);
+}
+
+int orc_compress_pad(int ndim, const size_t* dims, int r2r, double eb, int dict, int block, int pad_mode,
+                     int pad_type, float pad_constant, const float* x, uint8_t** archive, size_t* archive_size,
+                     size_t* n_outlier, double* eb_final_out) {
     if (ndim < 1 || ndim > 3 || dict < 2 || (dict % 2) != 0 || dict > 65536) return ORC_EINVAL;
     size_t n = 1;
     size_t dd[3] = {1, 1, 1};
@@ -378,7 +606,17 @@ int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
     int32_t* code = (int32_t*)malloc(n * sizeof(int32_t));
     float* outl = (float*)malloc(n * sizeof(float));
     if (!code || !outl) return ORC_ENOMEM;
-    rc = orc_dualquant(ndim, dd, block, radius, &ebs, x, code, outl);
+    size_t pad_len = 0;
+    rc = orc_pad_values(ndim, dd, block, pad_mode, pad_type, pad_constant, &ebs, x, NULL, &pad_len, NULL);
+    if (rc) return rc;
+    size_t nblocks = orc_nblk(dd[0], block) * (ndim >= 2 ? orc_nblk(dd[1], block) : 1) *
+                     (ndim >= 3 ? orc_nblk(dd[2], block) : 1);
+    float* pvals = (float*)malloc(4 * pad_len + 4);
+    float* halo = (float*)malloc(3 * nblocks * sizeof(float) + 4);
+    if (!pvals || !halo) return ORC_ENOMEM;
+    rc = orc_pad_values(ndim, dd, block, pad_mode, pad_type, pad_constant, &ebs, x, pvals, &pad_len, halo);
+    if (rc) return rc;
+    rc = orc_dualquant_halo(ndim, dd, block, radius, &ebs, x, pad_mode == ORC_PAD_NONE ? NULL : halo, code, outl);
     if (rc) return rc;
 
     int state_num = 2 * dict; /* createHuffmanTree(dict_size * 2) (src/vecsz_interface.hh:160) */
@@ -397,7 +635,7 @@ int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
     size_t nout = 0;
     for (size_t i = 0; i < n; i++) nout += code[i] == 0;
     size_t meta = sizeof(int32_t) * (size_t)(ndim + 1) + sizeof(double) + 3 * sizeof(uint64_t);
-    size_t total = meta + huff + 4 * nout;
+    size_t total = meta + 4 * pad_len + huff + 4 * nout;
     uint8_t* a = (uint8_t*)malloc(total + 8);
     if (!a) return ORC_ENOMEM;
     uint8_t* p = a;
@@ -405,8 +643,9 @@ int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
     memcpy(p, &nd, 4); p += 4;
     for (int i = 0; i < ndim; i++) { int32_t d = (int32_t)dd[i]; memcpy(p, &d, 4); p += 4; }
     memcpy(p, &eb, 8); p += 8; /* the user eb, not eb_final (src/vecsz_interface.hh:204) */
-    uint64_t lens[3] = {0, (uint64_t)huff, (uint64_t)nout};
+    uint64_t lens[3] = {(uint64_t)pad_len, (uint64_t)huff, (uint64_t)nout};
     memcpy(p, lens, 24); p += 24;
+    memcpy(p, pvals, 4 * pad_len); p += 4 * pad_len;
     put_u32be(p, (uint32_t)nc);
     put_u32be(p + 4, (uint32_t)(state_num / 2));
     p += 8;
@@ -421,7 +660,7 @@ int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
     *archive = a;
     *archive_size = total;
     *n_outlier = nout;
-    free(code); free(outl); free(freq); free(cb); free(cl);
+    free(code); free(outl); free(freq); free(cb); free(cl); free(pvals); free(halo);
     return ORC_OK;
 }
 
@@ -516,34 +755,46 @@ int orc_huffman_decode(const uint8_t* huff, size_t huff_size, size_t n, int32_t*
  * global raster walk computes the same values as the per-block loops. */
 int orc_reconstruct(int ndim, const size_t* dims, int block, int radius, double twoeb,
                     const int32_t* code, const float* outlier, float* out) {
+    return orc_reconstruct_halo(ndim, dims, block, radius, twoeb, code, outlier, NULL, out);
+}
+
+/* With padding the halo cells of _s hold the pad values (x_block_pad /
+ * x_edge_pad, src/utils/padding.hh:302-372; src/dualquant.hh:951-964,
+ * :1003-1016) and the 1D block start predicts from the pad value (:918-933). */
+int orc_reconstruct_halo(int ndim, const size_t* dims, int block, int radius, double twoeb,
+                         const int32_t* code, const float* outlier, const float* halo, float* out) {
     size_t d0 = dims[0], d1 = ndim >= 2 ? dims[1] : 1, d2 = ndim >= 3 ? dims[2] : 1;
     size_t n = d0 * d1 * d2, B = (size_t)block;
+    const size_t nb0 = orc_nblk(d0, block), nb1 = ndim >= 2 ? orc_nblk(d1, block) : 1;
     float* s = (float*)malloc(n * sizeof(float) + 4);
     if (!s) return ORC_ENOMEM;
+    const size_t sy = d0, sz = d0 * d1;
     for (size_t z = 0; z < d2; z++)
         for (size_t y = 0; y < d1; y++)
             for (size_t i = 0; i < d0; i++) {
                 size_t id = i + d0 * (y + d1 * z);
-                int hx = (i % B) != 0, hy = (y % B) != 0, hz = (z % B) != 0;
+                size_t ix = i % B, iy = y % B, iz = z % B;
+                const float* h = halo ? halo + 3 * ((i / B) + nb0 * ((y / B) + nb1 * (z / B))) : NULL;
                 float pred;
+#define NB(dz, dy, dx) halo_nb(ndim, h, dz, dy, dx, iz, iy, ix, s, id - (dx) - (dy) * sy - (dz) * sz)
                 if (ndim == 1) {
-                    pred = hx ? s[id - 1] : 0.0f;
+                    pred = NB(0, 0, 1);
                 } else if (ndim == 2) {
-                    float W = hx ? s[id - 1] : 0.0f;
-                    float N = hy ? s[id - d0] : 0.0f;
-                    float NW = (hx && hy) ? s[id - 1 - d0] : 0.0f;
+                    float W = NB(0, 0, 1);
+                    float N = NB(0, 1, 0);
+                    float NW = NB(0, 1, 1);
                     pred = N + W - NW;
                 } else {
-                    size_t sy = d0, sz = d0 * d1;
-                    float X = hx ? s[id - 1] : 0.0f;
-                    float Y = hy ? s[id - sy] : 0.0f;
-                    float Z = hz ? s[id - sz] : 0.0f;
-                    float XY = (hx && hy) ? s[id - 1 - sy] : 0.0f;
-                    float XZ = (hx && hz) ? s[id - 1 - sz] : 0.0f;
-                    float YZ = (hy && hz) ? s[id - sy - sz] : 0.0f;
-                    float XYZ = (hx && hy && hz) ? s[id - 1 - sy - sz] : 0.0f;
+                    float X = NB(0, 0, 1);
+                    float Y = NB(0, 1, 0);
+                    float Z = NB(1, 0, 0);
+                    float XY = NB(0, 1, 1);
+                    float XZ = NB(1, 0, 1);
+                    float YZ = NB(1, 1, 0);
+                    float XYZ = NB(1, 1, 1);
                     pred = XYZ - XY - XZ - YZ + X + Y + Z;
                 }
+#undef NB
                 int32_t c = code[id];
                 s[id] = c == 0 ? outlier[id] : (float)(pred + (float)(c - radius));
                 out[id] = (float)((double)s[id] * twoeb);
@@ -585,9 +836,18 @@ int orc_decompress(const uint8_t* a, size_t size, int r2r, double range, int dic
             o[i] = 0;
         }
     }
-    rc = orc_reconstruct(nd, dims, block, dict / 2, ebs.twoeb_d, code, o, out);
+    size_t nblocks = orc_nblk(dims[0], block) * (nd >= 2 ? orc_nblk(dims[1], block) : 1) *
+                     (nd >= 3 ? orc_nblk(dims[2], block) : 1);
+    float* halo = (float*)malloc(3 * nblocks * sizeof(float));
+    float* pv = (float*)malloc(4 * (size_t)pad_len + 4);
+    if (!halo || !pv) { free(code); free(o); free(halo); free(pv); return ORC_ENOMEM; }
+    memcpy(pv, a + meta, 4 * (size_t)pad_len);
+    orc_pad_halo_from_archive(nd, dims, block, pv, pad_len, halo);
+    rc = orc_reconstruct_halo(nd, dims, block, dict / 2, ebs.twoeb_d, code, o, halo, out);
     free(code);
     free(o);
+    free(halo);
+    free(pv);
     return rc;
 }
 

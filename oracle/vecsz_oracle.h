@@ -39,6 +39,36 @@ typedef struct {
 int orc_value_range(const float* x, size_t n, double* range);
 int orc_eb_family(int r2r, double eb, double range, orc_ebs* out);
 
+/* Padding (src/utils/padding.hh, src/dualquant.hh:36-46, :226-246,
+ * :553-573, :920-927, :951-964, :1003-1016, src/vecsz_interface.hh:70-90,
+ * :287-298).  Modes as szwf.{global,block,edge}_padding; value types as
+ * MAX_VALUE..CONST_VAL (src/utils/padding.hh:10-13). */
+enum { ORC_PAD_NONE = 0, ORC_PAD_GLOBAL = 1, ORC_PAD_BLOCK = 2, ORC_PAD_EDGE = 3 };
+enum { ORC_PADV_MAX = 0, ORC_PADV_MIN = 1, ORC_PADV_AVG = 2, ORC_PADV_CONST = 3 };
+
+/* Number of Lorenzo blocks along a dimension as the reference counts them:
+ * (d + B + 1) / B (src/dimensions.hh:31), one more than ceil(d / B) when
+ * d % B is 0 or B - 1. */
+size_t orc_nblk(size_t d, int block);
+
+/* Pad values of a compress: pad_vals[] as the archive stores them (*pad_len
+ * of them; pad_vals may be NULL to query the count) and, per block of the
+ * reference's block grid, the three halo values (z face, y face, x face) its
+ * dual quantization reads (halo may be NULL).  Elements outside the field
+ * read as +0.0. */
+int orc_pad_values(int ndim, const size_t* dims, int block, int mode, int type, float constant,
+                   const orc_ebs* ebs, const float* x, float* pad_vals, size_t* pad_len, float* halo);
+
+/* Halo values decompress derives from an archive's pad section
+ * (src/vecsz_interface.hh:287-298, x_lorenzo_*): 3 per block of the
+ * reference's block grid. */
+int orc_pad_halo_from_archive(int ndim, const size_t* dims, int block, const float* pad_vals,
+                              uint64_t pad_len, float* halo);
+
+/* Dual quantization with halo values (3 per block, NULL = the +0.0 halo). */
+int orc_dualquant_halo(int ndim, const size_t* dims, int block, int radius, const orc_ebs* ebs,
+                       const float* x, const float* halo, int32_t* code, float* outlier);
+
 /* Dual quantization of a whole field (dims x fastest, unused dims = 1).
  * code[] receives the int quant codes, outlier[] the per-element outlier
  * scratch exactly as Compress leaves them (src/vecsz_interface.hh:45-46,
@@ -66,6 +96,11 @@ int orc_compress(int ndim, const size_t* dims, int r2r, double eb, int dict, int
                  const float* x, uint8_t** archive, size_t* archive_size, size_t* n_outlier,
                  double* eb_final_out);
 
+/* Compress with padding (pad_mode ORC_PAD_*, pad_type ORC_PADV_*). */
+int orc_compress_pad(int ndim, const size_t* dims, int r2r, double eb, int dict, int block, int pad_mode,
+                     int pad_type, float pad_constant, const float* x, uint8_t** archive, size_t* archive_size,
+                     size_t* n_outlier, double* eb_final_out);
+
 /* Header fields of an archive payload (src/utils/datapack.hh:33-56). */
 int orc_archive_info(const uint8_t* archive, size_t size, int* ndim, size_t* dims, double* eb,
                      uint64_t* pad_len, uint64_t* huff_size, uint64_t* n_outlier);
@@ -81,6 +116,9 @@ int orc_decompress(const uint8_t* archive, size_t size, int r2r, double range, i
 /* Lorenzo reconstruction from codes + scattered outliers (src/dualquant.hh:915-1043). */
 int orc_reconstruct(int ndim, const size_t* dims, int block, int radius, double twoeb,
                     const int32_t* code, const float* outlier, float* out);
+/* The same with halo values (3 per block, NULL = the +0.0 halo). */
+int orc_reconstruct_halo(int ndim, const size_t* dims, int block, int radius, double twoeb,
+                         const int32_t* code, const float* outlier, const float* halo, float* out);
 
 void orc_free(void* p);
 

@@ -23,11 +23,12 @@ def golden():
     with open(os.path.join(d, "golden.json")) as f:
         man = json.load(f)
     npz = np.load(os.path.join(d, "golden.npz"), allow_pickle=False)
-    cases = []
+    cases, pad_cases = [], []
     for c in man["cases"]:
         n = c["name"]
-        cases.append(dict(c, x=npz[n + "__x"], archive=npz[n + "__archive"].tobytes(), out=npz[n + "__out"]))
-    return {"cases": cases, "digests": man["digests"]}
+        d = dict(c, x=npz[n + "__x"], archive=npz[n + "__archive"].tobytes(), out=npz[n + "__out"])
+        (pad_cases if "pad_mode" in c else cases).append(d)
+    return {"cases": cases, "pad_cases": pad_cases, "digests": man["digests"]}
 
 
 @pytest.fixture(scope="session")

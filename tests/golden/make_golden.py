@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 
 from oracle import pyoracle as po  # noqa: E402
 
-from tests.golden.cases import digest_cases, small_cases  # noqa: E402
+from tests.golden.cases import digest_cases, pad_cases, small_cases  # noqa: E402
 
 
 def sha(b: bytes) -> str:
@@ -52,6 +52,26 @@ def main():
             "value_range": po.value_range(x) if c["r2r"] else 0.0, "sha256_archive": sha(a),
         })
         print(f"{name:32s} {str(x.shape):16s} archive={len(a)} n_out={nout}")
+    # The reference appends pad values through one counter shared by its
+    # OpenMP threads: one thread gives the serial block order.
+    threads = po.ref().ref_set_threads(0)
+    po.ref().ref_set_threads(1)
+    for c in pad_cases():
+        x = c["x"]
+        kw = dict(pad_mode=c["pad_mode"], pad_type=c["pad_type"], pad_constant=c["pad_constant"])
+        a, nout = po.ref_compress(x, c["eb"], c["r2r"], dict_size=c["dict"], block=c["block"], threads=1, **kw)
+        y = po.ref_decompress(a, x, c["r2r"], dict_size=c["dict"], block=c["block"])
+        name = c["name"]
+        arrays[name + "__x"] = x
+        arrays[name + "__archive"] = np.frombuffer(a, dtype=np.uint8)
+        arrays[name + "__out"] = y.reshape(x.shape)
+        manifest["cases"].append({
+            "name": name, "shape": list(x.shape), "eb": c["eb"], "r2r": c["r2r"], "dict": c["dict"],
+            "block": c["block"], "n_outlier": int(nout), "archive_bytes": len(a),
+            "value_range": po.value_range(x) if c["r2r"] else 0.0, "sha256_archive": sha(a), **kw,
+        })
+        print(f"{name:32s} {str(x.shape):16s} archive={len(a)} n_out={nout}")
+    po.ref().ref_set_threads(threads)
     for c in digest_cases():
         x = c["x"]
         a, nout = po.ref_compress(x, c["eb"], c["r2r"], dict_size=c["dict"], block=c["block"])
