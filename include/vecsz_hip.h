@@ -54,7 +54,18 @@ typedef struct vecsz_params {
     double value_range;    /* r2r: value range to use; <= 0 on compress = compute max-min of the input */
     int block;             /* Lorenzo block size B: 8, 16, 32 or 64 (reference default 8) */
     int dict_size;         /* quant-code dictionary (reference default 4096); radius = dict_size / 2 */
+    /* Compress only: padding (--padding, src/argument_parser/argparse.cc:399-415;
+     * src/utils/padding.hh).  Decompress reads the mode from the archive's pad
+     * section (src/vecsz_interface.hh:287-298). */
+    int pad_mode;          /* VECSZ_PAD_NONE (default), _GLOBAL, _BLOCK or _EDGE */
+    int pad_type;          /* VECSZ_PADV_MAX, _MIN, _AVG or _CONST (default) */
+    float pad_constant;    /* value of VECSZ_PADV_CONST */
 } vecsz_params;
+
+/* Padding modes (szwf.global_padding / block_padding / edge_padding) and pad
+ * value types (MAX_VALUE..CONST_VAL, src/utils/padding.hh:10-13). */
+enum { VECSZ_PAD_NONE = 0, VECSZ_PAD_GLOBAL = 1, VECSZ_PAD_BLOCK = 2, VECSZ_PAD_EDGE = 3 };
+enum { VECSZ_PADV_MAX = 0, VECSZ_PADV_MIN = 1, VECSZ_PADV_AVG = 2, VECSZ_PADV_CONST = 3 };
 
 typedef struct vecsz_ctx vecsz_ctx;
 

@@ -23,6 +23,9 @@
 //    zstd at level 3 over the payload and returns that malloc'ed stream with
 //    *lossless_size set (:206-217); Decompress inflates a lossless_size > 0
 //    stream to the target_size payload first (:237-244).
+//  - padding (ap->szwf.{global,block,edge}_padding, ap->pad_type,
+//    ap->pad_constant) is applied when the argparse type has those members;
+//    Decompress takes it from the archive, as the reference does (:287-298).
 // Differences: errors throw std::runtime_error instead of exit(1); only
 // T = float is supported (the only type the reference instantiates,
 // src/vecsz.cc:37); the zstd pass uses the system zstd runtime (its frames
@@ -37,6 +40,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "vecsz_hip.h"
@@ -80,6 +84,18 @@ inline double value_range_of_file(const std::string& path, size_t n) {
     return value_range(buf.data(), n);
 }
 
+// The reference's ArgParse carries the padding options (szwf.*_padding,
+// pad_type, pad_constant; src/argument_parser/argparse.hh:34-36, :75-77);
+// an argparse type without them compresses without padding.
+template <typename AP, typename = void>
+struct has_padding : std::false_type {};
+template <typename AP>
+struct has_padding<AP, std::void_t<decltype(std::declval<const AP&>().szwf.global_padding),
+                                   decltype(std::declval<const AP&>().szwf.block_padding),
+                                   decltype(std::declval<const AP&>().szwf.edge_padding),
+                                   decltype(std::declval<const AP&>().pad_type),
+                                   decltype(std::declval<const AP&>().pad_constant)>> : std::true_type {};
+
 template <typename AP>
 vecsz_params params_of(const AP* ap) {
     vecsz_params p;
@@ -92,6 +108,15 @@ vecsz_params params_of(const AP* ap) {
     p.eb = ap->eb;
     p.block = ap->block_size;
     p.dict_size = ap->dict_size;
+    if constexpr (has_padding<AP>::value) {
+        // one mode at a time, global over block over edge (src/argument_parser/argparse.cc:156-169)
+        p.pad_mode = ap->szwf.global_padding ? VECSZ_PAD_GLOBAL
+                     : ap->szwf.block_padding ? VECSZ_PAD_BLOCK
+                     : ap->szwf.edge_padding  ? VECSZ_PAD_EDGE
+                                              : VECSZ_PAD_NONE;
+        p.pad_type = ap->pad_type;
+        p.pad_constant = ap->pad_constant;
+    }
     return p;
 }
 

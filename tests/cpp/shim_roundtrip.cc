@@ -19,6 +19,7 @@ struct Files {
 };
 struct Szwf {
     bool lossless_pass = false, lossless_gzip = false;
+    bool global_padding = false, block_padding = false, edge_padding = false;
 };
 struct ArgParse {
     int ndim = 3;
@@ -29,15 +30,25 @@ struct ArgParse {
     int dict_size = 4096;
     Szwf szwf;
     Files files;
+    float pad_constant = 0;
+    int pad_type = 3;
 };
 
 int main(int argc, char** argv) {
     const std::string lossless = argc > 1 ? argv[1] : "none";
+    const std::string padding = argc > 2 ? argv[2] : "none";  // "none" or "<mode>,<type id>"
     ArgParse ap;
     ap.dim4 = {40, 30, 20, 1};
     ap.eb = 1e-3;
     ap.szwf.lossless_pass = lossless != "none";
     ap.szwf.lossless_gzip = lossless == "gzip";
+    if (padding != "none") {
+        ap.szwf.global_padding = padding.find("global") != std::string::npos;
+        ap.szwf.block_padding = padding.find("block") != std::string::npos;
+        ap.szwf.edge_padding = padding.find("edge") != std::string::npos;
+        ap.pad_type = padding.back() - '0';
+        ap.pad_constant = 0.75f;
+    }
     const size_t n = 40 * 30 * 20;
     float* x = new float[n];
     float* orig = new float[n];

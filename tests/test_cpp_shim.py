@@ -42,3 +42,14 @@ def test_shim_round_trip_on_gpu(tmp_path, lossless):
     # the reference's f32 reciprocal and rounding allow |x| * 2^-22 beyond eb (D15); |x| <= 3.01 here
     assert maxerr <= eb + 3.01 * 2.0 ** -22
     assert out[5:] == ["3", "40", "20"]  # Decompress set ndim / dims from the archive
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("padding", ["block,0", "edge,2", "global,3"])
+def test_shim_round_trip_padded_on_gpu(tmp_path, padding):
+    """ap->szwf.*_padding / pad_type / pad_constant reach the library."""
+    exe = _build(tmp_path)
+    out = subprocess.run([exe, "none", padding], capture_output=True, text=True, timeout=120).stdout.split()
+    assert out[0] == "ok", out
+    maxerr, eb = float(out[3]), float(out[4])
+    assert maxerr <= eb + 3.01 * 2.0 ** -22

@@ -40,7 +40,14 @@ class Params(ctypes.Structure):
         ("value_range", ctypes.c_double),
         ("block", ctypes.c_int),
         ("dict_size", ctypes.c_int),
+        ("pad_mode", ctypes.c_int),
+        ("pad_type", ctypes.c_int),
+        ("pad_constant", ctypes.c_float),
     ]
+
+
+PAD_MODES = {"none": 0, "global": 1, "block": 2, "edge": 3}
+PAD_TYPES = {"max": 0, "min": 1, "avg": 2, "const": 3, "constant": 3}
 
 
 VECSZ_LOSSLESS_GZIP = 1

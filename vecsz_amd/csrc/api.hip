@@ -104,6 +104,7 @@ struct vecsz_ctx {
     int nworkers = 8;
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
+    DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
     Codebook cb;
     DecodeTables dt;  // kept between calls: its vectors keep their capacity
     // metadata of the last successful compress
@@ -227,10 +228,25 @@ int check_params(vecsz_ctx* c, const vecsz_params* p, bool need_dims) {
     }
     int R = p->dict_size / 2;
     if (vec_threshold(R) != R) return fail(c, VECSZ_EINVAL, "dict_size not supported (AVX test differs from |e|<R)");
+    if (p->pad_mode < VECSZ_PAD_NONE || p->pad_mode > VECSZ_PAD_EDGE) return fail(c, VECSZ_EINVAL, "pad_mode");
+    if (p->pad_type < VECSZ_PADV_MAX || p->pad_type > VECSZ_PADV_CONST) return fail(c, VECSZ_EINVAL, "pad_type");
     return VECSZ_OK;
 }
 
 uint64_t meta_bytes(int ndim) { return 4ull * (ndim + 1) + 8 + 24; }
+
+// Blocks of the reference's grid and the pad values a compress stores:
+// 1 (global), one per block (block) or ndim per block (edge)
+// (src/vecsz_interface.hh:73-90, src/dimensions.hh:31).
+uint64_t ref_blocks(int ndim, const uint64_t* dims, int B) {
+    uint64_t nb = 1;
+    for (int i = 0; i < ndim; i++) nb *= ref_nblk(dims[i], B);
+    return nb;
+}
+uint64_t pad_len_for(int ndim, const uint64_t* dims, int B, int mode) {
+    const uint64_t nb = ref_blocks(ndim, dims, B);
+    return mode == VECSZ_PAD_NONE ? 0 : mode == VECSZ_PAD_GLOBAL ? 1 : mode == VECSZ_PAD_BLOCK ? nb : (uint64_t)ndim * nb;
+}
 
 struct EbFamily {
     double eb_final, r_d, twoeb;
@@ -321,7 +337,46 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         a.r_f = eb.r_f;
         a.r_d = eb.r_d;
     }
-    const bool x4 = dualquant_use_x4(ndim, B, d_in, d0);
+    // padding: pad values and the halo the dual quantization reads
+    a.halo = nullptr;
+    a.halo_stride = 0;
+    a.hb0 = ref_nblk(d0, B);
+    a.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
+    const uint64_t hb2 = ndim >= 3 ? ref_nblk(d2, B) : 1;
+    const int pmode = p->pad_mode;
+    const uint64_t pad_len = pad_len_for(ndim, p->dims, B, pmode);
+    if (pmode != VECSZ_PAD_NONE) {
+        const uint64_t nbref = a.hb0 * a.hb1 * hb2;
+        if (!c->padv.ensure(pad_len * 4 + 16) || !c->halo.ensure((pmode == VECSZ_PAD_GLOBAL ? 1 : nbref) * 12 + 16) ||
+            !c->pad_scratch.ensure(64))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        PadArgs pa;
+        pa.x = d_in;
+        pa.d0 = d0;
+        pa.d1 = d1;
+        pa.d2 = d2;
+        pa.n = n;
+        pa.ndim = ndim;
+        pa.B = B;
+        pa.mode = pmode;
+        pa.type = p->pad_type;
+        pa.constant = p->pad_constant;
+        pa.hb0 = a.hb0;
+        pa.hb1 = a.hb1;
+        pa.hb2 = hb2;
+        pa.ebd = device_range ? c->ebdev.as<EbDev>() : nullptr;
+        pa.r_d = device_range ? 0.0 : eb.r_d;
+        StageScope s(c, "pad");
+        if (pmode == VECSZ_PAD_GLOBAL) {
+            launch_pad_global(pa, c->pad_scratch.p, c->padv.as<float>(), c->halo.as<float>(), c->stream);
+            a.halo_stride = 0;
+        } else {
+            launch_pad_blocks(pa, c->padv.as<float>(), c->halo.as<float>(), c->stream);
+            a.halo_stride = 3;
+        }
+        a.halo = c->halo.as<float>();
+    }
+    const bool x4 = pmode == VECSZ_PAD_NONE && dualquant_use_x4(ndim, B, d_in, d0);
     const uint64_t tiles_x = x4 ? (d0 + 255) / 256 : a.ntiles_x;
     const uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : tiles_x * a.ntiles_y * a.ntiles_z;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 768));
@@ -364,8 +419,9 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint64_t mb = meta_bytes(ndim);
     const uint64_t tb = cb.tree.size();
     const uint64_t head = mb + 8 + tb;
-    if (head > out_cap) {
-        *out_bytes = head;
+    const uint64_t pad_bytes = 4 * pad_len;
+    if (head + pad_bytes > out_cap) {
+        *out_bytes = head + pad_bytes;
         return fail(c, VECSZ_ENOMEM, "output buffer too small");
     }
     const size_t book_bytes = (size_t)dict * 9;
@@ -380,6 +436,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
             memcpy(&hdr[4 + 4 * i], &d, 4);
         }
         memcpy(&hdr[4 * (ndim + 1)], &p->eb, 8);
+        memcpy(&hdr[4 * (ndim + 1) + 8], &pad_len, 8);
         put_u32be(&hdr[mb], (uint32_t)cb.node_count);
         put_u32be(&hdr[mb + 4], (uint32_t)dict);  // stateNum / 2
         memcpy(&hdr[mb + 8], cb.tree.data(), tb);
@@ -388,7 +445,14 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
         memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
     }
-    HIPCHK(c, hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+    if (pad_bytes == 0) {
+        HIPCHK(c, hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+    } else {
+        // [metadata][pad values][Huffman header + tree] (src/utils/datapack.hh:17-29)
+        HIPCHK(c, hipMemcpyAsync(d_out, hdr, mb, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
+    }
     HIPCHK(c, hipMemcpyAsync(c->book.p, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
     const uint32_t* cw_hi = c->book.as<uint32_t>();
     const uint32_t* cw_lo = cw_hi + dict;
@@ -404,7 +468,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), nchunks,
                           c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
                           c->scan_tot.as<uint64_t>(), c->stream);
-        launch_enc_summary(c->scan_tot.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb, tb, out_cap,
+        launch_enc_summary(c->scan_tot.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb, pad_bytes, tb, out_cap,
                            c->err.as<uint32_t>(), c->stream);
     }
     {
@@ -552,7 +616,6 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     double ebu;
     rc = parse_meta(h, head_avail, in_bytes, &ndim, dims, &ebu, &pad_len, &huff_size, &nout);
     if (rc) return fail(c, rc, "archive header is corrupt");
-    if (pad_len != 0) return fail(c, VECSZ_EINVAL, "archives with padding values are not supported");
     const uint64_t n = dims[0] * dims[1] * dims[2];
     if (n > out_cap) return fail(c, VECSZ_ENOMEM, "output buffer too small");
     const bool r2r = p->mode == VECSZ_MODE_R2R;
@@ -569,6 +632,13 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         HostTimer ht(c, "host_tables");
         const uint64_t hoff = mb + 4 * pad_len;
         const uint8_t* hp = h + hoff;
+        if (hoff + 8 > head_avail) {
+            // a pad section longer than the first read: fetch the Huffman header
+            if (!c->pin2.ensure(64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+            hp = c->pin2.as<uint8_t>();
+            HIPCHK(c, hipMemcpyAsync(c->pin2.p, d_huff, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
         int32_t ncs = (int32_t)(((uint32_t)hp[0] << 24) | ((uint32_t)hp[1] << 16) | ((uint32_t)hp[2] << 8) | hp[3]);
         if (ncs <= 0) return fail(c, VECSZ_ECORRUPT, "Huffman node count");
         tb = tree_byte_size((uint64_t)ncs);
@@ -703,6 +773,26 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     // Lorenzo reconstruction.
     const uint64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
     ReconArgs ra;
+    ra.halo = nullptr;
+    ra.halo_stride = 0;
+    ra.hb0 = ref_nblk(d0, B);
+    ra.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
+    if (pad_len > 0) {
+        // padding mode from the pad value count (src/vecsz_interface.hh:287-298);
+        // 1D without a mode still reads pad_vals[0] (src/dualquant.hh:925-927)
+        const uint64_t nbref = ref_blocks(ndim, dims, B);
+        const int mode = pad_len == 1 ? VECSZ_PAD_GLOBAL
+                         : pad_len == nbref ? VECSZ_PAD_BLOCK
+                         : pad_len == (uint64_t)ndim * nbref ? VECSZ_PAD_EDGE : VECSZ_PAD_NONE;
+        if (mode != VECSZ_PAD_NONE || ndim == 1) {
+            if (!c->padv.ensure(pad_len * 4 + 16) || !c->halo.ensure(nbref * 12 + 16))
+                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+            HIPCHK(c, hipMemcpyAsync(c->padv.p, d_in + mb, 4 * pad_len, hipMemcpyDeviceToDevice, c->stream));
+            launch_pad_halo(ndim, mode, c->padv.as<float>(), pad_len, nbref, c->halo.as<float>(), c->stream);
+            ra.halo = c->halo.as<float>();
+            ra.halo_stride = 3;
+        }
+    }
     ra.d0 = d0;
     ra.d1 = d1;
     ra.d2 = d2;
@@ -760,6 +850,9 @@ void vecsz_params_default(vecsz_params* p) {
     p->value_range = 0;
     p->block = 8;
     p->dict_size = 4096;
+    p->pad_mode = VECSZ_PAD_NONE;
+    p->pad_type = VECSZ_PADV_CONST;  // pad_type{3} (src/argument_parser/argparse.hh:77)
+    p->pad_constant = 0.f;
 }
 
 int vecsz_ctx_create(vecsz_ctx** out, int device, void* stream) {
@@ -808,7 +901,8 @@ int vecsz_compress_bound(const vecsz_params* p, uint64_t* bytes) {
     const uint64_t nsym = std::min<uint64_t>((uint64_t)p->dict_size, n);
     const uint64_t tree = tree_byte_size(2 * nsym - 1);
     // every symbol at most 64 bits, every element possibly an outlier
-    *bytes = meta_bytes(p->ndim) + 8 + tree + n * 8 + n * 4 + 8;
+    const int B = valid_block(p->block) ? p->block : 8;
+    *bytes = meta_bytes(p->ndim) + 4 * pad_len_for(p->ndim, p->dims, B, p->pad_mode) + 8 + tree + n * 8 + n * 4 + 8;
     return VECSZ_OK;
 }
 

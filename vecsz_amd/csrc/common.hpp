@@ -44,7 +44,21 @@ struct DQArgs {
     int reverse;           // visit tiles last-to-first (reuses what a preceding pass left in cache)
     const EbDev* ebd;      // when set, r_f / r_d come from here (r2r range computed on the device)
     uint64_t ntiles_x, ntiles_y, ntiles_z;
+    // Padding: halo values {z face, y face, x face} of block (bx, by, bz) at
+    // halo + halo_stride * (bx + hb0 * (by + hb1 * bz)), blocks counted on the
+    // reference's grid ((d + B + 1) / B per dimension); halo_stride 0 = one
+    // set for every block; halo == nullptr = the +0.0 halo.
+    const float* halo;
+    uint64_t halo_stride, hb0, hb1;
 };
+
+// Padding modes and pad value types (szwf.*_padding; MAX_VALUE..CONST_VAL,
+// src/utils/padding.hh:10-13).
+enum PadMode { kPadNone = 0, kPadGlobal = 1, kPadBlock = 2, kPadEdge = 3 };
+enum PadType { kPadMax = 0, kPadMin = 1, kPadAvg = 2, kPadConst = 3 };
+
+// Blocks along a dimension as the reference counts them (src/dimensions.hh:31).
+__host__ __device__ inline uint64_t ref_nblk(uint64_t d, int B) { return (d + (uint64_t)B + 1) / (uint64_t)B; }
 
 // Per-field encode summary written by the scan kernel.
 struct EncSummary {

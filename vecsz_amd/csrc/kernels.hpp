@@ -18,7 +18,7 @@ bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0);
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
                        uint64_t nchunks, hipStream_t st);
 void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
-                        uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st);
+                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st);
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st);
@@ -61,11 +61,32 @@ struct ReconArgs {
     int radius;
     double twoeb;
     uint64_t ntiles_x, ntiles_y, ntiles_z;
+    const float* halo;  // as DQArgs::halo; nullptr = the +0.0 halo
+    uint64_t halo_stride, hb0, hb1;
 };
 void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* sparse, float* out, const ReconArgs& a,
                         uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st);
 void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
                                  const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st);
+
+// Padding (kernels_pad.hip)
+struct PadArgs {
+    const float* x;
+    uint64_t d0, d1, d2, n;
+    int ndim, B, mode, type;
+    float constant;
+    uint64_t hb0, hb1, hb2;  // reference block grid
+    const EbDev* ebd;        // r_d from here when set, else r_d below
+    double r_d;
+};
+// Per-block pad values (block / edge modes): pad_out in archive order, halo 3 per block.
+void launch_pad_blocks(const PadArgs& a, float* pad_out, float* halo, hipStream_t st);
+// Global mode: pad_out[0] as stored, halo[0..2] as the dual quantization uses it.
+// scratch: 32 bytes of device memory.
+void launch_pad_global(const PadArgs& a, void* scratch, float* pad_out, float* halo, hipStream_t st);
+// Decompress: halo (3 per block of nblocks) from an archive's pad section.
+void launch_pad_halo(int ndim, int mode, const float* pad, uint64_t pad_len, uint64_t nblocks, float* halo,
+                     hipStream_t st);
 
 // Reconstruction quality (kernels_verify.hip)
 struct VerifyPart1 {

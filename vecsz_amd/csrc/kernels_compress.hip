@@ -224,7 +224,25 @@ __device__ __forceinline__ uint32_t postquant(float q, float pred, float thr, fl
 // x; the x-1 neighbour comes from DPP, y-1 from the previous row's registers
 // and z-1 from the previous plane's registers.  A workgroup is 4 wavefronts
 // working on consecutive tiles of a grid-stride loop.
-template <int B>
+// Halo values {z, y, x face} of the block holding x (PAD kernels only).
+struct Halo {
+    float z, y, x;
+};
+__device__ __forceinline__ Halo load_halo(const DQArgs& a, bool xin, uint64_t x, int B, uint64_t by, uint64_t bz) {
+    Halo h{0.f, 0.f, 0.f};
+    if (xin) {
+        const float* p = a.halo + a.halo_stride * ((x / B) + a.hb0 * (by + a.hb1 * bz));
+        h.z = p[0];
+        h.y = p[1];
+        h.x = p[2];
+    }
+    return h;
+}
+
+// PAD: the halo cells hold pad values instead of +0.0 (src/dualquant.hh:
+// 226-246, :553-573): a neighbour across the block's z face reads h.z, else
+// across its y face h.y, else h.x (src/utils/padding.hh:245, :290).
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
                                                       uint32_t* __restrict__ partials, DQArgs a) {
@@ -248,6 +266,8 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
         const bool scal = x >= a.x8;  // partial x-group: scalar path
         const bool xstart = (x % B) == 0;
         const float thr = scal ? a.thr_scalar : a.thr_vec;
+        Halo h{0.f, 0.f, 0.f};
+        if constexpr (PAD) h = load_halo(a, xin, x, B, ty, tz);
         float prevplane[B];
 #pragma unroll
         for (int j = 0; j < B; j++) prevplane[j] = 0.f;
@@ -256,6 +276,7 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
             if (z >= a.d2) break;
             float qrow = 0.f, xrow = 0.f;      // q and X of row y-1 (this plane)
             float zrow = 0.f, xzrow = 0.f;     // Z and XZ of row y-1
+            const float hyz = zz > 0 ? h.y : h.z;  // the y-face halo of this plane's first row
 #pragma unroll
             for (int yy = 0; yy < B; yy++) {
                 const uint64_t y = y0 + yy;
@@ -265,16 +286,16 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
                     float v = ok ? __builtin_nontemporal_load(&in[idx]) : 0.f;
                     float q = scal ? prequant_scalar(v, a.r_d) : prequant_vec(v, a.r_f);
                     float X = wave_shr1(q);
-                    float Z = zz > 0 ? prevplane[yy] : 0.f;
+                    float Z = zz > 0 ? prevplane[yy] : h.z;
                     float XZ = wave_shr1(Z);
                     if (xstart) {
-                        X = 0.f;
-                        XZ = 0.f;
+                        X = h.x;
+                        XZ = zz > 0 ? h.x : h.z;
                     }
-                    float Y = yy > 0 ? qrow : 0.f;
-                    float XY = yy > 0 ? xrow : 0.f;
-                    float YZ = yy > 0 ? zrow : 0.f;
-                    float XYZ = yy > 0 ? xzrow : 0.f;
+                    float Y = yy > 0 ? qrow : h.y;
+                    float XY = yy > 0 ? xrow : h.y;
+                    float YZ = yy > 0 ? zrow : hyz;
+                    float XYZ = yy > 0 ? xzrow : hyz;
                     float pred;
                     if (!scal) {
                         float dist1 = X + (Y + Z);
@@ -306,7 +327,9 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
 // Tile of 64 (x) x B (y) per wavefront.  Elements with x >= 8*floor(DIM0/8)
 // are never coded by the reference (the `continue` at src/dualquant.hh:494-495):
 // code 0, outlier +0.0.
-template <int B>
+// PAD: a neighbour across the block's x face reads h.x, else across its
+// y face h.y (src/utils/padding.hh:226, :275).
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
                                                       uint32_t* __restrict__ partials, DQArgs a) {
@@ -327,6 +350,8 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
         const bool xin = x < a.d0;
         const bool lost = x >= a.x8;
         const bool xstart = (x % B) == 0;
+        Halo h{0.f, 0.f, 0.f};
+        if constexpr (PAD) h = load_halo(a, xin, x, B, ty, 0);
         float qrow = 0.f, xrow = 0.f;
 #pragma unroll
         for (int yy = 0; yy < B; yy++) {
@@ -336,9 +361,9 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
                 float v = xin ? __builtin_nontemporal_load(&in[idx]) : 0.f;
                 float q = prequant_vec(v, a.r_f);
                 float W = wave_shr1(q);
-                if (xstart) W = 0.f;
-                float N = yy > 0 ? qrow : 0.f;
-                float NW = yy > 0 ? xrow : 0.f;
+                if (xstart) W = h.x;
+                float N = yy > 0 ? qrow : h.y;
+                float NW = yy > 0 ? xrow : (xstart ? h.x : h.y);
                 float pred = W + (N - NW);
                 uint32_t code = postquant(q, pred, a.thr_vec, a.radius_f);
                 if (lost) code = 0;
@@ -570,7 +595,12 @@ __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict
 // elements take the AVX prequant (src/dualquant.hh:83-93); the postquant test
 // is the AVX one in blocks ending before DIM0 and the scalar one in the last
 // block (src/dualquant.hh:114, :190-205).
-template <int B>
+// PAD: a block's first element predicts from the pad value, which 1D block
+// and edge padding keep raw (src/dualquant.hh:37-42), so the prediction error
+// may be non-integral: then the AVX forms' sqrt(e*e) < R test and
+// round-to-nearest conversion (_mm256_cvtps_epi32, :172) and the scalar
+// forms' |e| < R and truncation (:200) are evaluated as written.
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                       float* __restrict__ sparse, unsigned long long* __restrict__ hist,
                                                       uint32_t* __restrict__ partials, DQArgs a) {
@@ -600,15 +630,27 @@ __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ 
         for (int j = 0; j < 8; j++) q[j] = prequant_vec(q[j], a.r_f);
         float prev = wave_shr1(q[7]);
         if (lane == 0 && x0 > 0 && g < ngroups) prev = prequant_vec(in[x0 - 1], a.r_f);
-        if ((x0 % B) == 0) prev = 0.f;
+        if ((x0 % B) == 0) {
+            prev = 0.f;
+            if constexpr (PAD)
+                if (g < ngroups) prev = a.halo[a.halo_stride * (x0 / B) + 2];
+        }
         if (g < ngroups) {
             const uint64_t bstart = x0 - (x0 % B);
-            const float thr = (bstart + B < a.d0) ? a.thr_vec : a.thr_scalar;
+            const bool vec = bstart + B < a.d0;
+            const float thr = vec ? a.thr_vec : a.thr_scalar;
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const uint64_t idx = x0 + j;
                 float pred = j == 0 ? prev : q[j - 1];
-                uint32_t code = postquant(q[j], pred, thr, a.radius_f);
+                uint32_t code;
+                if constexpr (PAD) {
+                    const float e = q[j] - pred;
+                    const bool quant = vec ? __builtin_sqrtf(e * e) < a.radius_f : __builtin_fabsf(e) < a.radius_f;
+                    code = quant ? (uint32_t)(int)(vec ? __builtin_rintf(e + a.radius_f) : e + a.radius_f) : 0u;
+                } else {
+                    code = postquant(q[j], pred, thr, a.radius_f);
+                }
                 if (idx < a.n) {
                     codes[idx] = (uint16_t)code;
                     if (code == 0) sparse[idx] = q[j];
@@ -713,19 +755,19 @@ __global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__
 // totals, and the two length fields of the payload header
 // (src/utils/datapack.hh:288) written in place.
 __global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
-                              uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t tree_bytes,
-                              uint64_t out_cap, uint32_t* __restrict__ err) {
+                              uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
+                              uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const uint64_t tb = totals[0], tz = totals[1];
     const uint64_t enc = (tb + 7) / 8;
     const uint64_t huff = 8 + tree_bytes + enc;
-    const uint64_t stream_byte = meta_bytes + 8 + tree_bytes;
+    const uint64_t stream_byte = meta_bytes + pad_bytes + 8 + tree_bytes;
     EncSummary s;
     s.total_bits = tb;
     s.n_outlier = tz;
     s.huff_size = huff;
-    s.archive_size = meta_bytes + huff + 4 * tz;
-    s.outlier_byte_off = meta_bytes + huff;
+    s.archive_size = meta_bytes + pad_bytes + huff + 4 * tz;
+    s.outlier_byte_off = meta_bytes + pad_bytes + huff;
     s.stream_bit_off = stream_byte * 8;
     s.stream_end_byte = stream_byte + enc;
     *sum = s;
@@ -1018,7 +1060,8 @@ void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* 
                       uint32_t* partials, const DQArgs& a0, int grid, hipStream_t st) {
     const int lds = dq_lds_bytes(a0.dict);
     DQArgs a = a0;
-    const bool x4 = dualquant_use_x4(ndim, B, in, a.d0);
+    const bool pad = a.halo != nullptr;
+    const bool x4 = !pad && dualquant_use_x4(ndim, B, in, a.d0);
     if (x4) a.ntiles_x = (a.d0 + 255) / 256;
 #define DQ_CASE(ND, KERN, BB)                                                                              \
     if (ndim == ND && B == BB) {                                                                           \
@@ -1032,6 +1075,27 @@ void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* 
         DQ_CASE(2, k_dualquant_2d_x4, 16)
         DQ_CASE(2, k_dualquant_2d_x4, 32)
         DQ_CASE(2, k_dualquant_2d_x4, 64)
+    }
+    if (pad) {
+#define DQ_PAD(ND, KERN, BB)                                                                                       \
+    if (ndim == ND && B == BB) {                                                                                   \
+        hipLaunchKernelGGL((KERN<BB, true>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
+        goto launched;                                                                                             \
+    }
+        DQ_PAD(3, k_dualquant_3d, 8)
+        DQ_PAD(3, k_dualquant_3d, 16)
+        DQ_PAD(3, k_dualquant_3d, 32)
+        DQ_PAD(3, k_dualquant_3d, 64)
+        DQ_PAD(2, k_dualquant_2d, 8)
+        DQ_PAD(2, k_dualquant_2d, 16)
+        DQ_PAD(2, k_dualquant_2d, 32)
+        DQ_PAD(2, k_dualquant_2d, 64)
+        DQ_PAD(1, k_dualquant_1d, 8)
+        DQ_PAD(1, k_dualquant_1d, 16)
+        DQ_PAD(1, k_dualquant_1d, 32)
+        DQ_PAD(1, k_dualquant_1d, 64)
+#undef DQ_PAD
+        return;
     }
     DQ_CASE(3, k_dualquant_3d, 8)
     DQ_CASE(3, k_dualquant_3d, 16)
@@ -1063,9 +1127,9 @@ void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab
 }
 
 void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
-                        uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st) {
-    hipLaunchKernelGGL(k_enc_summary, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, tree_bytes, out_cap,
-                       err);
+                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st) {
+    hipLaunchKernelGGL(k_enc_summary, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, pad_bytes,
+                       tree_bytes, out_cap, err);
 }
 
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,

@@ -609,7 +609,31 @@ __device__ __forceinline__ void flag_blocks(bool bad, int B, uint64_t blk_x0, ui
     }
 }
 
-template <int B>
+// Halo values of the block holding x, for the integer recurrences: exact
+// when every one is an integer below kExact (else the block is listed).
+struct IHalo {
+    int32_t z, y, x;
+    bool exact;
+};
+__device__ __forceinline__ IHalo load_ihalo(const ReconArgs& a, bool xin, uint64_t x, int B, uint64_t by,
+                                            uint64_t bz) {
+    IHalo h{0, 0, 0, true};
+    if (xin) {
+        const float* p = a.halo + a.halo_stride * ((x / B) + a.hb0 * (by + a.hb1 * bz));
+        h.exact = exact_outlier(p[0]) && exact_outlier(p[1]) && exact_outlier(p[2]);
+        h.z = h.exact ? (int32_t)p[0] : 0;
+        h.y = h.exact ? (int32_t)p[1] : 0;
+        h.x = h.exact ? (int32_t)p[2] : 0;
+    }
+    return h;
+}
+
+// PAD: with halo values h (x_block_pad / x_edge_pad), r = s - Y - Z + YZ
+// is still a running sum along x, with Y, Z, YZ read from the halo faces in
+// the first row / plane of the block and a start of h.x - h.y in the block's
+// first row of its first plane (the only row where X - XY - XZ + XYZ of the
+// halo does not cancel).
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
@@ -628,6 +652,11 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
         const bool xin = x < a.d0;
         const bool bstart = (lane % B) == 0;
         bool bad = false;
+        IHalo h{0, 0, 0, true};
+        if constexpr (PAD) {
+            h = load_ihalo(a, xin, x, B, ty, tz);
+            bad = !h.exact;
+        }
         int32_t prevplane[B];
 #pragma unroll
         for (int j = 0; j < B; j++) prevplane[j] = 0;
@@ -642,9 +671,9 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
                     const uint64_t idx = x + y * sy + z * sz;
                     const uint32_t c = xin ? codes[idx] : (uint32_t)a.radius;
                     const bool isout = xin && c == 0;
-                    const int32_t Y = yy > 0 ? srow : 0;
-                    const int32_t Z = zz > 0 ? prevplane[yy] : 0;
-                    const int32_t YZ = (yy > 0 && zz > 0) ? zrow : 0;
+                    const int32_t Y = yy > 0 ? srow : h.y;
+                    const int32_t Z = zz > 0 ? prevplane[yy] : h.z;
+                    const int32_t YZ = (yy > 0 && zz > 0) ? zrow : (zz == 0 ? h.z : h.y);
                     int32_t g0;
                     float v = 0.f;
                     if (isout) {
@@ -653,6 +682,7 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
                         g0 = (int32_t)v - Y - Z + YZ;
                     } else {
                         g0 = (int32_t)c - a.radius;
+                        if (PAD && bstart && zz == 0 && yy == 0) g0 += h.x - h.y;
                     }
                     const int32_t G = seg_scan<B>(g0, isout || bstart);
                     const int32_t sv = G + Y + Z - YZ;
@@ -668,7 +698,9 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
     }
 }
 
-template <int B>
+// PAD: r = s - N runs along x from 0 at the block start (W - NW of the
+// x-face halo cancels), with N = h.y in the block's first row.
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
@@ -685,6 +717,11 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
         const bool xin = x < a.d0;
         const bool bstart = (lane % B) == 0;
         bool bad = false;
+        IHalo h{0, 0, 0, true};
+        if constexpr (PAD) {
+            h = load_ihalo(a, xin, x, B, ty, 0);
+            bad = !h.exact;
+        }
         int32_t srow = 0;
 #pragma unroll
         for (int yy = 0; yy < B; yy++) {
@@ -693,7 +730,7 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
                 const uint64_t idx = x + y * a.d0;
                 const uint32_t c = xin ? codes[idx] : (uint32_t)a.radius;
                 const bool isout = xin && c == 0;
-                const int32_t N = yy > 0 ? srow : 0;
+                const int32_t N = yy > 0 ? srow : h.y;
                 int32_t g0;
                 float v = 0.f;
                 if (isout) {
@@ -714,7 +751,9 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
     }
 }
 
-template <int B>
+// PAD: the running sum starts from the block's pad value (a raw, usually
+// non-integral value for 1D block / edge padding: such blocks are listed).
+template <int B, bool PAD = false>
 __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
@@ -728,6 +767,11 @@ __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ c
         const bool xin = x < a.d0;
         const bool bstart = (lane % B) == 0;
         bool bad = false;
+        IHalo h{0, 0, 0, true};
+        if constexpr (PAD) {
+            h = load_ihalo(a, xin, x, B, 0, 0);
+            bad = !h.exact;
+        }
         const uint32_t c = xin ? codes[x] : (uint32_t)a.radius;
         const bool isout = xin && c == 0;
         int32_t g0;
@@ -738,6 +782,7 @@ __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ c
             g0 = (int32_t)v;
         } else {
             g0 = (int32_t)c - a.radius;
+            if (PAD && bstart) g0 += h.x;
         }
         const int32_t sv = seg_scan<B>(g0, isout || bstart);
         if (sv >= kExact || sv <= -kExact) bad = true;
@@ -760,29 +805,37 @@ __global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const ui
         const uint64_t id = flagged[f];
         const uint64_t bx = id % nbx, by = (id / nbx) % nby, bz = id / (nbx * nby);
         const uint64_t x0 = bx * B, y0 = by * B, z0 = bz * B;
+        // halo cells: z face, else y face, else x face in 3D; x face, else y in 2D
+        float hz = 0.f, hy = 0.f, hx = 0.f;
+        if (a.halo) {
+            const float* hp = a.halo + a.halo_stride * (bx + a.hb0 * (by + a.hb1 * bz));
+            hz = hp[0];
+            hy = hp[1];
+            hx = hp[2];
+        }
         const uint64_t xe = min(x0 + B, a.d0), ye = min(y0 + B, a.d1), ze = min(z0 + B, a.d2);
         for (uint64_t z = z0; z < ze; z++)
             for (uint64_t y = y0; y < ye; y++)
                 for (uint64_t x = x0; x < xe; x++) {
                     const uint64_t idx = x + y * sy + z * sz;
-                    const bool hx = x > x0, hy = y > y0, hz = z > z0;
+                    const bool ix = x > x0, iy = y > y0, iz = z > z0;  // neighbour inside the block
                     float pred;
                     if (ndim == 1) {
-                        pred = hx ? out[idx - 1] : 0.f;
+                        pred = ix ? out[idx - 1] : hx;
                     } else if (ndim == 2) {
-                        const float W = hx ? out[idx - 1] : 0.f;
-                        const float N = hy ? out[idx - sy] : 0.f;
-                        const float NW = (hx && hy) ? out[idx - 1 - sy] : 0.f;
+                        const float W = ix ? out[idx - 1] : hx;
+                        const float N = iy ? out[idx - sy] : hy;
+                        const float NW = (ix && iy) ? out[idx - 1 - sy] : (!ix ? hx : hy);
                         pred = N + W;
                         pred = pred - NW;
                     } else {
-                        const float X = hx ? out[idx - 1] : 0.f;
-                        const float Y = hy ? out[idx - sy] : 0.f;
-                        const float Z = hz ? out[idx - sz] : 0.f;
-                        const float XY = (hx && hy) ? out[idx - 1 - sy] : 0.f;
-                        const float XZ = (hx && hz) ? out[idx - 1 - sz] : 0.f;
-                        const float YZ = (hy && hz) ? out[idx - sy - sz] : 0.f;
-                        const float XYZ = (hx && hy && hz) ? out[idx - 1 - sy - sz] : 0.f;
+                        const float X = ix ? out[idx - 1] : hx;
+                        const float Y = iy ? out[idx - sy] : hy;
+                        const float Z = iz ? out[idx - sz] : hz;
+                        const float XY = (ix && iy) ? out[idx - 1 - sy] : (!iy ? hy : hx);
+                        const float XZ = (ix && iz) ? out[idx - 1 - sz] : (!iz ? hz : hx);
+                        const float YZ = (iy && iz) ? out[idx - sy - sz] : (!iz ? hz : hy);
+                        const float XYZ = (ix && iy && iz) ? out[idx - 1 - sy - sz] : (!iz ? hz : !iy ? hy : hx);
                         pred = XYZ - XY;
                         pred = pred - XZ;
                         pred = pred - YZ;
@@ -997,6 +1050,27 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
                         uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st) {
     // four x per lane when rows are whole multiples of 4 and the output is
     // 16-byte aligned (3D keeps B x 4 running values per lane: B <= 16)
+    if (a.halo) {
+#define RC_PAD(ND, KERN, BB)                                                                                      \
+    if (ndim == ND && B == BB) {                                                                                  \
+        hipLaunchKernelGGL((KERN<BB, true>), dim3(grid), dim3(256), 0, st, codes, sparse, out, a, flagged, nflagged); \
+        return;                                                                                                   \
+    }
+        RC_PAD(3, k_recon_3d, 8)
+        RC_PAD(3, k_recon_3d, 16)
+        RC_PAD(3, k_recon_3d, 32)
+        RC_PAD(3, k_recon_3d, 64)
+        RC_PAD(2, k_recon_2d, 8)
+        RC_PAD(2, k_recon_2d, 16)
+        RC_PAD(2, k_recon_2d, 32)
+        RC_PAD(2, k_recon_2d, 64)
+        RC_PAD(1, k_recon_1d, 8)
+        RC_PAD(1, k_recon_1d, 16)
+        RC_PAD(1, k_recon_1d, 32)
+        RC_PAD(1, k_recon_1d, 64)
+#undef RC_PAD
+        return;
+    }
     const bool x4 = a.d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
                     (reinterpret_cast<uintptr_t>(codes) & 7) == 0 && (ndim < 3 || B <= 16);
     if (x4) {

@@ -39,7 +39,11 @@ def _mode_id(mode) -> int:
 
 
 def make_params(dims_xfirst: Sequence[int], eb: float, mode="abs", block: int = 8, dict_size: int = 4096,
-                value_range: float = 0.0) -> Params:
+                value_range: float = 0.0, pad_mode="none", pad_type="const", pad_constant: float = 0.0) -> Params:
+    """Compress / decompress parameters.  pad_mode 'none' | 'global' | 'block'
+    | 'edge' and pad_type 'max' | 'min' | 'avg' | 'const' select the
+    reference's padding (--padding global,max etc.); decompress reads the
+    padding from the archive."""
     p = Params()
     d = [int(v) for v in dims_xfirst]
     if not 1 <= len(d) <= 3:
@@ -52,6 +56,9 @@ def make_params(dims_xfirst: Sequence[int], eb: float, mode="abs", block: int = 
     p.value_range = float(value_range)
     p.block = int(block)
     p.dict_size = int(dict_size)
+    p.pad_mode = _lib.PAD_MODES[pad_mode] if isinstance(pad_mode, str) else int(pad_mode)
+    p.pad_type = _lib.PAD_TYPES[pad_type] if isinstance(pad_type, str) else int(pad_type)
+    p.pad_constant = float(pad_constant)
     return p
 
 
@@ -342,23 +349,24 @@ class Context:
         return outs
 
     def compress(self, x, eb: float = 1e-4, mode="abs", block: int = 8, dict_size: int = 4096,
-                 value_range: float = 0.0, out=None):
+                 value_range: float = 0.0, out=None, pad_mode="none", pad_type="const", pad_constant: float = 0.0):
         """Compress a float32 field.
 
         torch CUDA tensor -> (uint8 CUDA tensor holding the archive, n_outlier);
         numpy array       -> (bytes, n_outlier) through host staging."""
+        pad = dict(pad_mode=pad_mode, pad_type=pad_type, pad_constant=pad_constant)
         if torch is not None and isinstance(x, torch.Tensor):
             if not x.is_cuda or x.dtype != torch.float32:
                 raise TypeError("expected a float32 CUDA tensor")
             x = x.contiguous()
-            p = make_params(dims_of(x.shape), eb, mode, block, dict_size, value_range)
+            p = make_params(dims_of(x.shape), eb, mode, block, dict_size, value_range, **pad)
             cap = self.compress_bound(p)
             if out is None or out.numel() < cap:
                 out = torch.empty(cap, dtype=torch.uint8, device=x.device)
             size, nout = self.compress_into(p, x.data_ptr(), out.data_ptr(), out.numel())
             return out[:size], nout
         a = np.ascontiguousarray(x, dtype=np.float32)
-        p = make_params(dims_of(a.shape), eb, mode, block, dict_size, value_range)
+        p = make_params(dims_of(a.shape), eb, mode, block, dict_size, value_range, **pad)
         cap = self.compress_bound(p)
         buf = np.empty(cap, dtype=np.uint8)
         size = ctypes.c_uint64()
@@ -475,6 +483,24 @@ class ArgParse:
     value_range: float = 0.0
     device: int = 0
     lossless: Optional[str] = None  # szwf.lossless_pass: None, "gzip" or "zstd" (argparse.hh:32-35)
+    # szwf.{global,block,edge}_padding, pad_type (0 max, 1 min, 2 avg, 3 constant)
+    # and pad_constant (argparse.hh:34-36, :75-77)
+    global_padding: bool = False
+    block_padding: bool = False
+    edge_padding: bool = False
+    pad_type: int = 3
+    pad_constant: float = 0.0
+
+    def pad_mode(self) -> str:
+        """One padding mode at a time: global wins over block and edge, block
+        over edge (src/argument_parser/argparse.cc:156-169)."""
+        if self.global_padding:
+            return "global"
+        if self.block_padding:
+            return "block"
+        if self.edge_padding:
+            return "edge"
+        return "none"
 
 
 _default_ctx: Optional[Context] = None
@@ -497,7 +523,8 @@ def Compress(ap: ArgParse, data_in):
         x = data_in.reshape(tuple(reversed(dims)))
     else:
         x = np.asarray(data_in, dtype=np.float32).reshape(tuple(reversed(dims)))
-    arch, nout = _ctx(ap).compress(x, ap.eb, ap.mode, ap.block_size, ap.dict_size, ap.value_range)
+    arch, nout = _ctx(ap).compress(x, ap.eb, ap.mode, ap.block_size, ap.dict_size, ap.value_range,
+                                   pad_mode=ap.pad_mode(), pad_type=int(ap.pad_type), pad_constant=ap.pad_constant)
     size = arch.numel() if hasattr(arch, "numel") else len(arch)
     if ap.lossless:
         payload = arch.cpu().numpy().tobytes() if hasattr(arch, "numel") else arch
