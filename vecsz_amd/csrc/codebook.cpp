@@ -9,7 +9,8 @@
 //  - each merge removes b then a and inserts node(left=a, right=b) (:262; g++
 //    evaluates the two qremove() arguments right to left);
 //  - codes: left = 0, right = 1, MSB first (build_code, :200-227).
-// The heap keeps (freq, node) pairs inline so the sift loops touch one array.
+// The queue keeps its entries inline (one word when the sums allow) so the
+// sift loops touch one array.
 #include "codebook.hpp"
 
 #include <string.h>
@@ -21,9 +22,25 @@ namespace vsz {
 
 namespace {
 
-struct HeapEntry {
-    uint64_t f;
-    int32_t n;
+// Queue entries.  PackedQueue keeps (freq << kNodeBits | node) in one word,
+// usable when every sum of frequencies stays below 2^(64 - kNodeBits); it
+// compares the frequency bits only, as the reference compares freq alone.
+// PairQueue keeps (freq, node) pairs for larger totals.
+constexpr int kNodeBits = 17;  // nodes < 2 * 65536
+struct PackedQueue {
+    using E = uint64_t;
+    static E make(uint64_t f, int32_t n) { return (f << kNodeBits) | (uint64_t)n; }
+    static uint64_t f(E e) { return e >> kNodeBits; }
+    static int32_t node(E e) { return (int32_t)(e & ((1u << kNodeBits) - 1)); }
+};
+struct PairQueue {
+    struct E {
+        uint64_t f;
+        int32_t n;
+    };
+    static E make(uint64_t f, int32_t n) { return E{f, n}; }
+    static uint64_t f(const E& e) { return e.f; }
+    static int32_t node(const E& e) { return e.n; }
 };
 
 // Per-thread work arrays, kept between calls so a codebook build allocates
@@ -31,9 +48,77 @@ struct HeapEntry {
 struct Work {
     std::vector<int32_t> sym, kid, len, size, idx;
     std::vector<uint64_t> bits;
-    std::vector<HeapEntry> heap;
+    std::vector<PackedQueue::E> heap_packed;
+    std::vector<PairQueue::E> heap_pair;
 };
 thread_local Work t_work;
+
+template <typename Q>
+std::vector<typename Q::E>& heap_of();
+template <>
+std::vector<PackedQueue::E>& heap_of<PackedQueue>() { return t_work.heap_packed; }
+template <>
+std::vector<PairQueue::E>& heap_of<PairQueue>() { return t_work.heap_pair; }
+
+// The reference's queue (src/huffman.hh:146-192) and merge loop (:244-262):
+// leaves enter in ascending symbol order; each merge removes b, then a, and
+// inserts node(left = a, right = b) with freq a + b.
+template <typename Q>
+void merge_all(const uint64_t* freq, int dict, int nsym, int32_t* sym, int32_t* kid) {
+    using E = typename Q::E;
+    std::vector<E>& qq = heap_of<Q>();
+    qq.resize(nsym + 2);
+    int qend = 1;
+    auto qinsert = [&](E e) {
+        int i = qend++, j;
+        const uint64_t fe = Q::f(e);
+        while ((j = i >> 1)) {
+            if (Q::f(qq[j]) <= fe) break;
+            qq[i] = qq[j];
+            i = j;
+        }
+        qq[i] = e;
+    };
+    // The hole walk is the hot loop.  Its child choices are data dependent
+    // and unpredictable, so they are index arithmetic, not branches, and two
+    // levels are resolved per step: the grandchildren's comparison is made
+    // for both children before the children's comparison is known.
+    // qq[l + 1] with l + 1 == qend is still inside the array.
+    auto qremove = [&]() -> E {
+        E top = qq[1];
+        const int end = --qend;
+        int i = 1, l;
+        while (4 * i + 3 < end) {
+            const int c = 2 * i + (int)(Q::f(qq[2 * i + 1]) < Q::f(qq[2 * i]));
+            const int g0 = 4 * i + (int)(Q::f(qq[4 * i + 1]) < Q::f(qq[4 * i]));
+            const int g1 = 4 * i + 2 + (int)(Q::f(qq[4 * i + 3]) < Q::f(qq[4 * i + 2]));
+            const int g = g0 + ((g1 - g0) & -(c & 1));
+            qq[i] = qq[c];
+            qq[c] = qq[g];
+            i = g;
+        }
+        while ((l = i << 1) < end) {
+            l += (int)((l + 1 < end) & (Q::f(qq[l + 1]) < Q::f(qq[l])));
+            qq[i] = qq[l];
+            i = l;
+        }
+        qq[i] = qq[end];
+        return top;
+    };
+    for (int s = 0, n = 0; s < dict; s++) {
+        if (!freq[s]) continue;
+        sym[n] = s;
+        qinsert(Q::make(freq[s], n));
+        n++;
+    }
+    for (int n = nsym; qend > 2; n++) {
+        E b = qremove();  // first removed: right child
+        E a = qremove();  // second removed: left child
+        kid[2 * (n - nsym)] = Q::node(a);
+        kid[2 * (n - nsym) + 1] = Q::node(b);
+        qinsert(Q::make(Q::f(a) + Q::f(b), n));
+    }
+}
 
 }  // namespace
 
@@ -60,48 +145,12 @@ int build_codebook(const uint64_t* freq, int dict, Codebook& cb) {
     std::vector<int32_t>& kid = t_work.kid;  // kid[2(n - nsym)] = left, +1 = right
     sym.resize(nsym);
     kid.resize(2 * (size_t)nsym);
-    std::vector<HeapEntry>& qq = t_work.heap;
-    qq.resize(nsym + 2);
-    int qend = 1;
-
-    auto qinsert = [&](HeapEntry e) {
-        int i = qend++, j;
-        while ((j = i >> 1)) {
-            if (qq[j].f <= e.f) break;
-            qq[i] = qq[j];
-            i = j;
-        }
-        qq[i] = e;
-    };
-    // The hole walk is the hot loop: the child choice is data dependent and
-    // unpredictable, so it is computed as an index increment, not a branch.
-    // qq[l + 1] with l + 1 == qend is still inside the array.
-    auto qremove = [&]() -> HeapEntry {
-        HeapEntry top = qq[1];
-        const int end = --qend;
-        int i = 1, l;
-        while ((l = i << 1) < end) {
-            l += (int)((l + 1 < end) & (qq[l + 1].f < qq[l].f));
-            qq[i] = qq[l];
-            i = l;
-        }
-        qq[i] = qq[end];
-        return top;
-    };
-
-    for (int s = 0, n = 0; s < dict; s++) {
-        if (!freq[s]) continue;
-        sym[n] = s;
-        qinsert({freq[s], n});
-        n++;
-    }
-    for (int n = nsym; qend > 2; n++) {
-        HeapEntry b = qremove();  // first removed: right child
-        HeapEntry a = qremove();  // second removed: left child
-        kid[2 * (n - nsym)] = a.n;
-        kid[2 * (n - nsym) + 1] = b.n;
-        qinsert({a.f + b.f, n});
-    }
+    uint64_t total = 0;
+    for (int s2 = 0; s2 < dict; s2++) total += freq[s2];
+    if (total < (1ull << (64 - kNodeBits)))
+        merge_all<PackedQueue>(freq, dict, nsym, sym.data(), kid.data());
+    else
+        merge_all<PairQueue>(freq, dict, nsym, sym.data(), kid.data());
 
     // Codewords top-down in descending node order (parents before children).
     std::vector<uint64_t>& bits = t_work.bits;

@@ -677,6 +677,31 @@ __device__ __forceinline__ T wave_incl_scan(T v) {
     return v;
 }
 
+// DPP move (GFX9 encodings): lanes without a source, or in rows outside
+// ROWS, read 0.  Every lane of the wavefront must be active.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ uint32_t dpp_move(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false);
+}
+
+// Inclusive scan of 32-bit values over the wavefront in six DPP steps and no
+// LDS traffic: row_shr 1, 2, 4, 8 scan each row of 16 lanes, row_bcast:15
+// adds row 0's total to row 1 and row 2's to row 3, row_bcast:31 adds the
+// total of rows 0-1 to rows 2-3.  OR = true gives the OR-scan.
+template <bool OR = false>
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+    auto op = [](uint32_t a, uint32_t b) { return OR ? (a | b) : (a + b); };
+    v = op(v, dpp_move<0x111>(v));
+    v = op(v, dpp_move<0x112>(v));
+    v = op(v, dpp_move<0x114>(v));
+    v = op(v, dpp_move<0x118>(v));
+    v = op(v, dpp_move<0x142, 0xA>(v));
+    v = op(v, dpp_move<0x143, 0xC>(v));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane63(uint32_t v) { return (uint32_t)__builtin_amdgcn_readlane((int)v, 63); }
+
 // Exclusive scan across a 256-thread workgroup; returns the thread's prefix and
 // the workgroup total through *total.
 template <typename T>
@@ -740,13 +765,11 @@ __global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__
                 zeros += sj == 0;
             }
         }
-        for (int d = 32; d > 0; d >>= 1) {
-            bits += __shfl_xor(bits, d);
-            zeros += __shfl_xor(zeros, d);
-        }
+        // bits (< 2^17) and zeros (< 2^11) summed in one wavefront scan
+        const uint32_t tot = lane63(wave_scan_dpp(bits | (zeros << 17)));
         if (lane == 0) {
-            chunk_bits[k] = bits;
-            chunk_zeros[k] = zeros;
+            chunk_bits[k] = tot & 0x1FFFF;
+            chunk_zeros[k] = tot >> 17;
         }
     }
 }
@@ -918,9 +941,9 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
         }
         // bits (low 17) and outliers (high 15) in one wavefront scan
         const uint32_t v = tbits | (tz << 17);
-        const uint32_t incl = wave_incl_scan(v);
+        const uint32_t incl = wave_scan_dpp(v);
         const uint32_t excl = incl - v;
-        const uint32_t cbits = __shfl(incl, 63) & 0x1FFFF;
+        const uint32_t cbits = lane63(incl) & 0x1FFFF;
 
         // outlier compaction, index order
         if (tz) {
@@ -992,15 +1015,14 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
         if (e != 0 && k + 1 < nchunks) {
             Code d{0, 0, 0};
             if (cur.next != 0xFFFFFFFFu) d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
-            const uint32_t p = wave_incl_scan(d.len) - d.len;
+            const uint32_t p = wave_scan_dpp(d.len) - d.len;
             uint32_t t = 0;
             if (d.len && p < 32) {
                 const uint32_t kk = min(d.len, 32u - p);
                 const uint64_t cw = ((uint64_t)d.hi << 32) | d.lo;
                 t = (uint32_t)((cw >> (d.len - kk)) << (32 - p - kk));
             }
-#pragma unroll
-            for (int m = 32; m > 0; m >>= 1) t |= __shfl_xor(t, m);
+            t = lane63(wave_scan_dpp<true>(t));
             if (lane == 0) atomicOr(&stage[nwords - 1], t >> e);
         }
         wave_lds_sync();

@@ -462,18 +462,36 @@ __global__ void __launch_bounds__(1024) k_scan_local(const uint32_t* __restrict_
     }
 }
 
+// Exclusive scan of the per-block sums, 1024 at a time on one workgroup
+// (an LDS Hillis-Steele scan per pass, carried across passes).
 __global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsum, uint64_t nb, uint64_t* __restrict__ totals) {
-    if (threadIdx.x == 0) {
-        uint64_t ra = 0, rb = 0;
-        for (uint64_t i = 0; i < nb; i++) {
-            uint64_t x = bsum[2 * i], y = bsum[2 * i + 1];
-            bsum[2 * i] = ra;
-            bsum[2 * i + 1] = rb;
-            ra += x;
-            rb += y;
+    __shared__ uint64_t sa[1024], sb[1024];
+    const uint32_t t = threadIdx.x;
+    uint64_t ca = 0, cb = 0;
+    for (uint64_t base = 0; base < nb; base += 1024) {
+        const uint64_t i = base + t;
+        const uint64_t x = i < nb ? bsum[2 * i] : 0, y = i < nb ? bsum[2 * i + 1] : 0;
+        sa[t] = x;
+        sb[t] = y;
+        __syncthreads();
+        for (uint32_t d = 1; d < 1024; d <<= 1) {
+            const uint64_t ax = t >= d ? sa[t - d] : 0, bx = t >= d ? sb[t - d] : 0;
+            __syncthreads();
+            sa[t] += ax;
+            sb[t] += bx;
+            __syncthreads();
         }
-        totals[0] = ra;
-        totals[1] = rb;
+        if (i < nb) {
+            bsum[2 * i] = ca + sa[t] - x;
+            bsum[2 * i + 1] = cb + sb[t] - y;
+        }
+        ca += sa[1023];
+        cb += sb[1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        totals[0] = ca;
+        totals[1] = cb;
     }
 }
 
@@ -1020,7 +1038,7 @@ void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_
                        uint64_t* totals, hipStream_t st) {
     const uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
     hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(1024), 0, st, a, b, n, oa, ob, block_tmp);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(64), 0, st, block_tmp, nb, totals);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_tmp, nb, totals);
     hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 1024)), dim3(1024), 0, st, oa, ob, n, block_tmp);
 }
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
