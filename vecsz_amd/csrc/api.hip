@@ -115,7 +115,7 @@ struct vecsz_ctx {
     DevBuf vparts;
     HostBuf vpin;
     // decompress scratch
-    DevBuf tabs, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sym_off, zero_off, scan_tmp, scan_tot,
+    DevBuf tabs, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sub_list, sym_off, zero_off, scan_tmp, scan_tot,
         dec_sum, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
@@ -674,6 +674,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
         if (!c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_e2.ensure(nsub * 8) ||
             !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) || !c->sub_dirty.ensure(nsub * 2) ||
+            !c->sub_list.ensure(nsub * 4) ||
             !c->sym_off.ensure(nsub * 8) || !c->zero_off.ensure(nsub * 8) ||
             !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
             !c->dec_sum.ensure(sizeof(DecSummary)))
@@ -715,10 +716,15 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         ss.cnt = c->sub_cnt.as<uint32_t>();
         ss.zer = c->sub_zeros.as<uint32_t>();
         ss.dirty = c->sub_dirty.as<uint8_t>();
+        ss.list = c->sub_list.as<uint32_t>();
         DecSummary* dsum = c->dec_sum.as<DecSummary>();
         {
             StageScope s(c, "decode_sync");
             launch_decode_sync(da, ss, dsum, c->stream);
+        }
+        {
+            StageScope s(c, "decode_redo");
+            launch_decode_redo_list(da, ss, dsum, c->stream);
         }
         HIPCHK(c, hipGetLastError());
         DecSummary ds;

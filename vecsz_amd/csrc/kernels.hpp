@@ -44,9 +44,12 @@ struct SubState {
     uint32_t* cnt;    // codewords starting in [b, e)
     uint32_t* zer;    // of which decode to symbol 0
     uint8_t* dirty;   // needs a re-decode from e[j - 1] (2 * nsub: current and next round)
+    uint32_t* list;   // subsequences whose continuation gave up (re-decoded from b[j]), nsub entries
 };
 
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
+// Re-decodes the subsequences the sync pass listed, from their starts b[j].
+void launch_decode_redo_list(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
 void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
                         uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st);
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,

@@ -70,6 +70,17 @@ __device__ __forceinline__ uint32_t win32(const uint32_t* st, uint32_t lp) {
     const uint32_t i0 = pad_idx(lp >> 5);
     return __builtin_elementwise_fshl(st[i0], st[i0 + 1], lp & 31);
 }
+struct PadWin {  // the rows-of-33 staging above
+    const uint32_t* st;
+    __device__ __forceinline__ uint32_t operator()(uint32_t lp) const { return win32(st, lp); }
+};
+struct RowWin {  // plain consecutive words
+    const uint32_t* row;
+    __device__ __forceinline__ uint32_t operator()(uint32_t lp) const {
+        const uint32_t i0 = lp >> 5;
+        return __builtin_elementwise_fshl(row[i0], row[i0 + 1], lp & 31);
+    }
+};
 
 // The same from global memory, at stream bit p.
 struct GlobalWin {
@@ -144,14 +155,18 @@ struct Decoder<true> {
     // Decodes the codeword at staged bit lp; len = 0: invalid path.  The
     // window covers codewords of up to 32 bits; longer ones re-read.
     __device__ __forceinline__ uint32_t take_at(const uint32_t* st, uint32_t lp, uint32_t& len) const {
-        const uint32_t w = win32(st, lp);
+        return take_w(PadWin{st}, lp, len);
+    }
+    template <typename W>
+    __device__ __forceinline__ uint32_t take_w(const W& win, uint32_t lp, uint32_t& len) const {
+        const uint32_t w = win(lp);
         uint32_t e = lut[w >> (32 - kLutBits)];
         uint32_t l = e >> 12;
         if (l == 0) {
             uint32_t used = kLutBits;
             for (int lv = 0; lv < 16; lv++) {
                 const uint32_t idx = used + kSubTabBits <= 32 ? (w << used) >> (32 - kSubTabBits)
-                                                              : win32(st, lp + used) >> (32 - kSubTabBits);
+                                                              : win(lp + used) >> (32 - kSubTabBits);
                 e = sub[((e & 0xFFF) << kSubTabBits) + idx];
                 if (e >> 12) {
                     l = used + (e >> 12);
@@ -181,13 +196,17 @@ struct Decoder<false> {
         return decode_sym(w, p, lut, sub, len);
     }
     __device__ __forceinline__ uint32_t take_at(const uint32_t* st, uint32_t lp, uint32_t& len) const {
-        const uint32_t w = win32(st, lp);
+        return take_w(PadWin{st}, lp, len);
+    }
+    template <typename W>
+    __device__ __forceinline__ uint32_t take_w(const W& win, uint32_t lp, uint32_t& len) const {
+        const uint32_t w = win(lp);
         uint32_t e = lut[w >> (32 - kLutBits)];
         if (e & kLutSlow) {
             uint32_t used = kLutBits;
             for (int lv = 0; lv < 16 && (e & kLutSlow); lv++) {
                 const uint32_t idx = used + kSubTabBits <= 32 ? (w << used) >> (32 - kSubTabBits)
-                                                              : win32(st, lp + used) >> (32 - kSubTabBits);
+                                                              : win(lp + used) >> (32 - kSubTabBits);
                 e = sub[(e & 0x7FFFFFFFu) + idx];
                 used += kSubTabBits;
             }
@@ -203,6 +222,10 @@ constexpr uint32_t kStageWords = kSyncWg * (kSubBits / 32) + 12;  // + shift0 an
 constexpr uint32_t kStageLds = kStageWords + kStageWords / 32 + 1;
 constexpr int kCkStride = kCkpts + 1;                            // odd row stride: one bank per thread
 constexpr uint32_t kNoCkpt = 0xFF;                               // window without a codeword start
+// Codewords a continuation decodes before it gives up and lists the
+// subsequence for a full re-decode: most continuations meet a checkpoint
+// within a few windows, and a wavefront otherwise waits for its slowest lane.
+constexpr int kContCap = 32;
 
 // Stages NW aligned-space words starting at w0 (a multiple of 4) into the
 // padded layout.  Every thread issues all of its 16-byte loads before the
@@ -311,9 +334,13 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         const uint32_t eu = local(min((j + 2) * (uint64_t)kSubBits, a.stream_bits));
         const uint32_t q0 = own_e[t];
         uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0;
-        bool met = false;
+        bool met = false, capped = false;
         uint32_t q = q0;
         for (;;) {
+            if (cc == kContCap) {
+                capped = true;
+                break;
+            }
             if (q >= eu) {
                 if (q == own_e[u]) {
                     met = true;
@@ -339,7 +366,12 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         }
         const uint64_t base = J0 * kSubBits - sv.shift0;
         s.b[j + 1] = base + q0;
-        if (met) {
+        if (capped) {
+            // re-decoded in full from q0 by k_decode_redo_list, which then
+            // compares its end with the own end that j + 2's continuation used
+            s.e[j + 1] = base + own_e[u];
+            s.list[atomicAdd(&sum->n_redo, 1u)] = (uint32_t)(j + 1);
+        } else if (met) {
             s.cnt[j + 1] = cc + own_cnt[u] - before_c;
             s.zer[j + 1] = cz + own_z[u] - before_z;
             s.e[j + 1] = base + own_e[u];
@@ -400,6 +432,61 @@ __global__ void __launch_bounds__(256) k_decode_redo(DecodeArgs a, SubState s, c
     if (pos != end_in[j] && j + 1 < a.nsub) {
         dirty_nxt[j + 1] = 1;
         atomicAdd(&sum->n_changed, 1u);
+    }
+}
+
+// Full re-decode of each listed subsequence from its start b[j] (set by the
+// sync pass): codeword and zero counts, and its end.  An end other than the
+// one j + 1's continuation started from marks j + 1 for the redo rounds.
+// A workgroup takes 256 list entries; their words (the subsequence plus the
+// tail its last codeword may read) are staged in LDS with coalesced loads,
+// one row per entry, then each thread decodes its row.
+constexpr uint32_t kRowWords = 38;   // 32 words + shift0 + the last codeword and window reads
+constexpr uint32_t kRowStride = 39;  // odd: one bank per thread
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_redo_list(DecodeArgs a, SubState s, DecSummary* sum) {
+    const uint32_t nr = sum->n_redo;
+    const uint64_t e0 = blockIdx.x * 256ull;
+    if (e0 >= nr) return;  // whole workgroup
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    uint32_t* rows = lds + dec.load(lds, a);
+    StreamView sv;
+    sv.setup(a.stream, a.stream_bits);
+    const uint32_t ne = (uint32_t)min<uint64_t>(256, nr - e0);
+    for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
+        const uint32_t e = p / kRowWords, w = p % kRowWords;
+        const uint64_t j = s.list[e0 + e];
+        rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
+    }
+    __syncthreads();
+    if (threadIdx.x >= ne) return;
+    const uint64_t j = s.list[e0 + threadIdx.x];
+    const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of the row's word 0
+    const RowWin win{rows + threadIdx.x * kRowStride};
+    const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
+    const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
+    uint32_t lp = (uint32_t)(s.b[j] + sv.shift0 - rowbit);
+    uint32_t cnt = 0, z = 0;
+    bool bad = false;
+    while (lp < tl) {
+        uint32_t len;
+        const uint32_t sym = dec.take_w(win, lp, len);
+        bad |= len == 0;
+        lp += len ? len : 1;
+        cnt++;
+        z += sym == 0;
+    }
+    if (bad) atomicOr(&sum->error, 1u);
+    const uint64_t pos = rowbit + lp - sv.shift0;
+    s.cnt[j] = cnt;
+    s.zer[j] = z;
+    if (pos != s.e[j]) {
+        s.e[j] = pos;
+        if (j + 1 < a.nsub) {
+            s.dirty[j + 1] = 1;
+            atomicAdd(&sum->n_changed, 1u);
+        }
     }
 }
 
@@ -1033,6 +1120,18 @@ void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in,
     else
         hipLaunchKernelGGL(k_decode_redo<false>, grid, dim3(256), lds, st, a, s, end_in, end_out, dirty_cur, dirty_nxt,
                            sum);
+}
+void launch_decode_redo_list(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
+    allow_big_lds(k_decode_redo_list<true>);
+    allow_big_lds(k_decode_redo_list<false>);
+    const bool t16 = a.lut16 != nullptr;
+    const size_t lds = table_lds_bytes(a, t16) + (size_t)256 * kRowStride * 4;
+    // one thread per possible entry; threads past the listed count return at once
+    const dim3 grid(grid_for(a.nsub, 256));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_redo_list<true>, grid, dim3(256), lds, st, a, s, sum);
+    else
+        hipLaunchKernelGGL(k_decode_redo_list<false>, grid, dim3(256), lds, st, a, s, sum);
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {
