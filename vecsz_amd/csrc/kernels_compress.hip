@@ -852,13 +852,14 @@ struct ChunkRaw {
     uint32_t w[8];
     uint32_t nvalid;
     uint32_t next;  // 0xFFFFFFFF: none
-    uint64_t bit_off, out_off;
+    uint64_t bit_off, out_off, nz;  // nz: outliers in the chunk
 };
 
+// Symbols past the field read as `fill` (an LDS table slot that holds 0).
 __device__ __forceinline__ void load_chunk(const uint16_t* __restrict__ codes, uint64_t n, uint64_t nchunks,
                                            const uint64_t* __restrict__ bit_off,
-                                           const uint64_t* __restrict__ out_off, uint64_t k, uint32_t lane,
-                                           ChunkRaw& r) {
+                                           const uint64_t* __restrict__ out_off, uint64_t n_outlier, uint64_t k,
+                                           uint32_t lane, uint32_t fill, ChunkRaw& r) {
     const uint64_t base = k * kChunk + lane * kSymPerThread;
     r.nvalid = base >= n ? 0u : (uint32_t)min<uint64_t>(kSymPerThread, n - base);
     if (r.nvalid == kSymPerThread) {
@@ -871,15 +872,26 @@ __device__ __forceinline__ void load_chunk(const uint16_t* __restrict__ codes, u
         for (int j = 0; j < 8; j++) r.w[j] = 0;
 #pragma unroll
         for (int j = 0; j < kSymPerThread; j++)
-            if ((uint32_t)j < r.nvalid) r.w[j >> 1] |= (uint32_t)codes[base + j] << (16 * (j & 1));
+            r.w[j >> 1] |= ((uint32_t)j < r.nvalid ? (uint32_t)codes[base + j] : fill) << (16 * (j & 1));
     }
     const uint64_t i = (k + 1) * kChunk + lane;
     r.next = (lane < 32 && k + 1 < nchunks && i < n) ? (uint32_t)codes[i] : 0xFFFFFFFFu;
     r.bit_off = bit_off[k];
     r.out_off = out_off[k];
+    r.nz = (k + 1 < nchunks ? out_off[k + 1] : n_outlier) - r.out_off;
 }
 
-template <bool LDS_BOOK>
+// Codebook layouts of k_pack: the global tables; LDS entries (codeword << 6)
+// | length with kLongCode for codewords over 26 bits; or, when no codeword
+// exceeds 32 bits, an LDS codeword array and an LDS length array.  The LDS
+// forms hold one more slot, `dict`, that is 0: symbols past the field.
+enum { kBookGlobal = 0, kBookPacked = 1, kBookSplit = 2 };
+
+__host__ __device__ constexpr uint32_t book_words(int book, int dict) {
+    return book == kBookPacked ? (uint32_t)dict + 1 : book == kBookSplit ? (uint32_t)dict + 1 + ((uint32_t)dict + 4) / 4 : 0;
+}
+
+template <int BOOK>
 __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes, uint64_t n,
                                               const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
                                               const uint8_t* __restrict__ len_tab, int dict,
@@ -887,17 +899,25 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
                                               uint64_t nchunks, const EncSummary* __restrict__ sum,
                                               const float* __restrict__ sparse, uint8_t* __restrict__ archive,
                                               const uint32_t* __restrict__ err, uint32_t stage_words) {
+    constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     if (*err) return;
     extern __shared__ uint32_t lds[];
-    uint32_t* tab = lds;
-    if (LDS_BOOK)
-        for (int i = threadIdx.x; i < dict; i += blockDim.x) {
-            const uint32_t l = len_tab[i];
-            tab[i] = l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
+    uint32_t* tab = lds;                                                        // packed entries, or split codewords
+    uint8_t* tlen = reinterpret_cast<uint8_t*>(lds + (uint32_t)dict + 1);       // split lengths
+    if (BOOK == kBookPacked) {
+        for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+            const uint32_t l = i < dict ? len_tab[i] : 0u;
+            tab[i] = l == 0 ? 0u : l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
         }
+    } else if (BOOK == kBookSplit) {
+        for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+            tab[i] = i < dict ? cw_lo[i] : 0u;
+            tlen[i] = i < dict ? len_tab[i] : 0u;
+        }
+    }
     __syncthreads();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint32_t* stage = lds + (LDS_BOOK ? dict : 0) + wave * stage_words;
+    uint32_t* stage = lds + book_words(BOOK, dict) + wave * stage_words;
     const EncSummary S = *sum;
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint32_t lane = lane_id();
@@ -906,34 +926,44 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     if (k >= nchunks) return;  // no workgroup barrier follows
 
     // software pipeline: the next chunk's loads are in flight while this one is packed
+    const uint32_t fill = LDS_BOOK ? (uint32_t)dict : 0u;
     ChunkRaw nxt;
-    load_chunk(codes, n, nchunks, bit_off, out_off, k, lane, nxt);
+    load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k, lane, fill, nxt);
     for (; k < nchunks; k += nw) {
         const ChunkRaw cur = nxt;
-        if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, k + nw, lane, nxt);
+        if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k + nw, lane, fill, nxt);
         const uint64_t base = k * kChunk + lane * kSymPerThread;
 
-        // LDS_BOOK keeps the raw table entry (one register per symbol)
+        // the LDS books keep the raw entries (one or two registers per symbol)
         uint32_t ent[LDS_BOOK ? kSymPerThread : 1];
+        uint32_t elen[BOOK == kBookSplit ? kSymPerThread : 1];
         Code c[LDS_BOOK ? 1 : kSymPerThread];
         uint32_t tbits = 0, tz = 0, long_mask = 0;
 #pragma unroll
         for (int j = 0; j < kSymPerThread; j++) {
             const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-            const bool valid = (uint32_t)j < cur.nvalid;
-            if constexpr (LDS_BOOK) {
-                ent[j] = valid ? tab[sj] : 0u;
+            if constexpr (BOOK == kBookPacked) {
+                ent[j] = tab[sj];
                 const uint32_t l = ent[j] & 63;
                 tbits += l;
                 long_mask |= (uint32_t)(l == kLongCode) << j;
+            } else if constexpr (BOOK == kBookSplit) {
+                ent[j] = tab[sj];
+                elen[j] = tlen[sj];
+                tbits += elen[j];
             } else {
+                const bool valid = (uint32_t)j < cur.nvalid;
                 c[j] = Code{0, 0, 0};
                 if (valid) c[j] = lookup_global(sj, cw_hi, cw_lo, len_tab);
                 tbits += c[j].len;
             }
-            tz += valid && sj == 0;
         }
-        if (LDS_BOOK && long_mask) {
+        if (cur.nz) {  // outliers are rare: count them per lane only in chunks that hold some
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++)
+                tz += (uint32_t)j < cur.nvalid && ((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF) == 0;
+        }
+        if (BOOK == kBookPacked && long_mask) {
             // codewords of more than 26 bits: true lengths from the global table
 #pragma unroll
             for (int j = 0; j < kSymPerThread; j++)
@@ -991,7 +1021,10 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
                 place(c.lo, c.len);
             }
         };
-        if constexpr (LDS_BOOK) {
+        if constexpr (BOOK == kBookSplit) {
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) place(ent[j], elen[j]);  // absent symbols: length 0
+        } else if constexpr (BOOK == kBookPacked) {
             if (!__any(long_mask != 0)) {
 #pragma unroll
                 for (int j = 0; j < kSymPerThread; j++) place(ent[j] >> 6, ent[j] & 63);  // absent symbols: ent 0
@@ -1014,7 +1047,12 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
         const uint32_t e = (a0 + cbits) & 31;
         if (e != 0 && k + 1 < nchunks) {
             Code d{0, 0, 0};
-            if (cur.next != 0xFFFFFFFFu) d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
+            if (cur.next != 0xFFFFFFFFu) {
+                if constexpr (BOOK == kBookSplit)
+                    d = Code{tab[cur.next], 0, tlen[cur.next]};
+                else
+                    d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
+            }
             const uint32_t p = wave_scan_dpp(d.len) - d.len;
             uint32_t t = 0;
             if (d.len && p < 32) {
@@ -1161,14 +1199,16 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
     // words one chunk can touch: 1024 codewords of at most max_len bits, plus
     // the partial words at either end
     const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
-    const bool lds_book = dict <= kMaxLdsBook;
-    const size_t lds = ((lds_book ? (size_t)dict : 0) + 4 * (size_t)stage_words) * 4;
-    if (lds_book)
-        hipLaunchKernelGGL(k_pack<true>, dim3(grid), dim3(256), lds, st, codes, n, cw_hi, cw_lo, len_tab, dict, bit_off,
-                           out_off, nchunks, sum, sparse, archive, err, stage_words);
+    const int book = dict > kMaxLdsBook ? kBookGlobal : max_len <= 32 ? kBookSplit : kBookPacked;
+    const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;
+#define PACK_ARGS codes, n, cw_hi, cw_lo, len_tab, dict, bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words
+    if (book == kBookSplit)
+        hipLaunchKernelGGL(k_pack<kBookSplit>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
+    else if (book == kBookPacked)
+        hipLaunchKernelGGL(k_pack<kBookPacked>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
     else
-        hipLaunchKernelGGL(k_pack<false>, dim3(grid), dim3(256), lds, st, codes, n, cw_hi, cw_lo, len_tab, dict,
-                           bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words);
+        hipLaunchKernelGGL(k_pack<kBookGlobal>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
+#undef PACK_ARGS
 }
 
 }  // namespace vsz
