@@ -598,8 +598,14 @@ __device__ __forceinline__ float read_f32_bytes(const uint8_t* p) {
 }
 
 // Final decode.  Thread j decodes cnt[j] codewords from its true start b[j]
-// out of the workgroup's staged bits; its symbols [O, O + cnt) go out as
-// single u16 stores up to an 8-symbol boundary, then 16-byte stores.
+// out of the workgroup's staged bits: its symbols are [O, E), O = sym_off[j].
+// Output goes out in whole 16-byte units of 8 symbols: the thread whose range
+// holds a unit's first symbol writes the unit, decoding on past E into the
+// following subsequences where the unit ends beyond it (the decode is on the
+// true path there), and skips the symbols before its first unit (written by
+// a predecessor).  Only the field's last unit may be partial.
+constexpr uint32_t kWriteStageWords = kSyncWg * (kSubBits / 32) + 32;  // + up to 7 codewords past the last range
+constexpr uint32_t kWriteStageLds = kWriteStageWords + kWriteStageWords / 32 + 3;
 template <bool T16>
 __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, const uint64_t* __restrict__ sym_off,
                                                       const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
@@ -610,13 +616,13 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     const uint64_t J0 = blockIdx.x * (uint64_t)kSyncWg;
-    stage_words<kStageWords>(st, sv, J0 * (kSubBits / 32));
+    stage_words<kWriteStageWords>(st, sv, J0 * (kSubBits / 32));
     __syncthreads();
     const uint64_t j = J0 + threadIdx.x;
     if (j >= a.nsub) return;
     uint32_t lp = (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0;
     const uint64_t O = sym_off[j];
-    const uint64_t iend = min(O + s.cnt[j], a.n);
+    const uint64_t E = min(O + s.cnt[j], a.n);
     uint64_t rank = zero_off[j];
     bool bad = false;
     auto next = [&]() -> uint32_t {
@@ -633,28 +639,29 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
             bad = true;
         rank++;
     };
+    const uint64_t W0 = (O + 7) & ~7ull;  // this thread's first unit
     uint64_t i = O;
-    for (; i < iend && (i & 7); i++) {
-        const uint32_t sym = next();
-        codes[i] = (uint16_t)sym;
-        if (sym == 0) outlier(i);
-    }
-    for (; i + 8 <= iend; i += 8) {
-        uint32_t w[4];
+    for (; i < E && i < W0; i++)          // a predecessor's unit: decode, count zeros, skip
+        rank += next() == 0;
+    if (W0 < E) {
+        const uint64_t W1 = min((E + 7) & ~7ull, a.n);
+        for (; i + 8 <= W1; i += 8) {
+            uint32_t w[4];
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
-            const uint32_t lo = next();
-            if (lo == 0) outlier(i + 2 * m);
-            const uint32_t hi = next();
-            if (hi == 0) outlier(i + 2 * m + 1);
-            w[m] = lo | (hi << 16);
+            for (int m = 0; m < 4; m++) {
+                const uint32_t lo = next();
+                if (lo == 0) outlier(i + 2 * m);
+                const uint32_t hi = next();
+                if (hi == 0) outlier(i + 2 * m + 1);
+                w[m] = lo | (hi << 16);
+            }
+            *reinterpret_cast<uint4*>(codes + i) = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        *reinterpret_cast<uint4*>(codes + i) = make_uint4(w[0], w[1], w[2], w[3]);
-    }
-    for (; i < iend; i++) {
-        const uint32_t sym = next();
-        codes[i] = (uint16_t)sym;
-        if (sym == 0) outlier(i);
+        for (; i < W1; i++) {  // the field's last, partial unit
+            const uint32_t sym = next();
+            codes[i] = (uint16_t)sym;
+            if (sym == 0) outlier(i);
+        }
     }
     if (bad) atomicOr(&sum->error, 1u);
 }
@@ -1145,7 +1152,7 @@ void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_of
     allow_big_lds(k_decode_write<true>);
     allow_big_lds(k_decode_write<false>);
     const bool t16 = a.lut16 != nullptr;
-    const size_t lds = table_lds_bytes(a, t16) + (size_t)kStageLds * 4;
+    const size_t lds = table_lds_bytes(a, t16) + (size_t)kWriteStageLds * 4;
     const dim3 grid(grid_for(a.nsub, kSyncWg));
     if (t16)
         hipLaunchKernelGGL(k_decode_write<true>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
