@@ -115,7 +115,7 @@ struct vecsz_ctx {
     DevBuf vparts;
     HostBuf vpin;
     // decompress scratch
-    DevBuf tabs, sub_b, sub_e, sub_e2, sub_cnt, sub_zeros, sub_dirty, sub_list, sym_off, zero_off, scan_tmp, scan_tot,
+    DevBuf tabs, sub_b, sub_e, sub_cnt, sub_zeros, sub_lj, sub_ls, sub_ln, sym_off, zero_off, scan_tmp, scan_tot,
         dec_sum, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
@@ -532,53 +532,6 @@ int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint
     return VECSZ_OK;
 }
 
-// Serial decode on the host of a stream whose chunk-parallel decode did not
-// synchronise (e.g. fixed-length codes): the true start, count and zero count
-// of every subsequence.
-// Serial decode on the host: the last resort when the device rounds have
-// not converged.  Fills the true start, end and counts of every subsequence.
-int host_resync(vecsz_ctx* c, const DecodeTables& dt, const uint8_t* d_stream, uint64_t stream_bytes, uint64_t n,
-                uint64_t nsub) {
-    std::vector<uint8_t> st(stream_bytes + 16, 0);
-    HIPCHK(c, hipMemcpyAsync(st.data(), d_stream, stream_bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<uint64_t> b(nsub, 0);
-    std::vector<uint32_t> cnt(nsub, 0), zeros(nsub, 0);
-    const uint64_t T = stream_bytes * 8;
-    uint64_t pos = 0;
-    auto bit = [&](uint64_t i) -> uint32_t { return (st[i >> 3] >> (7 - (i & 7))) & 1; };
-    uint64_t cur = 0;
-    b[0] = 0;
-    while (pos < T) {
-        while (cur + 1 < nsub && pos >= (cur + 1) * (uint64_t)kSubBits) {
-            cur++;
-            b[cur] = pos;
-        }
-        uint32_t node = 0;
-        uint32_t sym = 0;
-        for (;;) {
-            if (pos >= T + 64) return fail(c, VECSZ_ECORRUPT, "bitstream truncated");
-            uint32_t nx = dt.child[2 * node + (pos < T ? bit(pos) : 0)];
-            pos++;
-            if (nx == kNodeAbsent) return fail(c, VECSZ_ECORRUPT, "invalid code path");
-            if (nx & kNodeLeaf) {
-                sym = nx & 0xFFFF;
-                break;
-            }
-            node = nx;
-        }
-        cnt[cur]++;
-        zeros[cur] += sym == 0;
-    }
-    (void)n;
-    for (uint64_t j = cur + 1; j < nsub; j++) b[j] = pos;
-    HIPCHK(c, hipMemcpyAsync(c->sub_b.p, b.data(), nsub * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->sub_cnt.p, cnt.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->sub_zeros.p, zeros.data(), nsub * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VECSZ_OK;
-}
-
 bool tree_is_full(const DecodeTables& dt) {
     // every internal node reachable from the root has both children
     std::vector<uint32_t> stk{0};
@@ -672,9 +625,9 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     } else {
         const uint64_t T = stream_bytes * 8;
         const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
-        if (!c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_e2.ensure(nsub * 8) ||
-            !c->sub_cnt.ensure(nsub * 4) || !c->sub_zeros.ensure(nsub * 4) || !c->sub_dirty.ensure(nsub * 2) ||
-            !c->sub_list.ensure(nsub * 4) ||
+        if (!c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_cnt.ensure(nsub * 4) ||
+            !c->sub_zeros.ensure(nsub * 4) || !c->sub_lj.ensure(nsub * 8) || !c->sub_ls.ensure(nsub * 16) ||
+            !c->sub_ln.ensure(16) ||
             !c->sym_off.ensure(nsub * 8) || !c->zero_off.ensure(nsub * 8) ||
             !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
             !c->dec_sum.ensure(sizeof(DecSummary)))
@@ -697,7 +650,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         }
         HIPCHK(c, hipMemcpyAsync(c->tabs.p, tp, tab_bytes, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemsetAsync(c->dec_sum.p, 0, sizeof(DecSummary), c->stream));
-        HIPCHK(c, hipMemsetAsync(c->sub_dirty.p, 0, nsub * 2, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->sub_ln.p, 0, 16, c->stream));
         DecodeArgs da;
         da.stream = d_stream;
         da.stream_bits = T;
@@ -715,45 +668,39 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         ss.e = c->sub_e.as<uint64_t>();
         ss.cnt = c->sub_cnt.as<uint32_t>();
         ss.zer = c->sub_zeros.as<uint32_t>();
-        ss.dirty = c->sub_dirty.as<uint8_t>();
-        ss.list = c->sub_list.as<uint32_t>();
+        ss.list_j[0] = c->sub_lj.as<uint32_t>();
+        ss.list_j[1] = c->sub_lj.as<uint32_t>() + nsub;
+        ss.list_start[0] = c->sub_ls.as<uint64_t>();
+        ss.list_start[1] = c->sub_ls.as<uint64_t>() + nsub;
+        ss.list_n = c->sub_ln.as<uint32_t>();
         DecSummary* dsum = c->dec_sum.as<DecSummary>();
         {
             StageScope s(c, "decode_sync");
             launch_decode_sync(da, ss, dsum, c->stream);
         }
+        // Re-decode passes, two without a host check (most streams need one or
+        // none), then as many as the lists ask for: each pass resolves at
+        // least the first subsequence of every chain, so nsub passes suffice.
+        constexpr uint32_t kPassesAhead = 2;
+        uint32_t pass = 0;
         {
             StageScope s(c, "decode_redo");
-            launch_decode_redo_list(da, ss, dsum, c->stream);
+            for (; pass < kPassesAhead; pass++) launch_decode_pass(da, ss, pass, dsum, c->stream);
         }
         HIPCHK(c, hipGetLastError());
         DecSummary ds;
-        HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        memcpy(&ds, h, sizeof(ds));
-        if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
-        uint64_t* e_in = ss.e;
-        uint64_t* e_out = c->sub_e2.as<uint64_t>();
-        uint8_t* dcur = ss.dirty;
-        uint8_t* dnxt = ss.dirty + nsub;
-        int rounds = 0;
-        while (ds.n_changed > 0 && rounds < 32) {
-            HIPCHK(c, hipMemsetAsync(&dsum->n_changed, 0, 4, c->stream));
-            {
-                StageScope s(c, "decode_redo");
-                launch_decode_redo(da, ss, e_in, e_out, dcur, dnxt, dsum, c->stream);
-            }
+        uint32_t left = 0;
+        for (;;) {
             HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(h + 64, ss.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             memcpy(&ds, h, sizeof(ds));
+            memcpy(&left, h + 64, 4);
             if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
-            std::swap(e_in, e_out);
-            std::swap(dcur, dnxt);
-            rounds++;
-        }
-        if (ds.n_changed > 0) {
-            rc = host_resync(c, dt, d_stream, stream_bytes, n, nsub);
-            if (rc) return rc;
+            if (left == 0) break;
+            if (pass > nsub + 2) return fail(c, VECSZ_ECORRUPT, "subsequence boundaries do not converge");
+            StageScope s(c, "decode_redo");
+            launch_decode_pass(da, ss, pass++, dsum, c->stream);
         }
         {
             StageScope s(c, "decode_scan");
@@ -1082,9 +1029,9 @@ int vecsz_decompress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, con
         (void)hipSetDevice(w->device);
         const uint64_t nsub = bmax * 8 / kSubBits + 2;
         return w->codes.ensure(nmax * 2 + 64) && w->sparse.ensure(nmax * 4 + 64) && w->sub_b.ensure(nsub * 8) &&
-               w->sub_e.ensure(nsub * 8) && w->sub_e2.ensure(nsub * 8) && w->sub_cnt.ensure(nsub * 4) &&
-               w->sub_zeros.ensure(nsub * 4) && w->sub_dirty.ensure(nsub * 2) && w->sym_off.ensure(nsub * 8) &&
-               w->zero_off.ensure(nsub * 8);
+               w->sub_e.ensure(nsub * 8) && w->sub_cnt.ensure(nsub * 4) && w->sub_zeros.ensure(nsub * 4) &&
+               w->sub_lj.ensure(nsub * 8) && w->sub_ls.ensure(nsub * 16) && w->sub_ln.ensure(16) &&
+               w->sym_off.ensure(nsub * 8) && w->zero_off.ensure(nsub * 8);
     };
     return run_batch(c, nfields, weight.data(), status, prep, [&](vecsz_ctx* w, int k) {
         if (!d_in[k] || !d_out[k]) return fail(w, VECSZ_EINVAL, "NULL field");

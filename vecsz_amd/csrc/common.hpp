@@ -74,8 +74,6 @@ struct EncSummary {
 // Decoder summary.
 struct DecSummary {
     uint32_t error;       // nonzero: corrupt stream
-    uint32_t n_changed;   // sync rounds: subsequences whose end moved
-    uint32_t n_redo;      // subsequences listed for a full re-decode by the sync pass
     uint32_t pad_;
     uint64_t total_syms;  // symbols found by the sync pass
 };

@@ -43,15 +43,17 @@ struct SubState {
     uint64_t* e;      // its end: first codeword start >= (j + 1) * kSubBits
     uint32_t* cnt;    // codewords starting in [b, e)
     uint32_t* zer;    // of which decode to symbol 0
-    uint8_t* dirty;   // needs a re-decode from e[j - 1] (2 * nsub: current and next round)
-    uint32_t* list;   // subsequences whose continuation gave up (re-decoded from b[j]), nsub entries
+    // Re-decode lists, used in alternation by successive passes: subsequence
+    // list_j[.][k] to re-decode from bit list_start[.][k] (nsub entries each),
+    // and entry counts in a ring of three (pass p reads list_n[p % 3]).
+    uint32_t* list_j[2];
+    uint64_t* list_start[2];
+    uint32_t* list_n;
 };
 
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
-// Re-decodes the subsequences the sync pass listed, from their starts b[j].
-void launch_decode_redo_list(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
-void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
-                        uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st);
+// Re-decode pass `pass` over the list the previous pass (or the sync pass) built.
+void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSummary* sum, hipStream_t st);
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st);
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,

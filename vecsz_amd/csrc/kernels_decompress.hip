@@ -11,9 +11,11 @@
 //                     subsequence until a codeword start meets a checkpoint,
 //                     after which both decodes agree, and writes the next
 //                     subsequence's true start, end and counts
-//     k_decode_redo   subsequences whose predecessor's continuation found no
-//                     meeting point re-decode from the true start, in rounds,
-//                     until no end moves
+//                     (after 32 codewords without a meeting point it gives
+//                     up and lists the subsequence)
+//     k_decode_pass   listed subsequences re-decode in full from their true
+//                     starts, from LDS-staged rows; a moved end lists the
+//                     successor for the next pass, until no list is left
 //     k_decode_write  final decode from the true starts: quant codes in
 //                     16-byte stores + outlier scatter
 //                     (src/vecsz_interface.hh:313-317)
@@ -367,10 +369,12 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         const uint64_t base = J0 * kSubBits - sv.shift0;
         s.b[j + 1] = base + q0;
         if (capped) {
-            // re-decoded in full from q0 by k_decode_redo_list, which then
-            // compares its end with the own end that j + 2's continuation used
+            // re-decoded in full from q0 by the first pass, which then compares
+            // its end with the own end that j + 2's continuation started from
             s.e[j + 1] = base + own_e[u];
-            s.list[atomicAdd(&sum->n_redo, 1u)] = (uint32_t)(j + 1);
+            const uint32_t k = atomicAdd(&s.list_n[0], 1u);
+            s.list_j[0][k] = (uint32_t)(j + 1);
+            s.list_start[0][k] = base + q0;
         } else if (met) {
             s.cnt[j + 1] = cc + own_cnt[u] - before_c;
             s.zer[j + 1] = cz + own_z[u] - before_z;
@@ -382,112 +386,82 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
             s.zer[j + 1] = cz;
             s.e[j + 1] = base + q;
             if (j + 2 < a.nsub) {
-                s.dirty[j + 2] = 1;
-                atomicAdd(&sum->n_changed, 1u);
+                const uint32_t k = atomicAdd(&s.list_n[0], 1u);
+                s.list_j[0][k] = (uint32_t)(j + 2);
+                s.list_start[0][k] = base + q;
             }
         }
     }
     if (bad) atomicOr(&sum->error, 1u);
 }
 
-// One round of re-decoding from the previous subsequence's true end:
-// dirty_cur -> dirty_nxt, end_in -> end_out.
-template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_redo(DecodeArgs a, SubState s, const uint64_t* end_in, uint64_t* end_out,
-                                                     uint8_t* dirty_cur, uint8_t* dirty_nxt, DecSummary* sum) {
-    extern __shared__ uint32_t lds[];
-    Decoder<T16> dec;
-    dec.load(lds, a);
-    __syncthreads();
-    const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (j >= a.nsub) return;
-    if (!dirty_cur[j]) {
-        end_out[j] = end_in[j];
-        return;
-    }
-    dirty_cur[j] = 0;
-    GlobalWin win;
-    win.sv.setup(a.stream, a.stream_bits);
-    const uint64_t st = end_in[j - 1];
-    const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
-    uint64_t pos = st;
-    uint32_t cnt = 0, z = 0;
-    bool bad = false;
-    while (pos < target) {
-        uint32_t len;
-        const uint32_t sym = dec(win, pos, len);
-        if (len == 0) {
-            bad = true;
-            len = 1;
-        }
-        z += sym == 0;
-        pos += len;
-        cnt++;
-    }
-    if (bad) atomicOr(&sum->error, 1u);
-    s.b[j] = st;
-    s.cnt[j] = cnt;
-    s.zer[j] = z;
-    end_out[j] = pos;
-    if (pos != end_in[j] && j + 1 < a.nsub) {
-        dirty_nxt[j + 1] = 1;
-        atomicAdd(&sum->n_changed, 1u);
-    }
-}
-
-// Full re-decode of each listed subsequence from its start b[j] (set by the
-// sync pass): codeword and zero counts, and its end.  An end other than the
-// one j + 1's continuation started from marks j + 1 for the redo rounds.
-// A workgroup takes 256 list entries; their words (the subsequence plus the
-// tail its last codeword may read) are staged in LDS with coalesced loads,
-// one row per entry, then each thread decodes its row.
+// Re-decode pass: each listed subsequence j is decoded in full from its
+// listed start (the true end of j - 1); this gives its start, counts and
+// end.  An end other than the one j + 1's data was computed from lists j + 1,
+// from that end, for the next pass.  Chains shorten by at least one per
+// pass.  Workgroups take 256 entries at a time; their words (the
+// subsequence plus the tail its last codeword may read) are staged in LDS
+// with coalesced loads, one row per entry, then each thread decodes its row.
 constexpr uint32_t kRowWords = 38;   // 32 words + shift0 + the last codeword and window reads
 constexpr uint32_t kRowStride = 39;  // odd: one bank per thread
+constexpr uint32_t kPassGrid = 512;  // persistent workgroups of a pass
 template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_redo_list(DecodeArgs a, SubState s, DecSummary* sum) {
-    const uint32_t nr = sum->n_redo;
-    const uint64_t e0 = blockIdx.x * 256ull;
-    if (e0 >= nr) return;  // whole workgroup
+__global__ void __launch_bounds__(256) k_decode_pass(DecodeArgs a, SubState s, uint32_t pass, DecSummary* sum) {
+    const uint32_t m = s.list_n[pass % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.list_n[(pass + 2) % 3] = 0;  // the list after next
+    if ((uint64_t)blockIdx.x * 256 >= m) return;  // whole workgroup
+    const uint32_t* lj = s.list_j[pass & 1];
+    const uint64_t* ls = s.list_start[pass & 1];
+    uint32_t* oj = s.list_j[(pass + 1) & 1];
+    uint64_t* os = s.list_start[(pass + 1) & 1];
+    uint32_t* on = &s.list_n[(pass + 1) % 3];
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     uint32_t* rows = lds + dec.load(lds, a);
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
-    const uint32_t ne = (uint32_t)min<uint64_t>(256, nr - e0);
-    for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
-        const uint32_t e = p / kRowWords, w = p % kRowWords;
-        const uint64_t j = s.list[e0 + e];
-        rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
-    }
-    __syncthreads();
-    if (threadIdx.x >= ne) return;
-    const uint64_t j = s.list[e0 + threadIdx.x];
-    const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of the row's word 0
-    const RowWin win{rows + threadIdx.x * kRowStride};
-    const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
-    const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
-    uint32_t lp = (uint32_t)(s.b[j] + sv.shift0 - rowbit);
-    uint32_t cnt = 0, z = 0;
     bool bad = false;
-    while (lp < tl) {
-        uint32_t len;
-        const uint32_t sym = dec.take_w(win, lp, len);
-        bad |= len == 0;
-        lp += len ? len : 1;
-        cnt++;
-        z += sym == 0;
-    }
-    if (bad) atomicOr(&sum->error, 1u);
-    const uint64_t pos = rowbit + lp - sv.shift0;
-    s.cnt[j] = cnt;
-    s.zer[j] = z;
-    if (pos != s.e[j]) {
-        s.e[j] = pos;
-        if (j + 1 < a.nsub) {
-            s.dirty[j + 1] = 1;
-            atomicAdd(&sum->n_changed, 1u);
+    for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256) {
+        const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
+        __syncthreads();  // the tables (first time) / the previous rows' readers
+        for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
+            const uint32_t e = p / kRowWords, w = p % kRowWords;
+            const uint64_t j = lj[e0 + e];
+            rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
+        }
+        __syncthreads();
+        if (threadIdx.x < ne) {
+            const uint64_t j = lj[e0 + threadIdx.x];
+            const uint64_t start = ls[e0 + threadIdx.x];
+            const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
+            const RowWin win{rows + threadIdx.x * kRowStride};
+            const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
+            const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
+            uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
+            uint32_t cnt = 0, z = 0;
+            while (lp < tl) {
+                uint32_t len;
+                const uint32_t sym = dec.take_w(win, lp, len);
+                bad |= len == 0;
+                lp += len ? len : 1;
+                cnt++;
+                z += sym == 0;
+            }
+            const uint64_t pos = rowbit + lp - sv.shift0;
+            s.b[j] = start;
+            s.cnt[j] = cnt;
+            s.zer[j] = z;
+            if (pos != s.e[j]) {
+                s.e[j] = pos;
+                if (j + 1 < a.nsub) {
+                    const uint32_t k = atomicAdd(on, 1u);
+                    oj[k] = (uint32_t)(j + 1);
+                    os[k] = pos;
+                }
+            }
         }
     }
+    if (bad) atomicOr(&sum->error, 1u);
 }
 
 // Device-wide exclusive scan of two u32 arrays into u64 offsets.
@@ -1116,29 +1090,17 @@ void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStr
     else
         hipLaunchKernelGGL(k_decode_sync<false>, grid, dim3(kSyncWg), lds, st, a, s, sum);
 }
-void launch_decode_redo(const DecodeArgs& a, SubState s, const uint64_t* end_in, uint64_t* end_out, uint8_t* dirty_cur,
-                        uint8_t* dirty_nxt, DecSummary* sum, hipStream_t st) {
-    const bool t16 = a.lut16 != nullptr;
-    const size_t lds = table_lds_bytes(a, t16);
-    const dim3 grid(grid_for(a.nsub, 256));
-    if (t16)
-        hipLaunchKernelGGL(k_decode_redo<true>, grid, dim3(256), lds, st, a, s, end_in, end_out, dirty_cur, dirty_nxt,
-                           sum);
-    else
-        hipLaunchKernelGGL(k_decode_redo<false>, grid, dim3(256), lds, st, a, s, end_in, end_out, dirty_cur, dirty_nxt,
-                           sum);
-}
-void launch_decode_redo_list(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
-    allow_big_lds(k_decode_redo_list<true>);
-    allow_big_lds(k_decode_redo_list<false>);
+void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSummary* sum, hipStream_t st) {
+    allow_big_lds(k_decode_pass<true>);
+    allow_big_lds(k_decode_pass<false>);
     const bool t16 = a.lut16 != nullptr;
     const size_t lds = table_lds_bytes(a, t16) + (size_t)256 * kRowStride * 4;
-    // one thread per possible entry; threads past the listed count return at once
-    const dim3 grid(grid_for(a.nsub, 256));
+    // persistent workgroups; those past the listed count return at once
+    const dim3 grid((unsigned)std::min<uint64_t>(kPassGrid, std::max<uint64_t>(1, grid_for(a.nsub, 256))));
     if (t16)
-        hipLaunchKernelGGL(k_decode_redo_list<true>, grid, dim3(256), lds, st, a, s, sum);
+        hipLaunchKernelGGL(k_decode_pass<true>, grid, dim3(256), lds, st, a, s, pass, sum);
     else
-        hipLaunchKernelGGL(k_decode_redo_list<false>, grid, dim3(256), lds, st, a, s, sum);
+        hipLaunchKernelGGL(k_decode_pass<false>, grid, dim3(256), lds, st, a, s, pass, sum);
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {
