@@ -172,6 +172,19 @@ def bench_batch(args):
         log(f"batch with {nw} workers: {sweep[nw] * 1e3:.3f} ms")
     tb = sweep[int(args.workers.split(",")[0])]
     same = r1 == rb and all(torch.equal(a[:s], b[:s]) for a, b, (s, _) in zip(outs, outs1, rb))
+    # the worker-context path on the same batch (the fused path is the default)
+    ctx.set_workers(int(args.workers.split(",")[0]))
+    ctx.set_batch_fused(False)
+    batch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rw = batch()
+    torch.cuda.synchronize()
+    tw = (time.perf_counter() - t0) / args.steps
+    ctx.set_batch_fused(True)
+    log(f"worker-context batch: {tw * 1e3:.3f} ms")
+    same = same and rw == rb and all(torch.equal(a[:s], b[:s]) for a, b, (s, _) in zip(outs, outs1, rb))
 
     # decompress both ways
     vr = [float((f.max() - f.min()).item()) for f in fields]
@@ -223,6 +236,7 @@ def bench_batch(args):
         "per_field_calls_ms": round(tp * 1e3, 3),
         "batch_ms_by_workers": {str(k): round(v * 1e3, 3) for k, v in sorted(sweep.items())},
         "batch_speedup": round(tp / tb, 3),
+        "batch_worker_contexts_ms": round(tw * 1e3, 3),
         "identical_archives": bool(same),
         "decompress_gbps": round(total / tdb / 1e9, 2),
         "decompress_per_field_calls_ms": round(tdp * 1e3, 3),

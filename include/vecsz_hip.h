@@ -113,11 +113,16 @@ int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint
 /* Batched form: many independent fields per call, each compressed exactly as
  * vecsz_compress would (identical archives).  No counterpart in the
  * reference, whose main compresses one file per run (src/vecsz.cc:37).
- * Fields are spread, largest first, over worker contexts (own HIP stream and
- * scratch each, set_workers of them), so small fields overlap on the GPU and
- * host codebook work overlaps kernels.  status[k] receives each field's
- * VECSZ_* status; the call returns the first failure (or VECSZ_OK) after all
- * fields are done.  Inputs must be complete on the context's stream. */
+ * Compress of two or more fields that share block size and dict_size (at
+ * most 4096) and use no padding runs fused: one launch per stage covers every
+ * field (a work queue of Lorenzo-block tiles and encode chunks spanning all
+ * fields, per-field histograms and codebooks, the codebooks built on
+ * set_workers host threads).  Other batches, and every decompress batch, are
+ * spread, largest first, over worker contexts (own HIP stream and scratch
+ * each, set_workers of them), so small fields overlap on the GPU and host
+ * codebook work overlaps kernels.  status[k] receives each field's VECSZ_*
+ * status; the call returns the first failure (or VECSZ_OK) after all fields
+ * are done.  Inputs must be complete on the context's stream. */
 int vecsz_compress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* params, const float* const* d_in,
                          uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes, uint64_t* n_outlier,
                          int* status);
@@ -125,6 +130,9 @@ int vecsz_decompress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* para
                            const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status);
 /* Worker contexts used by the batched calls (1..16, default 8). */
 int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
+/* 0: compress batches always take the worker-context path; 1 (default): the
+ * fused path when the batch qualifies.  Both write identical archives. */
+int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
 
 /* Served form: a request server on one GPU (no counterpart in the
  * reference).  nslots slot threads, each with its own context (stream and

@@ -102,9 +102,15 @@ struct vecsz_ctx {
     // batch workers: each a context of its own (stream + scratch), created on first use
     std::vector<vecsz_ctx*> workers;
     int nworkers = 8;
+    bool fused_batch = true;  // compress batches as one launch per stage when they qualify
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
+    // fused batch compress: field descriptors, unit starts, per-field histograms,
+    // range words, eb families, summaries, status words, codebooks, headers
+    DevBuf b_desc, b_start, b_hist, b_range, b_ebd, b_sums, b_errs, b_book, b_head, b_rows;
+    HostBuf b_pin, b_pin_hist;
+    std::vector<Codebook> b_cb;
     Codebook cb;
     DecodeTables dt;  // kept between calls: its vectors keep their capacity
     // metadata of the last successful compress
@@ -984,11 +990,348 @@ static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, 
     return VECSZ_OK;
 }
 
+// Fused batch compress: every field shares the block size and dictionary, has
+// no padding and a dictionary small enough for the LDS codebooks.  Each stage
+// is one launch over a queue of work units that spans every field, so the
+// launch count does not grow with the number of fields: range + eb family,
+// dual quantization (per-field histograms), one copy of all histograms to
+// the host, the codebooks built on up to c->nworkers host threads, one copy
+// of all codebooks and headers back, then chunk bits, one scan over every
+// chunk of the batch, per-field sizes and the bit packing.  The archives are
+// the ones the single-field path writes.
+static bool batch_fusable(const vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
+                          uint8_t* const* d_out) {
+    if (nf < 2 || nf > 4096) return false;
+    for (int k = 0; k < nf; k++) {
+        const vecsz_params& q = p[k];
+        if (!d_in[k] || !d_out[k]) return false;
+        if (q.block != p[0].block || q.dict_size != p[0].dict_size || q.pad_mode != VECSZ_PAD_NONE) return false;
+        if (check_params(const_cast<vecsz_ctx*>(c), &q, true)) return false;
+    }
+    return p[0].dict_size <= kMaxLdsBook;
+}
+
+static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
+                                uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes,
+                                uint64_t* n_outlier, int* status) {
+    HIPCHK(c, hipSetDevice(c->device));
+    timing_begin(c);
+    const int B = p[0].block, dict = p[0].dict_size, R = dict / 2;
+    const float thr_vec = (float)vec_threshold(R);
+    // code / outlier-scratch regions: each field starts on a 64-element boundary
+    std::vector<uint64_t> off(nf), nel(nf);
+    uint64_t ntot = 0;
+    for (int k = 0; k < nf; k++) {
+        uint64_t n = 1;
+        for (int i = 0; i < p[k].ndim; i++) n *= p[k].dims[i];
+        nel[k] = n;
+        off[k] = ntot;
+        ntot += (n + 63) / 64 * 64;
+    }
+    // unit starts: range, chunks (the histogram pass borrows them first), then
+    // one dual-quant array per tile form
+    const size_t start_bytes = (2 + 5) * (size_t)(nf + 1) * 8;
+    if (!c->codes.ensure(ntot * 2 + 64) || !c->sparse.ensure(ntot * 4 + 64) ||
+        !c->b_hist.ensure((size_t)nf * dict * 8) || !c->b_range.ensure((size_t)nf * 8) ||
+        !c->b_ebd.ensure((size_t)nf * sizeof(EbDev)) || !c->b_desc.ensure((size_t)nf * sizeof(BatchField)) ||
+        !c->b_start.ensure(start_bytes) || !c->b_sums.ensure((size_t)nf * sizeof(EncSummary)) ||
+        !c->b_errs.ensure((size_t)nf * 4))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    const size_t desc_bytes = (size_t)nf * sizeof(BatchField);
+    const size_t pin_fixed = desc_bytes + start_bytes + (size_t)nf * 4 + 256;
+    if (!c->b_pin.ensure(pin_fixed)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    // descriptors and unit starts are built here and copied into pinned memory
+    // for each upload (the pinned block may be reallocated in between)
+    std::vector<BatchField> fd(nf);
+    std::vector<uint64_t> st(7 * (size_t)(nf + 1), 0);
+    uint64_t* rg0 = st.data();
+    uint64_t* ck0 = rg0 + (nf + 1);
+    uint64_t* dqk = rg0 + 2 * (nf + 1);  // dqk + kind * (nf + 1), kinds kDq1 .. kDq3x4
+    std::vector<char> dev_range(nf, 0);
+    BatchField* fdd = c->b_desc.as<BatchField>();
+    for (int k = 0; k < nf; k++) {
+        const vecsz_params& q = p[k];
+        const int ndim = q.ndim;
+        const uint64_t d0 = q.dims[0], d1 = ndim >= 2 ? q.dims[1] : 1, d2 = ndim >= 3 ? q.dims[2] : 1;
+        const bool r2r = q.mode == VECSZ_MODE_R2R;
+        dev_range[k] = r2r && !(q.value_range > 0);
+        BatchField& f = fd[k];
+        memset(&f, 0, sizeof(f));
+        DQArgs& a = f.a;
+        a.d0 = d0;
+        a.d1 = d1;
+        a.d2 = d2;
+        a.n = nel[k];
+        a.x8 = (d0 / 8) * 8;
+        a.radius = R;
+        a.radius_f = (float)R;
+        a.thr_vec = thr_vec;
+        a.thr_scalar = (float)R;
+        a.dict = dict;
+        a.reverse = 0;
+        a.halo = nullptr;
+        a.hb0 = ref_nblk(d0, B);
+        a.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
+        const bool x4 = dualquant_use_x4(ndim, B, d_in[k], d0);
+        a.ntiles_x = x4 ? (d0 + 255) / 256 : (d0 + 63) / 64;
+        a.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
+        a.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
+        if (dev_range[k]) {
+            f.range = c->b_range.as<uint32_t>() + 2 * k;
+            f.ebd = c->b_ebd.as<EbDev>() + k;
+            a.ebd = f.ebd;
+        } else {
+            const EbFamily eb = eb_family(q.eb, r2r, r2r ? q.value_range : 0.0);
+            a.ebd = nullptr;
+            a.r_f = eb.r_f;
+            a.r_d = eb.r_d;
+        }
+        f.in = d_in[k];
+        f.codes = c->codes.as<uint16_t>() + off[k];
+        f.sparse = c->sparse.as<float>() + off[k];
+        f.hist = c->b_hist.as<unsigned long long>() + (size_t)k * dict;
+        f.eb = q.eb;
+        f.kind = ndim == 1 ? kDq1 : ndim == 2 ? (x4 ? kDq2x4 : kDq2) : (x4 ? kDq3x4 : kDq3);
+        f.dq_tiles = ndim == 1 ? ((nel[k] + 7) / 8 + 63) / 64 : a.ntiles_x * a.ntiles_y * a.ntiles_z;
+        f.archive = d_out[k];
+        f.out_cap = out_cap[k];
+        f.meta_bytes = meta_bytes(ndim);
+        rg0[k + 1] = rg0[k] + (dev_range[k] ? range_units(nel[k]) : 0);
+        for (int kind = kDq1; kind <= kDq3x4; kind++) {
+            uint64_t* s0 = dqk + (size_t)kind * (nf + 1);
+            s0[k + 1] = s0[k] + (kind == f.kind ? (f.dq_tiles + 3) / 4 : 0);
+        }
+        status[k] = VECSZ_OK;
+        out_bytes[k] = 0;
+        n_outlier[k] = 0;
+    }
+    // Partial histogram rows of the code histogram pass: workgroup g writes
+    // one row per field its unit range [U g / G, U (g + 1) / G) touches, so
+    // each field's rows are contiguous; they are summed in groups of up to 32.
+    uint64_t* hs0 = ck0;  // the chunk starts are filled later: borrow their slots
+    hs0[0] = 0;
+    for (int k = 0; k < nf; k++) hs0[k + 1] = hs0[k] + hist_units(nel[k]);
+    const uint64_t U = hs0[nf];
+    const unsigned G = hist_multi_grid(U);
+    std::vector<uint32_t> rowtab(G);
+    std::vector<uint32_t> frow(nf, 0), fcnt(nf, 0);
+    uint32_t nrows = 0;
+    for (unsigned g = 0; g < G; g++) {
+        const uint64_t u0 = U * g / G, u1 = U * (g + 1) / G;
+        rowtab[g] = nrows;
+        if (u0 >= u1) continue;
+        const int f0 = (int)(std::upper_bound(hs0, hs0 + nf + 1, u0) - hs0) - 1;
+        const int f1 = (int)(std::upper_bound(hs0, hs0 + nf + 1, u1 - 1) - hs0) - 1;
+        for (int f = f0; f <= f1; f++) {
+            if (fcnt[f]++ == 0) frow[f] = nrows;
+            nrows++;
+        }
+    }
+    for (int f = 0; f < nf; f++)
+        for (uint32_t r = 0; r < fcnt[f]; r += 32) {
+            rowtab.push_back((uint32_t)f);
+            rowtab.push_back(frow[f] + r);
+            rowtab.push_back(std::min<uint32_t>(32, fcnt[f] - r));
+        }
+    const uint32_t ngroups = (uint32_t)((rowtab.size() - G) / 3);
+    if (!c->partials.ensure((size_t)nrows * dict * 4 + 64) || !c->b_rows.ensure(rowtab.size() * 4 + 64))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    if (!c->b_pin.ensure(pin_fixed + rowtab.size() * 4 + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    memcpy(c->b_pin.p, fd.data(), desc_bytes);
+    memcpy(c->b_pin.as<uint8_t>() + desc_bytes, st.data(), start_bytes);
+    memcpy(c->b_pin.as<uint8_t>() + pin_fixed, rowtab.data(), rowtab.size() * 4);
+    HIPCHK(c, hipMemcpyAsync(c->b_desc.p, c->b_pin.p, desc_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b_start.p, c->b_pin.as<uint8_t>() + desc_bytes, start_bytes, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->b_rows.p, c->b_pin.as<uint8_t>() + pin_fixed, rowtab.size() * 4,
+                             hipMemcpyHostToDevice, c->stream));
+    const uint64_t* rg0_d = c->b_start.as<uint64_t>();
+    const uint64_t* ck0_d = rg0_d + (nf + 1);
+    const uint64_t* dqk_d = rg0_d + 2 * (nf + 1);
+    {
+        StageScope s(c, "range");
+        launch_range_multi(fdd, nf, rg0_d, rg0[nf], c->stream);
+    }
+    HIPCHK(c, hipMemsetAsync(c->b_hist.p, 0, (size_t)nf * dict * 8, c->stream));
+    static const char* const kDqStage[] = {"dualquant_1d", "dualquant_2d", "dualquant_2d_x4", "dualquant_3d",
+                                           "dualquant_3d_x4"};
+    for (int kind = kDq1; kind <= kDq3x4; kind++) {
+        const uint64_t nu = dqk[(size_t)kind * (nf + 1) + nf];
+        if (!nu) continue;
+        StageScope s(c, kDqStage[kind]);
+        launch_dualquant_multi(B, kind, fdd, nf, dqk_d + (size_t)kind * (nf + 1), nu, c->stream);
+    }
+    {
+        StageScope s(c, "histogram");
+        const uint32_t* rows_d = c->b_rows.as<uint32_t>();
+        launch_hist_multi(fdd, nf, ck0_d, U, dict, c->partials.as<uint32_t>(), rows_d, rows_d + G, ngroups,
+                          c->b_hist.as<unsigned long long>(), c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    const size_t hist_bytes = (size_t)nf * dict * 8;
+    if (!c->b_pin_hist.ensure(hist_bytes + (size_t)nf * sizeof(EbDev) + 64))
+        return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    uint8_t* hp = c->b_pin_hist.as<uint8_t>();
+    HIPCHK(c, hipMemcpyAsync(hp, c->b_hist.p, hist_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hp + hist_bytes, c->b_ebd.p, (size_t)nf * sizeof(EbDev), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+
+    // codebooks on host threads
+    if ((int)c->b_cb.size() < nf) c->b_cb.resize(nf);
+    for (int k = 0; k < nf; k++) {
+        if (!dev_range[k]) continue;
+        EbDev e;
+        memcpy(&e, hp + hist_bytes + (size_t)k * sizeof(EbDev), sizeof(e));
+        if (e.zero_range) status[k] = VECSZ_ERANGE0;
+    }
+    {
+        HostTimer ht(c, "host_codebook");
+        std::atomic<int> next{0};
+        auto body = [&]() {
+            for (int k; (k = next.fetch_add(1)) < nf;)
+                if (status[k] == VECSZ_OK)
+                    status[k] = build_codebook(reinterpret_cast<const uint64_t*>(hp) + (size_t)k * dict, dict,
+                                               c->b_cb[k]);
+        };
+        const int nt = std::max(1, std::min(c->nworkers, nf));
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(body);
+        body();
+        for (auto& t : th) t.join();
+    }
+
+    // headers and codebooks, staged in one pinned block
+    const size_t book_stride = ((size_t)dict * 9 + 15) / 16 * 16;
+    std::vector<uint64_t> hoff(nf, 0);
+    uint64_t head_tot = 0;
+    for (int k = 0; k < nf; k++) {
+        if (status[k]) continue;
+        const uint64_t hb = fd[k].meta_bytes + 8 + c->b_cb[k].tree.size();
+        if (hb > out_cap[k]) {
+            out_bytes[k] = hb;
+            status[k] = VECSZ_ENOMEM;
+            continue;
+        }
+        hoff[k] = head_tot;
+        head_tot += (hb + 15) / 16 * 16;
+    }
+    const size_t books_bytes = book_stride * nf;
+    if (!c->b_pin.ensure(pin_fixed + books_bytes + head_tot + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    BatchField* fd_h = c->b_pin.as<BatchField>();
+    uint64_t* ck0_h = reinterpret_cast<uint64_t*>(c->b_pin.as<uint8_t>() + desc_bytes);
+    uint32_t* errs_h = reinterpret_cast<uint32_t*>(ck0_h + (nf + 1));
+    uint8_t* books_h = c->b_pin.as<uint8_t>() + pin_fixed;
+    uint8_t* heads_h = books_h + books_bytes;
+    if (!c->b_book.ensure(books_bytes + 64) || !c->b_head.ensure(head_tot + 64))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    uint64_t ctot = 0;
+    int max_len = 1;
+    ck0[0] = 0;
+    for (int k = 0; k < nf; k++) {
+        BatchField& f = fd[k];
+        errs_h[k] = status[k] ? 2u : 0u;
+        f.nchunks = 0;
+        f.chunk0 = ctot;
+        if (!status[k]) {
+            const Codebook& cb = c->b_cb[k];
+            const vecsz_params& q = p[k];
+            const uint64_t mb = f.meta_bytes, tb = cb.tree.size();
+            uint8_t* hdr = heads_h + hoff[k];
+            memset(hdr, 0, mb);
+            int32_t nd = q.ndim;
+            memcpy(&hdr[0], &nd, 4);
+            for (int i = 0; i < q.ndim; i++) {
+                int32_t d = (int32_t)q.dims[i];
+                memcpy(&hdr[4 + 4 * i], &d, 4);
+            }
+            memcpy(&hdr[4 * (q.ndim + 1)], &q.eb, 8);  // pad_len (next 8 bytes) stays 0
+            put_u32be(&hdr[mb], (uint32_t)cb.node_count);
+            put_u32be(&hdr[mb + 4], (uint32_t)dict);
+            memcpy(&hdr[mb + 8], cb.tree.data(), tb);
+            uint8_t* bk = books_h + book_stride * k;
+            memcpy(bk, cb.cw_hi.data(), (size_t)dict * 4);
+            memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
+            memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
+            f.book = reinterpret_cast<const uint32_t*>(c->b_book.as<uint8_t>() + book_stride * k);
+            f.head = c->b_head.as<uint8_t>() + hoff[k];
+            f.head_bytes = mb + 8 + tb;
+            f.tree_bytes = tb;
+            f.nchunks = (nel[k] + kChunk - 1) / kChunk;
+            max_len = std::max(max_len, cb.max_len);
+        }
+        ctot += f.nchunks;
+        ck0[k + 1] = ck0[k] + (f.nchunks + 3) / 4;
+    }
+    if (ctot) {
+        if (!c->chunk_bits.ensure(ctot * 4) || !c->chunk_zeros.ensure(ctot * 4) || !c->bit_off.ensure(ctot * 8) ||
+            !c->out_off.ensure(ctot * 8) || !c->scan_tmp.ensure(((ctot + 4095) / 4096 + 1) * 16) ||
+            !c->scan_tot.ensure(16))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        HIPCHK(c, hipMemcpyAsync(c->b_book.p, books_h, books_bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->b_head.p, heads_h, head_tot, hipMemcpyHostToDevice, c->stream));
+        memcpy(fd_h, fd.data(), desc_bytes);
+        memcpy(ck0_h, ck0, (size_t)(nf + 1) * 8);
+        HIPCHK(c, hipMemcpyAsync(c->b_desc.p, fd_h, desc_bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync((void*)ck0_d, ck0_h, (size_t)(nf + 1) * 8, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->b_errs.p, errs_h, (size_t)nf * 4, hipMemcpyHostToDevice, c->stream));
+        uint32_t* errs_d = c->b_errs.as<uint32_t>();
+        launch_place_heads(fdd, nf, errs_d, c->stream);
+        {
+            StageScope s(c, "chunk_bits");
+            launch_chunk_bits_multi(fdd, nf, ck0_d, ck0[nf], dict, c->chunk_bits.as<uint32_t>(),
+                                    c->chunk_zeros.as<uint32_t>(), c->stream);
+        }
+        {
+            StageScope s(c, "scan");
+            launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), ctot,
+                              c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
+                              c->scan_tot.as<uint64_t>(), c->stream);
+            launch_enc_summary_multi(fdd, nf, c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(),
+                                     c->scan_tot.as<uint64_t>(), ctot, c->b_sums.as<EncSummary>(), errs_d, c->stream);
+        }
+        {
+            StageScope s(c, "pack");
+            launch_pack_multi(fdd, nf, ck0_d, ck0[nf], dict, max_len, c->bit_off.as<uint64_t>(),
+                              c->out_off.as<uint64_t>(), c->b_sums.as<EncSummary>(), errs_d, c->stream);
+        }
+        HIPCHK(c, hipGetLastError());
+        EncSummary* sums_h = reinterpret_cast<EncSummary*>(c->b_pin_hist.as<uint8_t>());
+        uint32_t* errs_back = reinterpret_cast<uint32_t*>(c->b_pin_hist.as<uint8_t>() + (size_t)nf * sizeof(EncSummary));
+        HIPCHK(c, hipMemcpyAsync(sums_h, c->b_sums.p, (size_t)nf * sizeof(EncSummary), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipMemcpyAsync(errs_back, errs_d, (size_t)nf * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < nf; k++) {
+            if (status[k]) continue;
+            out_bytes[k] = sums_h[k].archive_size;
+            n_outlier[k] = sums_h[k].n_outlier;
+            if (errs_back[k]) status[k] = VECSZ_ENOMEM;
+        }
+    }
+    timing_end(c);
+    for (int k = 0; k < nf; k++)
+        if (status[k]) {
+            const char* why = status[k] == VECSZ_ERANGE0   ? "r2r value range is zero"
+                              : status[k] == VECSZ_ENOMEM  ? "output buffer too small"
+                              : status[k] == VECSZ_ECODELEN ? "Huffman code longer than 64 bits"
+                                                            : "codebook construction failed";
+            return fail(c, status[k], "batch field " + std::to_string(k) + " failed: " + why);
+        }
+    return VECSZ_OK;
+}
+
 extern "C" {
 
 int vecsz_ctx_set_workers(vecsz_ctx* c, int nworkers) {
     if (!c || nworkers < 1 || nworkers > 16) return VECSZ_EINVAL;
     c->nworkers = nworkers;
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_set_batch_fused(vecsz_ctx* c, int on) {
+    if (!c) return VECSZ_EINVAL;
+    c->fused_batch = on != 0;
     return VECSZ_OK;
 }
 
@@ -1003,6 +1346,8 @@ int vecsz_compress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, const
         for (int i = 0; i < std::max(1, std::min(3, p[k].ndim)); i++) n *= p[k].dims[i];
         weight[k] = n;
     }
+    if (c->fused_batch && batch_fusable(c, nfields, p, d_in, d_out))
+        return compress_batch_fused(c, nfields, p, d_in, d_out, out_cap, out_bytes, n_outlier, status);
     const uint64_t nmax = nfields ? *std::max_element(weight.begin(), weight.end()) : 0;
     auto prep = [&](vecsz_ctx* w) {
         (void)hipSetDevice(w->device);

@@ -71,6 +71,29 @@ struct EncSummary {
     uint64_t stream_end_byte;   // first byte after the bitstream
 };
 
+// One field of a fused batch compress.  A device array of these drives the
+// batched kernels; each kernel walks a queue of work units that spans every
+// field (unit u belongs to field f when start[f] <= u < start[f + 1]).
+enum DqKind { kDq1 = 0, kDq2 = 1, kDq2x4 = 2, kDq3 = 3, kDq3x4 = 4 };
+struct BatchField {
+    DQArgs a;                   // dual-quant scalars (a.ebd = ebd when the range is found on the device)
+    const float* in;
+    uint16_t* codes;            // this field's quant codes and outlier scratch
+    float* sparse;
+    unsigned long long* hist;   // dict bins
+    uint32_t* range;            // min/max words (range found on the device), else null
+    EbDev* ebd;
+    double eb;                  // the caller's error bound (relative for r2r)
+    int kind;                   // DqKind
+    uint64_t dq_tiles;          // wave tiles of that form
+    const uint32_t* book;       // cw_hi[dict], cw_lo[dict], then dict u8 lengths
+    const uint8_t* head;        // metadata + Huffman header + tree, staged on the device
+    uint64_t head_bytes, meta_bytes, tree_bytes;
+    uint8_t* archive;
+    uint64_t out_cap;
+    uint64_t nchunks, chunk0;   // 1024-symbol chunks, and the first one's index in the batch
+};
+
 // Decoder summary.
 struct DecSummary {
     uint32_t error;       // nonzero: corrupt stream

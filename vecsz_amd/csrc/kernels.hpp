@@ -23,6 +23,31 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st);
 
+// Batched compress (kernels_compress.hip): every field of a batch per launch.
+// start arrays hold nf + 1 unit offsets (see BatchField).
+uint64_t range_units(uint64_t n);
+void launch_range_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, hipStream_t st);
+// Dual quantization of the fields whose tile form is `kind` (DqKind); start
+// gives units only to those fields.
+void launch_dualquant_multi(int B, int kind, const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits,
+                            hipStream_t st);
+// Histograms of the codes: workgroup g of hist_multi_grid(U) takes units
+// [U g / G, U (g + 1) / G) and writes one partial row per field it visits,
+// from row0[g] on; groups: {field, first row, rows <= 32} triples.
+uint64_t hist_units(uint64_t n);
+unsigned hist_multi_grid(uint64_t nunits);
+void launch_hist_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
+                       uint32_t* partials, const uint32_t* row0, const uint32_t* groups, uint32_t ngroups,
+                       unsigned long long* hist, hipStream_t st);
+void launch_place_heads(const BatchField* fd, int nf, const uint32_t* errs, hipStream_t st);
+void launch_chunk_bits_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
+                             uint32_t* cb, uint32_t* cz, hipStream_t st);
+void launch_enc_summary_multi(const BatchField* fd, int nf, const uint64_t* bit_off, const uint64_t* out_off,
+                              const uint64_t* totals, uint64_t ctot, EncSummary* sums, uint32_t* errs, hipStream_t st);
+void launch_pack_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict, int max_len,
+                       const uint64_t* bit_off, const uint64_t* out_off, const EncSummary* sums, const uint32_t* errs,
+                       hipStream_t st);
+
 // Decoder (kernels_decompress.hip)
 struct DecodeArgs {
     const uint8_t* stream;   // first byte of the bitstream (any alignment)

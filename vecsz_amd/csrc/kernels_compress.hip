@@ -101,8 +101,8 @@ __global__ void __launch_bounds__(256) k_range(const float* __restrict__ x, uint
 // r2r error-bound family from the device-side min/max.  std::max_element /
 // std::min_element keep a leading NaN (then the range is NaN); otherwise
 // they ignore NaNs, which is what the NaN-skipping reduction found.
-__global__ void k_ebfam(const uint32_t* __restrict__ mm, const float* __restrict__ x, double eb, EbDev* __restrict__ out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void ebfam(const uint32_t* __restrict__ mm, const float* __restrict__ x, double eb,
+                                      EbDev* __restrict__ out) {
     const float x0 = x[0];
     float mn, mx;
     if (x0 != x0) {
@@ -120,6 +120,10 @@ __global__ void k_ebfam(const uint32_t* __restrict__ mm, const float* __restrict
     e.twoeb = 2 * e.eb_final;
     e.r_f = (float)e.r_d;
     *out = e;
+}
+
+__global__ void k_ebfam(const uint32_t* __restrict__ mm, const float* __restrict__ x, double eb, EbDev* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ebfam(mm, x, eb, out);
 }
 
 // Replaces a's r_f / r_d by the device-computed family when there is one.
@@ -174,6 +178,18 @@ __device__ void hist_init(LdsHist& H, uint32_t* lds, int dict, int radius, unsig
 // Folds the lane-private hot copies into the bins, then writes this
 // workgroup's histogram densely to partials[blockIdx.x][*] (k_hist_reduce sums
 // them) -- or, for dictionaries too large for LDS, adds to the global bins.
+__device__ void hist_flush_row(LdsHist& H, uint32_t* __restrict__ dst) {
+    __syncthreads();
+    for (int h = threadIdx.x; h < kHotBins; h += blockDim.x) {
+        uint32_t s = 0;
+        for (int c = 0; c < 64; c++) s += H.hot[h * 64 + c];
+        const int bin = H.lo + h;
+        if (s && bin >= 0 && bin < H.dict) H.cold[bin] += s;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < H.dict; b += blockDim.x) dst[b] = H.cold[b];
+}
+
 __device__ void hist_flush(LdsHist& H, uint32_t* __restrict__ partials) {
     __syncthreads();
     for (int h = threadIdx.x; h < kHotBins; h += blockDim.x) {
@@ -242,21 +258,12 @@ __device__ __forceinline__ Halo load_halo(const DQArgs& a, bool xin, uint64_t x,
 // PAD: the halo cells hold pad values instead of +0.0 (src/dualquant.hh:
 // 226-246, :553-573): a neighbour across the block's z face reads h.z, else
 // across its y face h.y, else h.x (src/utils/padding.hh:245, :290).
-template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ in, uint16_t* __restrict__ codes,
-                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      uint32_t* __restrict__ partials, DQArgs a) {
-    extern __shared__ uint32_t lds[];
-    LdsHist H;
-    hist_init(H, lds, a.dict, a.radius, hist);
-    dq_resolve(a);
-    const uint32_t lane = lane_id();
-    const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
+template <int B, bool PAD, class Hist>
+__device__ __forceinline__ void dq3_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                         float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
+                                         uint32_t lane) {
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
-        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
+    {
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
         const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
@@ -319,6 +326,22 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
             }
         }
     }
+}
+
+template <int B, bool PAD = false>
+__global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+        dq3_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
@@ -329,20 +352,11 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
 // code 0, outlier +0.0.
 // PAD: a neighbour across the block's x face reads h.x, else across its
 // y face h.y (src/utils/padding.hh:226, :275).
-template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ in, uint16_t* __restrict__ codes,
-                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      uint32_t* __restrict__ partials, DQArgs a) {
-    extern __shared__ uint32_t lds[];
-    LdsHist H;
-    hist_init(H, lds, a.dict, a.radius, hist);
-    dq_resolve(a);
-    const uint32_t lane = lane_id();
-    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
-        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
+template <int B, bool PAD, class Hist>
+__device__ __forceinline__ void dq2_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                         float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
+                                         uint32_t lane) {
+    {
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = t / a.ntiles_x;
         const uint64_t x = tx * 64 + lane;
@@ -377,6 +391,22 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
             }
         }
     }
+}
+
+template <int B, bool PAD = false>
+__global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+        dq2_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
@@ -391,9 +421,9 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
 // One 256 x B x B tile.  SCAL is true only for tiles that contain lanes of
 // the partial x-group (x >= 8*floor(DIM0/8)); every other tile runs the AVX
 // forms alone, without evaluating the scalar ones.
-template <int B, bool SCAL>
+template <int B, bool SCAL, class Hist>
 __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
-                                            float* __restrict__ sparse, LdsHist& H, const DQArgs& a, uint64_t x,
+                                            float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t x,
                                             uint64_t y0, uint64_t z0) {
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
     const bool xin = x < a.d0;
@@ -493,6 +523,22 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
     }
 }
 
+// Tile t of the 256 x B x B grid.
+template <int B, class Hist>
+__device__ __forceinline__ void dq3_x4_at(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                          float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
+                                          uint32_t lane) {
+    const uint64_t tx = t % a.ntiles_x;
+    const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
+    const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
+    const uint64_t x = tx * 256 + 4 * lane;
+    // the partial x-group only exists in the last tile column
+    if (tx * 256 + 256 > a.x8 && a.x8 < a.d0)
+        dq3_x4_tile<B, true>(in, codes, sparse, H, a, x, ty * B, tz * B);
+    else
+        dq3_x4_tile<B, false>(in, codes, sparse, H, a, x, ty * B, tz * B);
+}
+
 template <int B>
 __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                          float* __restrict__ sparse,
@@ -506,36 +552,16 @@ __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
-        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
-        const uint64_t tx = t % a.ntiles_x;
-        const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
-        const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
-        const uint64_t x = tx * 256 + 4 * lane;
-        // the partial x-group only exists in the last tile column
-        if (tx * 256 + 256 > a.x8 && a.x8 < a.d0)
-            dq3_x4_tile<B, true>(in, codes, sparse, H, a, x, ty * B, tz * B);
-        else
-            dq3_x4_tile<B, false>(in, codes, sparse, H, a, x, ty * B, tz * B);
-    }
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+        dq3_x4_at<B>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
-template <int B>
-__global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
-                                                         float* __restrict__ sparse,
-                                                         unsigned long long* __restrict__ hist,
-                                                         uint32_t* __restrict__ partials, DQArgs a) {
-    extern __shared__ uint32_t lds[];
-    LdsHist H;
-    hist_init(H, lds, a.dict, a.radius, hist);
-    dq_resolve(a);
-    const uint32_t lane = lane_id();
-    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves) {
-        const uint64_t t = a.reverse ? ntiles - 1 - tt : tt;
+template <int B, class Hist>
+__device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                            float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
+                                            uint32_t lane) {
+    {
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = t / a.ntiles_x;
         const uint64_t x = tx * 256 + 4 * lane;
@@ -585,6 +611,23 @@ __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict
             }
         }
     }
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                         float* __restrict__ sparse,
+                                                         unsigned long long* __restrict__ hist,
+                                                         uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+        dq2_x4_tile<B>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
@@ -600,20 +643,13 @@ __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict
 // may be non-integral: then the AVX forms' sqrt(e*e) < R test and
 // round-to-nearest conversion (_mm256_cvtps_epi32, :172) and the scalar
 // forms' |e| < R and truncation (:200) are evaluated as written.
-template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ in, uint16_t* __restrict__ codes,
-                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
-                                                      uint32_t* __restrict__ partials, DQArgs a) {
-    extern __shared__ uint32_t lds[];
-    LdsHist H;
-    hist_init(H, lds, a.dict, a.radius, hist);
-    dq_resolve(a);
-    const uint32_t lane = lane_id();
+// Groups wb .. wb + 63 (wave tile wb / 64).
+template <int B, bool PAD, class Hist>
+__device__ __forceinline__ void dq1_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                         float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t wb,
+                                         uint32_t lane, bool aligned) {
     const uint64_t ngroups = (a.n + 7) / 8;
-    const bool aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t wb = wave * 64; wb < ngroups; wb += nwaves * 64) {
+    {
         const uint64_t g = wb + lane;
         const uint64_t x0 = g * 8;
         float q[8];
@@ -659,6 +695,23 @@ __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ 
             }
         }
     }
+}
+
+template <int B, bool PAD = false>
+__global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ in, uint16_t* __restrict__ codes,
+                                                      float* __restrict__ sparse, unsigned long long* __restrict__ hist,
+                                                      uint32_t* __restrict__ partials, DQArgs a) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    hist_init(H, lds, a.dict, a.radius, hist);
+    dq_resolve(a);
+    const uint32_t lane = lane_id();
+    const uint64_t ngroups = (a.n + 7) / 8;
+    const bool aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t wb = wave * 64; wb < ngroups; wb += nwaves * 64)
+        dq1_tile<B, PAD>(in, codes, sparse, H, a, wb, lane, aligned);
     hist_flush(H, partials);
 }
 
@@ -777,11 +830,9 @@ __global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__
 // Archive sizes of src/vecsz_interface.hh:189-202 from the scanned chunk
 // totals, and the two length fields of the payload header
 // (src/utils/datapack.hh:288) written in place.
-__global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
-                              uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
-                              uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint64_t tb = totals[0], tz = totals[1];
+__device__ __forceinline__ void enc_summary(uint64_t tb, uint64_t tz, EncSummary* __restrict__ sum,
+                                            uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
+                                            uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
     const uint64_t enc = (tb + 7) / 8;
     const uint64_t huff = 8 + tree_bytes + enc;
     const uint64_t stream_byte = meta_bytes + pad_bytes + 8 + tree_bytes;
@@ -801,6 +852,13 @@ __global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* _
         for (int i = 0; i < 8; i++) f[i] = (uint8_t)(huff >> (8 * i));
         for (int i = 0; i < 8; i++) f[8 + i] = (uint8_t)(tz >> (8 * i));
     }
+}
+
+__global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
+                              uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
+                              uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        enc_summary(totals[0], totals[1], sum, archive, meta_bytes, pad_bytes, tree_bytes, out_cap, err);
 }
 
 // Bit packing, one wavefront per 1024-symbol chunk, no workgroup barriers or
@@ -891,6 +949,168 @@ __host__ __device__ constexpr uint32_t book_words(int book, int dict) {
     return book == kBookPacked ? (uint32_t)dict + 1 : book == kBookSplit ? (uint32_t)dict + 1 + ((uint32_t)dict + 4) / 4 : 0;
 }
 
+// Packs chunk k (codes already in `cur`) into the archive; see k_pack.
+template <int BOOK>
+__device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint64_t nchunks, uint32_t lane,
+                                           const uint32_t* tab, const uint8_t* tlen,
+                                           const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
+                                           const uint8_t* __restrict__ len_tab, const EncSummary& S,
+                                           const float* __restrict__ sparse, uint8_t* __restrict__ archive,
+                                           uint32_t* stage) {
+    constexpr bool LDS_BOOK = BOOK != kBookGlobal;
+    uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
+    const uint64_t base = k * kChunk + lane * kSymPerThread;
+
+    // the LDS books keep the raw entries (one or two registers per symbol)
+    uint32_t ent[LDS_BOOK ? kSymPerThread : 1];
+    uint32_t elen[BOOK == kBookSplit ? kSymPerThread : 1];
+    Code c[LDS_BOOK ? 1 : kSymPerThread];
+    uint32_t tbits = 0, tz = 0, long_mask = 0;
+#pragma unroll
+    for (int j = 0; j < kSymPerThread; j++) {
+        const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+        if constexpr (BOOK == kBookPacked) {
+            ent[j] = tab[sj];
+            const uint32_t l = ent[j] & 63;
+            tbits += l;
+            long_mask |= (uint32_t)(l == kLongCode) << j;
+        } else if constexpr (BOOK == kBookSplit) {
+            ent[j] = tab[sj];
+            elen[j] = tlen[sj];
+            tbits += elen[j];
+        } else {
+            const bool valid = (uint32_t)j < cur.nvalid;
+            c[j] = Code{0, 0, 0};
+            if (valid) c[j] = lookup_global(sj, cw_hi, cw_lo, len_tab);
+            tbits += c[j].len;
+        }
+    }
+    if (cur.nz) {  // outliers are rare: count them per lane only in chunks that hold some
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++)
+            tz += (uint32_t)j < cur.nvalid && ((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF) == 0;
+    }
+    if (BOOK == kBookPacked && long_mask) {
+        // codewords of more than 26 bits: true lengths from the global table
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++)
+            if ((long_mask >> j) & 1) tbits += (uint32_t)len_tab[(cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF] - kLongCode;
+    }
+    // bits (low 17) and outliers (high 15) in one wavefront scan
+    const uint32_t v = tbits | (tz << 17);
+    const uint32_t incl = wave_scan_dpp(v);
+    const uint32_t excl = incl - v;
+    const uint32_t cbits = lane63(incl) & 0x1FFFF;
+
+    // outlier compaction, index order
+    if (tz) {
+        uint64_t r = cur.out_off + (excl >> 17);
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) {
+            const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+            if ((uint32_t)j < cur.nvalid && sj == 0) {
+                const uint32_t bv = __float_as_uint(sparse[base + j]);
+                uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
+                dst[0] = (uint8_t)bv;
+                dst[1] = (uint8_t)(bv >> 8);
+                dst[2] = (uint8_t)(bv >> 16);
+                dst[3] = (uint8_t)(bv >> 24);
+                r++;
+            }
+        }
+    }
+    if (S.total_bits == 0) return;  // one-symbol tree: no bitstream
+
+    const uint64_t Ak = S.stream_bit_off + cur.bit_off;
+    const uint64_t wbase = Ak >> 5;
+    const uint32_t a0 = (uint32_t)(Ak & 31);
+    const uint32_t nwords = (a0 + cbits + 31) >> 5;
+    for (uint32_t i = lane; i < nwords; i += kWave) stage[i] = 0;
+    wave_lds_sync();
+
+    // Each codeword is ORed into the one or two stage words it overlaps
+    // (codeword at bit offset o of word w, MSB first).  The second OR may
+    // add zero to the word after the chunk's last; the stage has room.
+    uint32_t pos = a0 + (excl & 0x1FFFF);
+    auto place = [&](uint32_t cw, uint32_t l) {  // l <= 32; l = 0 places nothing (cw = 0)
+        const uint32_t o = pos & 31;
+        const uint64_t v = (uint64_t)cw << ((64 - o - l) & 63);
+        uint32_t* w = stage + (pos >> 5);
+        atomicOr(w, (uint32_t)(v >> 32));
+        atomicOr(w + 1, (uint32_t)v);
+        pos += l;
+    };
+    auto place_code = [&](const Code& c) {
+        if (c.len > 32) {
+            place(c.hi, c.len - 32);
+            place(c.lo, 32);
+        } else if (c.len) {
+            place(c.lo, c.len);
+        }
+    };
+    if constexpr (BOOK == kBookSplit) {
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) place(ent[j], elen[j]);  // absent symbols: length 0
+    } else if constexpr (BOOK == kBookPacked) {
+        if (!__any(long_mask != 0)) {
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) place(ent[j] >> 6, ent[j] & 63);  // absent symbols: ent 0
+        } else {
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) {
+                const uint32_t l = ent[j] & 63;
+                if (l == kLongCode)
+                    place_code(lookup_global((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF, cw_hi, cw_lo, len_tab));
+                else if (l)
+                    place(ent[j] >> 6, l);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kSymPerThread; j++) place_code(c[j]);
+    }
+
+    // complete the last word with the next chunk's leading bits
+    const uint32_t e = (a0 + cbits) & 31;
+    if (e != 0 && k + 1 < nchunks) {
+        Code d{0, 0, 0};
+        if (cur.next != 0xFFFFFFFFu) {
+            if constexpr (BOOK == kBookSplit)
+                d = Code{tab[cur.next], 0, tlen[cur.next]};
+            else
+                d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
+        }
+        const uint32_t p = wave_scan_dpp(d.len) - d.len;
+        uint32_t t = 0;
+        if (d.len && p < 32) {
+            const uint32_t kk = min(d.len, 32u - p);
+            const uint64_t cw = ((uint64_t)d.hi << 32) | d.lo;
+            t = (uint32_t)((cw >> (d.len - kk)) << (32 - p - kk));
+        }
+        t = lane63(wave_scan_dpp<true>(t));
+        if (lane == 0) atomicOr(&stage[nwords - 1], t >> e);
+    }
+    wave_lds_sync();
+
+    for (uint32_t i = ((a0 != 0 && k != 0) ? 1u : 0u) + lane; i < nwords; i += kWave) {
+        const uint64_t Wg = wbase + i;
+        uint32_t wv = stage[i];
+        if (k == 0 && i == 0 && a0) {
+            // keep the tree bytes that precede the stream in this word
+            const uint32_t mask = ~0u << (32 - a0);
+            wv = (__builtin_bswap32(out32[Wg]) & mask) | (wv & ~mask);
+        }
+        if ((Wg + 1) * 4 > S.stream_end_byte) {
+            uint8_t* bp = archive + Wg * 4;
+            for (int j = 0; j < 4; j++)
+                if (Wg * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(wv >> (24 - 8 * j));
+        } else {
+            out32[Wg] = __builtin_bswap32(wv);
+        }
+    }
+    wave_lds_sync();
+}
+
 template <int BOOK>
 __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes, uint64_t n,
                                               const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
@@ -919,7 +1139,6 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t* stage = lds + book_words(BOOK, dict) + wave * stage_words;
     const EncSummary S = *sum;
-    uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint32_t lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
     uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + wave;
@@ -932,156 +1151,341 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     for (; k < nchunks; k += nw) {
         const ChunkRaw cur = nxt;
         if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k + nw, lane, fill, nxt);
-        const uint64_t base = k * kChunk + lane * kSymPerThread;
+        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
+    }
+}
 
-        // the LDS books keep the raw entries (one or two registers per symbol)
-        uint32_t ent[LDS_BOOK ? kSymPerThread : 1];
-        uint32_t elen[BOOK == kBookSplit ? kSymPerThread : 1];
-        Code c[LDS_BOOK ? 1 : kSymPerThread];
-        uint32_t tbits = 0, tz = 0, long_mask = 0;
+// ------------------------------------------------------------------ batched fields
+
+// Field of work unit u: the f with start[f] <= u < start[f + 1] (start is
+// nondecreasing, start[0] = 0, start[nf] = total units > u).  u is uniform
+// across the workgroup, so the search runs on scalar loads.
+__device__ __forceinline__ int field_of(const uint64_t* __restrict__ start, int nf, uint64_t u) {
+    int lo = 0, hi = nf;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Min/max words of every field whose range is found on the device.
+__global__ void k_range_init_multi(const BatchField* __restrict__ fd, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nf && fd[f].range) {
+        fd[f].range[0] = 0xFFFFFFFFu;
+        fd[f].range[1] = 0u;
+    }
+}
+
+// One workgroup per unit of kRangeUnit floats of one field (same reduction
+// and NaN rule as k_range).
+constexpr uint64_t kRangeUnit = 65536;
+
+__global__ void __launch_bounds__(256) k_range_multi(const BatchField* __restrict__ fd,
+                                                     const uint64_t* __restrict__ start, int nf, uint64_t nunits) {
+    __shared__ float smn[4], smx[4];
+    for (uint64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int f = field_of(start, nf, u);
+        const float* x = fd[f].in;
+        const uint64_t n = fd[f].a.n;
+        const uint64_t b = (u - start[f]) * kRangeUnit, e = min(n, b + kRangeUnit);
+        float mn = __builtin_inff(), mx = -__builtin_inff();
+        uint64_t tail = b;
+        if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+            const float4* x4 = reinterpret_cast<const float4*>(x);
+            const uint64_t i0 = b / 4, i1 = e / 4;
+            uint64_t i = i0 + threadIdx.x;
+            for (; i + 768 < i1; i += 1024) {
+                float4 v[4];
 #pragma unroll
-        for (int j = 0; j < kSymPerThread; j++) {
-            const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-            if constexpr (BOOK == kBookPacked) {
-                ent[j] = tab[sj];
-                const uint32_t l = ent[j] & 63;
-                tbits += l;
-                long_mask |= (uint32_t)(l == kLongCode) << j;
-            } else if constexpr (BOOK == kBookSplit) {
-                ent[j] = tab[sj];
-                elen[j] = tlen[sj];
-                tbits += elen[j];
-            } else {
-                const bool valid = (uint32_t)j < cur.nvalid;
-                c[j] = Code{0, 0, 0};
-                if (valid) c[j] = lookup_global(sj, cw_hi, cw_lo, len_tab);
-                tbits += c[j].len;
+                for (int k = 0; k < 4; k++) v[k] = x4[i + 256 * k];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    mn = fminf(mn, fminf(fminf(v[k].x, v[k].y), fminf(v[k].z, v[k].w)));
+                    mx = fmaxf(mx, fmaxf(fmaxf(v[k].x, v[k].y), fmaxf(v[k].z, v[k].w)));
+                }
+            }
+            for (; i < i1; i += 256) {
+                const float4 v = x4[i];
+                mn = fminf(mn, fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+                mx = fmaxf(mx, fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
+            }
+            tail = i1 * 4;
+        }
+        for (uint64_t i = tail + threadIdx.x; i < e; i += 256) {
+            const float v = x[i];
+            mn = fminf(mn, v);
+            mx = fmaxf(mx, v);
+        }
+        for (int d = 32; d > 0; d >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, d));
+            mx = fmaxf(mx, __shfl_xor(mx, d));
+        }
+        const int w = threadIdx.x >> 6;
+        if (lane_id() == 0) {
+            smn[w] = mn;
+            smx[w] = mx;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < 4; i++) {
+                mn = fminf(mn, smn[i]);
+                mx = fmaxf(mx, smx[i]);
+            }
+            if (mn == mn) atomicMin(&fd[f].range[0], f2ord(mn));
+            if (mx == mx) atomicMax(&fd[f].range[1], f2ord(mx));
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_ebfam_multi(const BatchField* __restrict__ fd, int nf) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < nf && fd[f].range) ebfam(fd[f].range, fd[f].in, fd[f].eb, fd[f].ebd);
+}
+
+// No histogram: the batched dual quantization counts nothing (the codes are
+// counted by k_hist_codes_multi).
+struct NoHist {
+    __device__ __forceinline__ void add(int) {}
+};
+
+// Dual quantization of every field of a batch whose tiles take form KIND
+// (one launch per form, so each keeps its own register budget).  The work
+// units (four wave tiles of one field each) are taken grid-stride, so the
+// waves in flight work on neighbouring tiles of the same planes as in the
+// single-field kernels; fields of other forms have no units.  The tile bodies
+// are the single-field kernels'.
+template <int B, int KIND>
+__global__ void __launch_bounds__(256) k_dualquant_multi(const BatchField* __restrict__ fd,
+                                                         const uint64_t* __restrict__ start, int nf,
+                                                         uint64_t nunits) {
+    NoHist H;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    DQArgs a;
+    const float* in = nullptr;
+    uint16_t* codes = nullptr;
+    float* sparse = nullptr;
+    uint64_t ntiles = 0, ub = 0, ue = 0;
+    bool aligned = false;
+    for (uint64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+        if (u >= ue) {  // next field (uniform)
+            const int f = field_of(start, nf, u);
+            a = fd[f].a;
+            dq_resolve(a);
+            in = fd[f].in;
+            codes = fd[f].codes;
+            sparse = fd[f].sparse;
+            ntiles = fd[f].dq_tiles;
+            ub = start[f];
+            ue = start[f + 1];
+            aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+        }
+        const uint64_t t = (u - ub) * 4 + wave;
+        if (t >= ntiles) continue;
+        if constexpr (KIND == kDq3x4) {
+            if constexpr (B <= 16) dq3_x4_at<B>(in, codes, sparse, H, a, t, lane);
+        } else if constexpr (KIND == kDq3) {
+            dq3_tile<B, false>(in, codes, sparse, H, a, t, lane);
+        } else if constexpr (KIND == kDq2x4) {
+            dq2_x4_tile<B>(in, codes, sparse, H, a, t, lane);
+        } else if constexpr (KIND == kDq2) {
+            dq2_tile<B, false>(in, codes, sparse, H, a, t, lane);
+        } else {
+            dq1_tile<B, false>(in, codes, sparse, H, a, t * 64, lane, aligned);
+        }
+    }
+}
+
+// Histogram of every field's codes.  A unit is kHistUnit codes of one field;
+// workgroup g takes the contiguous units [U g / G, U (g + 1) / G) (streaming
+// reads, one field at a time) and writes its LDS histogram of each field it
+// visits as one dense row of `partials`, rows row0[g], row0[g] + 1, ... in
+// field order, so a field's rows are contiguous (k_hist_reduce_multi sums
+// them).
+constexpr uint64_t kHistUnit = 16384;
+
+__global__ void __launch_bounds__(256) k_hist_codes_multi(const BatchField* __restrict__ fd,
+                                                          const uint64_t* __restrict__ start, int nf, uint64_t nunits,
+                                                          int dict, int radius, uint32_t* __restrict__ partials,
+                                                          const uint32_t* __restrict__ row0) {
+    extern __shared__ uint32_t lds[];
+    LdsHist H;
+    uint64_t u = nunits * blockIdx.x / gridDim.x;
+    const uint64_t u1 = nunits * (blockIdx.x + 1) / gridDim.x;
+    if (u >= u1) return;
+    uint32_t row = row0[blockIdx.x];
+    for (int f = field_of(start, nf, u); u < u1; f++) {
+        const uint64_t ue = min(u1, start[f + 1]);
+        if (u >= ue) continue;
+        __syncthreads();  // the previous field's row has been read out of LDS
+        hist_init(H, lds, dict, radius, nullptr);
+        const uint16_t* codes = fd[f].codes;  // 128-byte aligned
+        const uint64_t n = fd[f].a.n;
+        const uint64_t b = (u - start[f]) * kHistUnit, e = min(n, (ue - start[f]) * kHistUnit);
+        const uint64_t e8 = e / 8 * 8;  // whole groups of 8 codes
+        const uint4* c8 = reinterpret_cast<const uint4*>(codes);
+        for (uint64_t i = b / 8 + threadIdx.x; i < e8 / 8; i += 256) {
+            const uint4 v = c8[i];
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                H.add((int)(w[j] & 0xFFFF));
+                H.add((int)(w[j] >> 16));
             }
         }
-        if (cur.nz) {  // outliers are rare: count them per lane only in chunks that hold some
-#pragma unroll
-            for (int j = 0; j < kSymPerThread; j++)
-                tz += (uint32_t)j < cur.nvalid && ((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF) == 0;
-        }
-        if (BOOK == kBookPacked && long_mask) {
-            // codewords of more than 26 bits: true lengths from the global table
-#pragma unroll
-            for (int j = 0; j < kSymPerThread; j++)
-                if ((long_mask >> j) & 1) tbits += (uint32_t)len_tab[(cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF] - kLongCode;
-        }
-        // bits (low 17) and outliers (high 15) in one wavefront scan
-        const uint32_t v = tbits | (tz << 17);
-        const uint32_t incl = wave_scan_dpp(v);
-        const uint32_t excl = incl - v;
-        const uint32_t cbits = lane63(incl) & 0x1FFFF;
+        for (uint64_t i = max(b, e8) + threadIdx.x; i < e; i += 256) H.add((int)codes[i]);
+        u = ue;
+        hist_flush_row(H, partials + (uint64_t)row * dict);
+        row++;
+    }
+}
 
-        // outlier compaction, index order
-        if (tz) {
-            uint64_t r = cur.out_off + (excl >> 17);
+// Sums each field's partial rows: blockIdx.x picks 256 bins, blockIdx.y a
+// group {field, first row, rows (<= kHistRows)}; one 64-bit atomic per bin
+// and group.
+__global__ void __launch_bounds__(256) k_hist_reduce_multi(const uint32_t* __restrict__ partials, int dict,
+                                                           const uint32_t* __restrict__ groups,
+                                                           unsigned long long* __restrict__ hist) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= dict) return;
+    const uint32_t f = groups[3 * blockIdx.y], r0 = groups[3 * blockIdx.y + 1], nr = groups[3 * blockIdx.y + 2];
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t r = 0; r < nr; r++) s += partials[(uint64_t)(r0 + r) * dict + b];
+    if (s) atomicAdd(&hist[(uint64_t)f * dict + b], (unsigned long long)s);
+}
+
+// Copies each field's staged header (metadata, Huffman header, tree) to the
+// front of its archive; one workgroup per field.
+__global__ void k_place_heads(const BatchField* __restrict__ fd, int nf, const uint32_t* __restrict__ errs) {
+    const int f = blockIdx.x;
+    if (f >= nf || errs[f]) return;
+    const uint8_t* src = fd[f].head;
+    uint8_t* dst = fd[f].archive;
+    for (uint64_t i = threadIdx.x; i < fd[f].head_bytes; i += blockDim.x) dst[i] = src[i];
+}
+
+// Bits and outliers of every chunk of every field (as k_chunk_bits); a unit
+// is four chunks of one field, whose length table the workgroup holds in LDS.
+__global__ void __launch_bounds__(256) k_chunk_bits_multi(const BatchField* __restrict__ fd,
+                                                          const uint64_t* __restrict__ start, int nf, uint64_t nunits,
+                                                          int dict, uint32_t* __restrict__ chunk_bits,
+                                                          uint32_t* __restrict__ chunk_zeros) {
+    __shared__ uint8_t slen[kMaxLdsBook];
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    uint64_t u = nunits * blockIdx.x / gridDim.x;
+    const uint64_t u1 = nunits * (blockIdx.x + 1) / gridDim.x;
+    if (u >= u1) return;
+    for (int f = field_of(start, nf, u); u < u1; f++) {
+        const uint64_t ue = min(u1, start[f + 1]);
+        if (u >= ue) continue;
+        const uint8_t* len_tab = reinterpret_cast<const uint8_t*>(fd[f].book + 2 * dict);
+        __syncthreads();
+        for (int i = threadIdx.x; i < dict; i += blockDim.x) slen[i] = len_tab[i];
+        __syncthreads();
+        const uint16_t* codes = fd[f].codes;
+        const uint64_t n = fd[f].a.n, nch = fd[f].nchunks, c0 = fd[f].chunk0, ub = start[f];
+        for (; u < ue; u++) {
+            const uint64_t k = (u - ub) * 4 + wave;
+            if (k >= nch) continue;
+            uint32_t sym[kSymPerThread];
+            load_syms(codes, n, k * kChunk + lane * kSymPerThread, sym);
+            uint32_t bits = 0, zeros = 0;
 #pragma unroll
             for (int j = 0; j < kSymPerThread; j++) {
-                const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-                if ((uint32_t)j < cur.nvalid && sj == 0) {
-                    const uint32_t bv = __float_as_uint(sparse[base + j]);
-                    uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
-                    dst[0] = (uint8_t)bv;
-                    dst[1] = (uint8_t)(bv >> 8);
-                    dst[2] = (uint8_t)(bv >> 16);
-                    dst[3] = (uint8_t)(bv >> 24);
-                    r++;
+                const uint32_t sj = sym[j];
+                if (sj != 0xFFFFFFFFu) {
+                    bits += slen[sj];
+                    zeros += sj == 0;
                 }
+            }
+            const uint32_t tot = lane63(wave_scan_dpp(bits | (zeros << 17)));
+            if (lane == 0) {
+                chunk_bits[c0 + k] = tot & 0x1FFFF;
+                chunk_zeros[c0 + k] = tot >> 17;
             }
         }
-        if (S.total_bits == 0) continue;  // one-symbol tree: no bitstream
+    }
+}
 
-        const uint64_t Ak = S.stream_bit_off + cur.bit_off;
-        const uint64_t wbase = Ak >> 5;
-        const uint32_t a0 = (uint32_t)(Ak & 31);
-        const uint32_t nwords = (a0 + cbits + 31) >> 5;
-        for (uint32_t i = lane; i < nwords; i += kWave) stage[i] = 0;
-        wave_lds_sync();
+// Per-field sizes from the batch-wide exclusive scan of chunk bits and
+// outliers (a field's totals are the differences of the scan at its first
+// chunk and at the next field's); one thread per field.
+__global__ void k_enc_summary_multi(const BatchField* __restrict__ fd, int nf, const uint64_t* __restrict__ bit_off,
+                                    const uint64_t* __restrict__ out_off, const uint64_t* __restrict__ totals,
+                                    uint64_t ctot, EncSummary* __restrict__ sums, uint32_t* __restrict__ errs) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf || errs[f]) return;
+    const uint64_t c0 = fd[f].chunk0, c1 = c0 + fd[f].nchunks;
+    const uint64_t tb = (c1 < ctot ? bit_off[c1] : totals[0]) - bit_off[c0];
+    const uint64_t tz = (c1 < ctot ? out_off[c1] : totals[1]) - out_off[c0];
+    enc_summary(tb, tz, sums + f, fd[f].archive, fd[f].meta_bytes, 0, fd[f].tree_bytes, fd[f].out_cap, errs + f);
+}
 
-        // Each codeword is ORed into the one or two stage words it overlaps
-        // (codeword at bit offset o of word w, MSB first).  The second OR may
-        // add zero to the word after the chunk's last; the stage has room.
-        uint32_t pos = a0 + (excl & 0x1FFFF);
-        auto place = [&](uint32_t cw, uint32_t l) {  // l <= 32; l = 0 places nothing (cw = 0)
-            const uint32_t o = pos & 31;
-            const uint64_t v = (uint64_t)cw << ((64 - o - l) & 63);
-            uint32_t* w = stage + (pos >> 5);
-            atomicOr(w, (uint32_t)(v >> 32));
-            atomicOr(w + 1, (uint32_t)v);
-            pos += l;
-        };
-        auto place_code = [&](const Code& c) {
-            if (c.len > 32) {
-                place(c.hi, c.len - 32);
-                place(c.lo, 32);
-            } else if (c.len) {
-                place(c.lo, c.len);
-            }
-        };
-        if constexpr (BOOK == kBookSplit) {
-#pragma unroll
-            for (int j = 0; j < kSymPerThread; j++) place(ent[j], elen[j]);  // absent symbols: length 0
-        } else if constexpr (BOOK == kBookPacked) {
-            if (!__any(long_mask != 0)) {
-#pragma unroll
-                for (int j = 0; j < kSymPerThread; j++) place(ent[j] >> 6, ent[j] & 63);  // absent symbols: ent 0
-            } else {
-#pragma unroll
-                for (int j = 0; j < kSymPerThread; j++) {
-                    const uint32_t l = ent[j] & 63;
-                    if (l == kLongCode)
-                        place_code(lookup_global((cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF, cw_hi, cw_lo, len_tab));
-                    else if (l)
-                        place(ent[j] >> 6, l);
-                }
+// Bit packing of every field (as k_pack, without the load pipeline); a unit
+// is four chunks of one field, whose codebook the workgroup holds in LDS.
+template <int BOOK>
+__global__ void __launch_bounds__(256) k_pack_multi(const BatchField* __restrict__ fd,
+                                                    const uint64_t* __restrict__ start, int nf, uint64_t nunits,
+                                                    int dict, const uint64_t* __restrict__ bit_off,
+                                                    const uint64_t* __restrict__ out_off,
+                                                    const EncSummary* __restrict__ sums,
+                                                    const uint32_t* __restrict__ errs, uint32_t stage_words) {
+    static_assert(BOOK != kBookGlobal, "batched pack keeps the codebook in LDS");
+    extern __shared__ uint32_t lds[];
+    uint32_t* tab = lds;
+    uint8_t* tlen = reinterpret_cast<uint8_t*>(lds + (uint32_t)dict + 1);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* stage = lds + book_words(BOOK, dict) + wave * stage_words;
+    const uint32_t lane = lane_id();
+    const uint32_t fill = (uint32_t)dict;
+    uint64_t u = nunits * blockIdx.x / gridDim.x;
+    const uint64_t u1 = nunits * (blockIdx.x + 1) / gridDim.x;
+    if (u >= u1) return;
+    for (int f = field_of(start, nf, u); u < u1; f++) {
+        const uint64_t ue = min(u1, start[f + 1]);
+        if (u >= ue) continue;
+        const uint32_t* cw_hi = fd[f].book;
+        const uint32_t* cw_lo = cw_hi + dict;
+        const uint8_t* len_tab = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
+        __syncthreads();
+        if (BOOK == kBookPacked) {
+            for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+                const uint32_t l = i < dict ? len_tab[i] : 0u;
+                tab[i] = l == 0 ? 0u : l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
             }
         } else {
-#pragma unroll
-            for (int j = 0; j < kSymPerThread; j++) place_code(c[j]);
-        }
-
-        // complete the last word with the next chunk's leading bits
-        const uint32_t e = (a0 + cbits) & 31;
-        if (e != 0 && k + 1 < nchunks) {
-            Code d{0, 0, 0};
-            if (cur.next != 0xFFFFFFFFu) {
-                if constexpr (BOOK == kBookSplit)
-                    d = Code{tab[cur.next], 0, tlen[cur.next]};
-                else
-                    d = lookup<LDS_BOOK>(cur.next, tab, cw_hi, cw_lo, len_tab);
-            }
-            const uint32_t p = wave_scan_dpp(d.len) - d.len;
-            uint32_t t = 0;
-            if (d.len && p < 32) {
-                const uint32_t kk = min(d.len, 32u - p);
-                const uint64_t cw = ((uint64_t)d.hi << 32) | d.lo;
-                t = (uint32_t)((cw >> (d.len - kk)) << (32 - p - kk));
-            }
-            t = lane63(wave_scan_dpp<true>(t));
-            if (lane == 0) atomicOr(&stage[nwords - 1], t >> e);
-        }
-        wave_lds_sync();
-
-        for (uint32_t i = ((a0 != 0 && k != 0) ? 1u : 0u) + lane; i < nwords; i += kWave) {
-            const uint64_t Wg = wbase + i;
-            uint32_t wv = stage[i];
-            if (k == 0 && i == 0 && a0) {
-                // keep the tree bytes that precede the stream in this word
-                const uint32_t mask = ~0u << (32 - a0);
-                wv = (__builtin_bswap32(out32[Wg]) & mask) | (wv & ~mask);
-            }
-            if ((Wg + 1) * 4 > S.stream_end_byte) {
-                uint8_t* bp = archive + Wg * 4;
-                for (int j = 0; j < 4; j++)
-                    if (Wg * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(wv >> (24 - 8 * j));
-            } else {
-                out32[Wg] = __builtin_bswap32(wv);
+            for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+                tab[i] = i < dict ? cw_lo[i] : 0u;
+                tlen[i] = i < dict ? len_tab[i] : 0u;
             }
         }
-        wave_lds_sync();
+        __syncthreads();
+        if (!errs[f]) {
+            const EncSummary S = sums[f];
+            const uint16_t* codes = fd[f].codes;
+            const float* sparse = fd[f].sparse;
+            uint8_t* archive = fd[f].archive;
+            const uint64_t n = fd[f].a.n, nch = fd[f].nchunks, c0 = fd[f].chunk0, ub = start[f];
+            const uint64_t bb = bit_off[c0], zb = out_off[c0];
+            for (uint64_t v = u; v < ue; v++) {
+                const uint64_t k = (v - ub) * 4 + wave;
+                if (k >= nch) continue;
+                ChunkRaw cur;
+                load_chunk(codes, n, nch, bit_off + c0, out_off + c0, zb + S.n_outlier, k, lane, fill, cur);
+                cur.bit_off -= bb;
+                cur.out_off -= zb;
+                pack_chunk<BOOK>(cur, k, nch, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
+            }
+        }
+        u = ue;
     }
 }
 
@@ -1209,6 +1613,78 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
     else
         hipLaunchKernelGGL(k_pack<kBookGlobal>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
 #undef PACK_ARGS
+}
+
+// Batched launches (one per stage for all fields).
+void launch_range_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, hipStream_t st) {
+    hipLaunchKernelGGL(k_range_init_multi, dim3((nf + 63) / 64), dim3(64), 0, st, fd, nf);
+    if (nunits)
+        hipLaunchKernelGGL(k_range_multi, dim3((unsigned)std::min<uint64_t>(nunits, 2048)), dim3(256), 0, st, fd, start,
+                           nf, nunits);
+    hipLaunchKernelGGL(k_ebfam_multi, dim3((nf + 63) / 64), dim3(64), 0, st, fd, nf);
+}
+
+uint64_t range_units(uint64_t n) { return (n + kRangeUnit - 1) / kRangeUnit; }
+
+void launch_dualquant_multi(int B, int kind, const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits,
+                            hipStream_t st) {
+    if (!nunits) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 1024);
+#define DQM(BB, KK)                                                                                              \
+    if (B == BB && kind == KK)                                                                                   \
+        hipLaunchKernelGGL((k_dualquant_multi<BB, KK>), dim3(grid), dim3(256), 0, st, fd, start, nf, nunits);
+#define DQM_B(BB) DQM(BB, kDq1) DQM(BB, kDq2) DQM(BB, kDq2x4) DQM(BB, kDq3)
+    DQM_B(8) DQM_B(16) DQM_B(32) DQM_B(64)
+    DQM(8, kDq3x4) DQM(16, kDq3x4)
+#undef DQM_B
+#undef DQM
+}
+
+uint64_t hist_units(uint64_t n) { return (n + kHistUnit - 1) / kHistUnit; }
+unsigned hist_multi_grid(uint64_t nunits) { return (unsigned)std::min<uint64_t>(nunits, 1024); }
+
+void launch_hist_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
+                       uint32_t* partials, const uint32_t* row0, const uint32_t* groups, uint32_t ngroups,
+                       unsigned long long* hist, hipStream_t st) {
+    if (!nunits) return;
+    hipLaunchKernelGGL(k_hist_codes_multi, dim3(hist_multi_grid(nunits)), dim3(256), dq_lds_bytes(dict), st, fd, start,
+                       nf, nunits, dict, dict / 2, partials, row0);
+    hipLaunchKernelGGL(k_hist_reduce_multi, dim3((dict + 255) / 256, ngroups), dim3(256), 0, st, partials, dict, groups,
+                       hist);
+}
+
+void launch_place_heads(const BatchField* fd, int nf, const uint32_t* errs, hipStream_t st) {
+    if (nf) hipLaunchKernelGGL(k_place_heads, dim3(nf), dim3(256), 0, st, fd, nf, errs);
+}
+
+void launch_chunk_bits_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
+                             uint32_t* cb, uint32_t* cz, hipStream_t st) {
+    if (!nunits) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 2048);
+    hipLaunchKernelGGL(k_chunk_bits_multi, dim3(grid), dim3(256), 0, st, fd, start, nf, nunits, dict, cb, cz);
+}
+
+void launch_enc_summary_multi(const BatchField* fd, int nf, const uint64_t* bit_off, const uint64_t* out_off,
+                              const uint64_t* totals, uint64_t ctot, EncSummary* sums, uint32_t* errs,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(k_enc_summary_multi, dim3((nf + 63) / 64), dim3(64), 0, st, fd, nf, bit_off, out_off, totals,
+                       ctot, sums, errs);
+}
+
+void launch_pack_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict, int max_len,
+                       const uint64_t* bit_off, const uint64_t* out_off, const EncSummary* sums, const uint32_t* errs,
+                       hipStream_t st) {
+    if (!nunits) return;
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 2048);
+    const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
+    const int book = max_len <= 32 ? kBookSplit : kBookPacked;
+    const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;
+    if (book == kBookSplit)
+        hipLaunchKernelGGL(k_pack_multi<kBookSplit>, dim3(grid), dim3(256), lds, st, fd, start, nf, nunits, dict,
+                           bit_off, out_off, sums, errs, stage_words);
+    else
+        hipLaunchKernelGGL(k_pack_multi<kBookPacked>, dim3(grid), dim3(256), lds, st, fd, start, nf, nunits, dict,
+                           bit_off, out_off, sums, errs, stage_words);
 }
 
 }  // namespace vsz

@@ -271,6 +271,11 @@ class Context:
         """Worker contexts of the batched calls (1..16)."""
         self._check(self._lib.vecsz_ctx_set_workers(self._ctx, int(n)))
 
+    def set_batch_fused(self, on: bool):
+        """Fused compress batches (one launch per stage for all fields) when
+        they qualify; False forces the worker-context path."""
+        self._check(self._lib.vecsz_ctx_set_batch_fused(self._ctx, 1 if on else 0))
+
     def compress_bound(self, params: Params) -> int:
         b = ctypes.c_uint64()
         self._check(self._lib.vecsz_compress_bound(ctypes.byref(params), ctypes.byref(b)))
