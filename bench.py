@@ -114,6 +114,25 @@ def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
     }
 
 
+def apply_launch_shape(ctx, args, x, eb, mode):
+    """Launch shapes for the run: given (--launch-shape), none (--no-autotune)
+    or tuned on the field x by vecsz_autotune (archives are the same either
+    way).  Returns what was applied, for the JSON line."""
+    if args.launch_shape:
+        dq, enc = (int(v) for v in args.launch_shape.split(","))
+        ctx.set_launch_shape(dq, enc)
+        return {"dq_grid": dq, "enc_grid": enc, "source": "given"}
+    if args.no_autotune:
+        return {"dq_grid": 768, "enc_grid": 2048, "source": "default"}
+    log("autotuning launch shapes")
+    t = ctx.autotune(x, eb, mode, args.block, args.dict)
+    log("autotune:", t)
+    return {"dq_grid": t["dq_grid"], "enc_grid": t["enc_grid"], "source": "autotune",
+            "dq_ms": round(t["dq_ms"], 4), "enc_ms": round(t["enc_ms"], 4), "fastest_block": t["block"],
+            "ms_by_block": {str(k): round(v, 4) for k, v in t["ms_by_block"].items()},
+            "bytes_by_block": {str(k): v for k, v in t["bytes_by_block"].items()}}
+
+
 def bench_batch(args):
     """96 fields of uneven shape (2D and 3D GRFs, per-field r2r eb): one
     batched call against the same fields compressed by one call each, on one
@@ -134,6 +153,8 @@ def bench_batch(args):
         p = make_params(tuple(reversed(shape)), eb, "r2r", args.block, args.dict)
         params.append(p)
     ctx = Context(0)
+    big = max(range(len(fields)), key=lambda k: fields[k].numel())
+    tune = apply_launch_shape(ctx, args, fields[big], mix[big][1], "r2r")
     caps = [ctx.compress_bound(p) for p in params]
     outs = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
     outs1 = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
@@ -237,6 +258,7 @@ def bench_batch(args):
         "batch_ms_by_workers": {str(k): round(v * 1e3, 3) for k, v in sorted(sweep.items())},
         "batch_speedup": round(tp / tb, 3),
         "batch_worker_contexts_ms": round(tw * 1e3, 3),
+        "launch_shape": tune,
         "identical_archives": bool(same),
         "decompress_gbps": round(total / tdb / 1e9, 2),
         "decompress_per_field_calls_ms": round(tdp * 1e3, 3),
@@ -374,6 +396,9 @@ def main():
     ap.add_argument("--requests", type=int, default=96, help="c5: requests in the stream")
     ap.add_argument("--load", type=float, default=0.7, help="c5: offered load as a fraction of measured peak")
     ap.add_argument("--workers", default="8", help="c4: comma-separated worker counts to time (first is reported)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the launch-shape autotuner (shapes from --launch-shape, else the defaults)")
+    ap.add_argument("--launch-shape", default="", help="DQ_GRID,ENC_GRID: set launch shapes instead of tuning")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain device copy of the field (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -416,6 +441,7 @@ def main():
     value_range = float((x.max() - x.min()).item()) if mode == "r2r" else 0.0
 
     ctx = Context(0)
+    tune = apply_launch_shape(ctx, args, x, args.eb, mode)
     p = make_params(dims, args.eb, mode, args.block, args.dict)
     cap = ctx.compress_bound(p)
     arch = torch.empty(cap, dtype=torch.uint8, device="cuda")
@@ -530,6 +556,7 @@ def main():
         "stages_ms": {"compress": {k: round(v, 4) for k, v in cst.items()},
                       "decompress": {k: round(v, 4) for k, v in dst.items()}},
         "roofline": roofline,
+        "launch_shape": tune,
         "pipeline_hbm_frac": {"compress": round((4 * n + size) / tc / 1e9 / HBM_PEAK_GBS, 4),
                               "decompress": round((4 * n + size) / td / 1e9 / HBM_PEAK_GBS, 4)},
     }

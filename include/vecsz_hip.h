@@ -134,6 +134,34 @@ int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
  * fused path when the batch qualifies.  Both write identical archives. */
 int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
 
+/* GPU launch-shape autotuner; replaces autotune_vector_len /
+ * autotune_block_sizes (src/autotune.hh:80-127, called from
+ * src/vecsz_interface.hh:60-68).  Compresses the field repeatedly (d_out is
+ * scratch of out_cap bytes; it ends holding the B = 64 archive) and:
+ *  - keeps on the context the workgroup caps of the dual quantization and of
+ *    the encode kernels (chunk bits, pack) that ran fastest; every later
+ *    compress on the context, single or batched, uses them, and the archives
+ *    do not depend on them;
+ *  - like the reference's search over B in {8, 16, 32, 64}, times the whole
+ *    compress for each B and reports the fastest as `block`, with each B's
+ *    archive size.  B is a format parameter (decompress needs it), so it is
+ *    reported, not applied: the caller sets params->block and keeps it with
+ *    the archive (the sidecar records it). */
+typedef struct vecsz_tune {
+    int block;                  /* fastest B of {8, 16, 32, 64} */
+    int dq_grid, enc_grid;      /* launch shapes now set on the context */
+    int pad_;
+    double ms_by_block[4];      /* whole compress, B = 8, 16, 32, 64 */
+    uint64_t bytes_by_block[4]; /* archive bytes for each B */
+    double dq_ms, enc_ms;       /* stage times at the chosen shapes */
+} vecsz_tune;
+int vecsz_autotune(vecsz_ctx* ctx, const vecsz_params* params, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                   vecsz_tune* out);
+/* Sets the launch shapes vecsz_autotune chooses (e.g. from a stored result):
+ * workgroup caps of the dual quantization and of the encode kernels, each in
+ * [1, 65536] (defaults 768 and 2048). */
+int vecsz_ctx_set_launch_shape(vecsz_ctx* ctx, int dq_grid, int enc_grid);
+
 /* Served form: a request server on one GPU (no counterpart in the
  * reference).  nslots slot threads, each with its own context (stream and
  * scratch kept resident between requests), take requests from one FIFO

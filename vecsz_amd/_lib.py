@@ -24,7 +24,8 @@ EXPORTS = (
     "vecsz_ctx_create", "vecsz_ctx_destroy", "vecsz_strerror", "vecsz_ctx_last_error", "vecsz_params_default",
     "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
     "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_compress_batch", "vecsz_decompress_batch",
-    "vecsz_ctx_set_workers", "vecsz_ctx_set_batch_fused", "vecsz_server_create", "vecsz_server_destroy", "vecsz_server_submit_compress",
+    "vecsz_ctx_set_workers", "vecsz_ctx_set_batch_fused", "vecsz_autotune", "vecsz_ctx_set_launch_shape",
+    "vecsz_server_create", "vecsz_server_destroy", "vecsz_server_submit_compress",
     "vecsz_server_submit_decompress", "vecsz_server_wait", "vecsz_lossless_bound", "vecsz_lossless_compress",
     "vecsz_lossless_kind", "vecsz_lossless_decompress", "vecsz_sz_pack", "vecsz_sz_unpack", "vecsz_ctx_last_sidecar",
     "vecsz_params_from_sidecar", "vecsz_verify", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
@@ -72,6 +73,19 @@ class Stat(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("minimum", "maximum", "range", "PSNR", "MSE", "NRMSE", "coeff",
                                                "max_abserr_vs_range", "max_pwr_rel_abserr")] + [
         ("len", ctypes.c_uint64), ("max_abserr_index", ctypes.c_uint64), ("max_abserr", ctypes.c_double)]
+
+
+class Tune(ctypes.Structure):
+    _fields_ = [
+        ("block", ctypes.c_int),
+        ("dq_grid", ctypes.c_int),
+        ("enc_grid", ctypes.c_int),
+        ("pad_", ctypes.c_int),
+        ("ms_by_block", ctypes.c_double * 4),
+        ("bytes_by_block", ctypes.c_uint64 * 4),
+        ("dq_ms", ctypes.c_double),
+        ("enc_ms", ctypes.c_double),
+    ]
 
 
 class ServerResult(ctypes.Structure):
@@ -124,6 +138,8 @@ def load():
     lib.vecsz_decompress_batch.argtypes = [vp, ctypes.c_int, P, vp, u64p, vp, u64p, ctypes.POINTER(ctypes.c_int)]
     lib.vecsz_ctx_set_workers.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_set_batch_fused.argtypes = [vp, ctypes.c_int]
+    lib.vecsz_autotune.argtypes = [vp, P, vp, vp, u64, ctypes.POINTER(Tune)]
+    lib.vecsz_ctx_set_launch_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.vecsz_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, u64]
     lib.vecsz_server_destroy.argtypes = [vp]
     lib.vecsz_server_submit_compress.argtypes = [vp, P, vp, vp, u64, u64p]

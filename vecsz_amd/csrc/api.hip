@@ -103,6 +103,9 @@ struct vecsz_ctx {
     std::vector<vecsz_ctx*> workers;
     int nworkers = 8;
     bool fused_batch = true;  // compress batches as one launch per stage when they qualify
+    // launch shapes (workgroup caps of the grid-stride compress kernels), set
+    // by vecsz_autotune; the output does not depend on them
+    int dq_grid = 768, enc_grid = 2048;
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
@@ -385,7 +388,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const bool x4 = pmode == VECSZ_PAD_NONE && dualquant_use_x4(ndim, B, d_in, d0);
     const uint64_t tiles_x = x4 ? (d0 + 255) / 256 : a.ntiles_x;
     const uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : tiles_x * a.ntiles_y * a.ntiles_z;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 768));
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, (uint64_t)c->dq_grid));
     if (!c->partials.ensure((size_t)grid * dict * 4)) return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
 
     HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)dict * 8, c->stream));
@@ -467,7 +470,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     {
         StageScope s(c, "chunk_bits");
         launch_chunk_bits(c->codes.as<uint16_t>(), n, cw_len, dict, c->chunk_bits.as<uint32_t>(),
-                          c->chunk_zeros.as<uint32_t>(), nchunks, c->stream);
+                          c->chunk_zeros.as<uint32_t>(), nchunks, c->enc_grid, c->stream);
     }
     {
         StageScope s(c, "scan");
@@ -481,7 +484,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         StageScope s(c, "pack");
         launch_pack(c->codes.as<uint16_t>(), n, cw_hi, cw_lo, cw_len, dict, c->bit_off.as<uint64_t>(),
                     c->out_off.as<uint64_t>(), nchunks, c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out,
-                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->stream);
+                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->enc_grid, c->stream);
     }
     HIPCHK(c, hipGetLastError());
     if (!c->pin2.ensure(256)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
@@ -967,8 +970,11 @@ static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, 
     }
     // size every worker's field-proportional scratch for the largest field
     // up front: growing it later means hipFree, which stalls all streams
-    for (int t = 0; t < nw; t++)
+    for (int t = 0; t < nw; t++) {
+        c->workers[t]->dq_grid = c->dq_grid;
+        c->workers[t]->enc_grid = c->enc_grid;
         if (!prep(c->workers[t])) return fail(c, VECSZ_ENOMEM, "batch worker scratch");
+    }
     std::vector<int> order(nf);
     for (int k = 0; k < nf; k++) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return weight[a] > weight[b]; });
@@ -1159,7 +1165,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         const uint64_t nu = dqk[(size_t)kind * (nf + 1) + nf];
         if (!nu) continue;
         StageScope s(c, kDqStage[kind]);
-        launch_dualquant_multi(B, kind, fdd, nf, dqk_d + (size_t)kind * (nf + 1), nu, c->stream);
+        launch_dualquant_multi(B, kind, fdd, nf, dqk_d + (size_t)kind * (nf + 1), nu, c->dq_grid, c->stream);
     }
     {
         StageScope s(c, "histogram");
@@ -1280,7 +1286,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         {
             StageScope s(c, "chunk_bits");
             launch_chunk_bits_multi(fdd, nf, ck0_d, ck0[nf], dict, c->chunk_bits.as<uint32_t>(),
-                                    c->chunk_zeros.as<uint32_t>(), c->stream);
+                                    c->chunk_zeros.as<uint32_t>(), c->enc_grid, c->stream);
         }
         {
             StageScope s(c, "scan");
@@ -1293,7 +1299,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         {
             StageScope s(c, "pack");
             launch_pack_multi(fdd, nf, ck0_d, ck0[nf], dict, max_len, c->bit_off.as<uint64_t>(),
-                              c->out_off.as<uint64_t>(), c->b_sums.as<EncSummary>(), errs_d, c->stream);
+                              c->out_off.as<uint64_t>(), c->b_sums.as<EncSummary>(), errs_d, c->enc_grid, c->stream);
         }
         HIPCHK(c, hipGetLastError());
         EncSummary* sums_h = reinterpret_cast<EncSummary*>(c->b_pin_hist.as<uint8_t>());
@@ -1333,6 +1339,85 @@ int vecsz_ctx_set_batch_fused(vecsz_ctx* c, int on) {
     if (!c) return VECSZ_EINVAL;
     c->fused_batch = on != 0;
     return VECSZ_OK;
+}
+
+int vecsz_ctx_set_launch_shape(vecsz_ctx* c, int dq_grid, int enc_grid) {
+    if (!c) return VECSZ_EINVAL;
+    if (dq_grid < 1 || dq_grid > 65536 || enc_grid < 1 || enc_grid > 65536)
+        return fail(c, VECSZ_EINVAL, "launch shape out of range");
+    c->dq_grid = dq_grid;
+    c->enc_grid = enc_grid;
+    return VECSZ_OK;
+}
+
+int vecsz_autotune(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                   vecsz_tune* t) {
+    if (!c) return VECSZ_EINVAL;
+    if (!p || !d_in || !d_out || !t) return fail(c, VECSZ_EINVAL, "NULL argument");
+    int rc = check_params(c, p, true);
+    if (rc) return rc;
+    memset(t, 0, sizeof(*t));
+    const bool was_timing = c->timing;
+    c->timing = true;
+    uint64_t bytes = 0, nout = 0;
+    // best (smallest) time of two runs of the named stages
+    auto stage_ms = [&](const vecsz_params* q, const char* a, const char* b, double* ms) -> int {
+        *ms = 1e30;
+        for (int rep = 0; rep < 2; rep++) {
+            const int r = compress_device(c, q, d_in, d_out, out_cap, &bytes, &nout);
+            if (r) return r;
+            double sum = 0;
+            for (size_t i = 0; i < c->tnames.size(); i++)
+                if (c->tnames[i] == a || (b && c->tnames[i] == b)) sum += c->tms[i];
+            *ms = std::min(*ms, sum);
+        }
+        return VECSZ_OK;
+    };
+    do {
+        if ((rc = compress_device(c, p, d_in, d_out, out_cap, &bytes, &nout))) break;  // warm-up
+        static const int kDqGrid[] = {256, 512, 768, 1024, 1536, 2048, 4096};
+        static const int kEncGrid[] = {512, 1024, 2048, 4096, 8192};
+        double best = 1e30, ms = 0;
+        int pick = c->dq_grid;
+        for (int g : kDqGrid) {
+            c->dq_grid = g;
+            if ((rc = stage_ms(p, "dualquant", nullptr, &ms))) break;
+            if (ms < best) best = ms, pick = g;
+        }
+        if (rc) break;
+        c->dq_grid = pick;
+        t->dq_grid = pick;
+        t->dq_ms = best;
+        best = 1e30;
+        pick = c->enc_grid;
+        for (int g : kEncGrid) {
+            c->enc_grid = g;
+            if ((rc = stage_ms(p, "chunk_bits", "pack", &ms))) break;
+            if (ms < best) best = ms, pick = g;
+        }
+        if (rc) break;
+        c->enc_grid = pick;
+        t->enc_grid = pick;
+        t->enc_ms = best;
+        // block sizes: whole-compress wall time (best of two) and archive size
+        vecsz_params q = *p;
+        static const int kBlocks[4] = {8, 16, 32, 64};
+        double bt = 1e30;
+        for (int i = 0; i < 4 && !rc; i++) {
+            q.block = kBlocks[i];
+            double m = 1e30;
+            for (int rep = 0; rep < 2 && !rc; rep++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = compress_device(c, &q, d_in, d_out, out_cap, &bytes, &nout);
+                m = std::min(m, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            }
+            t->ms_by_block[i] = m;
+            t->bytes_by_block[i] = bytes;
+            if (m < bt) bt = m, t->block = kBlocks[i];
+        }
+    } while (false);
+    c->timing = was_timing;
+    return rc;
 }
 
 int vecsz_compress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, const float* const* d_in,

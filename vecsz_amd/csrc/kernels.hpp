@@ -15,13 +15,15 @@ void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hip
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
                       uint32_t* partials, const DQArgs& a, int grid, hipStream_t st);
 bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0);
+// grid_cap: most workgroups of a launch (a tuned launch shape; the output
+// does not depend on it).
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
-                       uint64_t nchunks, hipStream_t st);
+                       uint64_t nchunks, int grid_cap, hipStream_t st);
 void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
                         uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st);
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
-                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st);
+                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap, hipStream_t st);
 
 // Batched compress (kernels_compress.hip): every field of a batch per launch.
 // start arrays hold nf + 1 unit offsets (see BatchField).
@@ -30,7 +32,7 @@ void launch_range_multi(const BatchField* fd, int nf, const uint64_t* start, uin
 // Dual quantization of the fields whose tile form is `kind` (DqKind); start
 // gives units only to those fields.
 void launch_dualquant_multi(int B, int kind, const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits,
-                            hipStream_t st);
+                            int grid_cap, hipStream_t st);
 // Histograms of the codes: workgroup g of hist_multi_grid(U) takes units
 // [U g / G, U (g + 1) / G) and writes one partial row per field it visits,
 // from row0[g] on; groups: {field, first row, rows <= 32} triples.
@@ -41,12 +43,12 @@ void launch_hist_multi(const BatchField* fd, int nf, const uint64_t* start, uint
                        unsigned long long* hist, hipStream_t st);
 void launch_place_heads(const BatchField* fd, int nf, const uint32_t* errs, hipStream_t st);
 void launch_chunk_bits_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
-                             uint32_t* cb, uint32_t* cz, hipStream_t st);
+                             uint32_t* cb, uint32_t* cz, int grid_cap, hipStream_t st);
 void launch_enc_summary_multi(const BatchField* fd, int nf, const uint64_t* bit_off, const uint64_t* out_off,
                               const uint64_t* totals, uint64_t ctot, EncSummary* sums, uint32_t* errs, hipStream_t st);
 void launch_pack_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict, int max_len,
                        const uint64_t* bit_off, const uint64_t* out_off, const EncSummary* sums, const uint32_t* errs,
-                       hipStream_t st);
+                       int grid_cap, hipStream_t st);
 
 // Decoder (kernels_decompress.hip)
 struct DecodeArgs {

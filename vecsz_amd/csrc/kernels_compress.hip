@@ -1582,8 +1582,8 @@ launched:
 }
 
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
-                       uint64_t nchunks, hipStream_t st) {
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, 2048));
+                       uint64_t nchunks, int grid_cap, hipStream_t st) {
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, grid_cap));
     if (dict <= kMaxLdsLenTab)
         hipLaunchKernelGGL(k_chunk_bits<true>, dim3(grid), dim3(256), 0, st, codes, n, nchunks, len_tab, dict, cb, cz);
     else
@@ -1598,8 +1598,8 @@ void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archiv
 
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
-                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, hipStream_t st) {
-    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, 2048));
+                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap, hipStream_t st) {
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, grid_cap));
     // words one chunk can touch: 1024 codewords of at most max_len bits, plus
     // the partial words at either end
     const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
@@ -1627,9 +1627,9 @@ void launch_range_multi(const BatchField* fd, int nf, const uint64_t* start, uin
 uint64_t range_units(uint64_t n) { return (n + kRangeUnit - 1) / kRangeUnit; }
 
 void launch_dualquant_multi(int B, int kind, const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits,
-                            hipStream_t st) {
+                            int grid_cap, hipStream_t st) {
     if (!nunits) return;
-    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 1024);
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, grid_cap);
 #define DQM(BB, KK)                                                                                              \
     if (B == BB && kind == KK)                                                                                   \
         hipLaunchKernelGGL((k_dualquant_multi<BB, KK>), dim3(grid), dim3(256), 0, st, fd, start, nf, nunits);
@@ -1658,9 +1658,9 @@ void launch_place_heads(const BatchField* fd, int nf, const uint32_t* errs, hipS
 }
 
 void launch_chunk_bits_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict,
-                             uint32_t* cb, uint32_t* cz, hipStream_t st) {
+                             uint32_t* cb, uint32_t* cz, int grid_cap, hipStream_t st) {
     if (!nunits) return;
-    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 2048);
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, grid_cap);
     hipLaunchKernelGGL(k_chunk_bits_multi, dim3(grid), dim3(256), 0, st, fd, start, nf, nunits, dict, cb, cz);
 }
 
@@ -1673,9 +1673,9 @@ void launch_enc_summary_multi(const BatchField* fd, int nf, const uint64_t* bit_
 
 void launch_pack_multi(const BatchField* fd, int nf, const uint64_t* start, uint64_t nunits, int dict, int max_len,
                        const uint64_t* bit_off, const uint64_t* out_off, const EncSummary* sums, const uint32_t* errs,
-                       hipStream_t st) {
+                       int grid_cap, hipStream_t st) {
     if (!nunits) return;
-    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, 2048);
+    const unsigned grid = (unsigned)std::min<uint64_t>(nunits, grid_cap);
     const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
     const int book = max_len <= 32 ? kBookSplit : kBookPacked;
     const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;

@@ -271,6 +271,33 @@ class Context:
         """Worker contexts of the batched calls (1..16)."""
         self._check(self._lib.vecsz_ctx_set_workers(self._ctx, int(n)))
 
+    def autotune(self, x, eb: float, mode="abs", block: int = 8, dict_size: int = 4096, value_range: float = 0.0):
+        """Launch-shape autotuner (src/autotune.hh:80-127 extended to GPU
+        launch shapes) on a float32 CUDA tensor: keeps the fastest dual-quant
+        and encode workgroup caps on this context and reports, per Lorenzo
+        block size, the whole-compress time and archive size (B itself is
+        not applied)."""
+        if not x.is_cuda or x.dtype != torch.float32:
+            raise TypeError("expected a float32 CUDA tensor")
+        x = x.contiguous()
+        p = make_params(dims_of(x.shape), eb, mode, block, dict_size, value_range)
+        q = Params()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(p))
+        q.block = 64
+        cap = max(self.compress_bound(p), self.compress_bound(q))
+        out = torch.empty(cap, dtype=torch.uint8, device=x.device)
+        t = _lib.Tune()
+        self._check(self._lib.vecsz_autotune(self._ctx, ctypes.byref(p), ctypes.c_void_p(x.data_ptr()),
+                                             ctypes.c_void_p(out.data_ptr()), cap, ctypes.byref(t)))
+        return {"block": t.block, "dq_grid": t.dq_grid, "enc_grid": t.enc_grid, "dq_ms": t.dq_ms,
+                "enc_ms": t.enc_ms, "ms_by_block": dict(zip((8, 16, 32, 64), list(t.ms_by_block))),
+                "bytes_by_block": dict(zip((8, 16, 32, 64), [int(b) for b in t.bytes_by_block]))}
+
+    def set_launch_shape(self, dq_grid: int, enc_grid: int):
+        """Workgroup caps of the dual-quantization and encode kernels (what
+        autotune() picks); archives do not depend on them."""
+        self._check(self._lib.vecsz_ctx_set_launch_shape(self._ctx, int(dq_grid), int(enc_grid)))
+
     def set_batch_fused(self, on: bool):
         """Fused compress batches (one launch per stage for all fields) when
         they qualify; False forces the worker-context path."""
