@@ -38,7 +38,8 @@ def stage_avg(ctx, fn, reps):
 
 
 def dq_kernel_name(ndim, block):
-    return {3: f"k_dualquant_3d_x4<{block}>", 2: "k_dualquant_2d_x4", 1: "k_dualquant_1d"}[ndim]
+    return {3: f"k_dualquant_3d_x4<{block}, true>", 2: f"k_dualquant_2d_x4<{block}, true>",
+            1: f"k_dualquant_1d<{block}, false>"}[ndim]
 
 
 def measured_traffic(kernel, dims, eb):

@@ -16,7 +16,7 @@ def _mix(seed=0):
 
     rng = np.random.default_rng(seed)
     shapes = [(5000,), (1 << 16,), (3, 7), (64, 64), (130, 257), (512, 384), (8, 8, 8), (32, 32, 32),
-              (17, 33, 65), (48, 64, 96), (64, 96, 128), (1,), (2, 3, 4), (1000, 4)]
+              (17, 33, 65), (48, 64, 96), (64, 96, 128), (1,), (2, 3, 4), (1000, 4), (9, 20, 131)]
     fields, ebs, modes = [], [], []
     for i, s in enumerate(shapes):
         if len(s) >= 2 and min(s) >= 8:

@@ -68,10 +68,16 @@ def test_reference_shaped_interface(golden):
     assert np.array_equal(y.view(np.uint32), c["out"].view(np.uint32))
 
 
-def _check_vs_oracle(ctx, x, eb, r2r, block=8, dict_size=4096, tag=""):
-    _say("oracle-compare", tag, x.shape, eb, r2r, block)
+def _check_vs_oracle(ctx, x, eb, r2r, block=8, dict_size=4096, tag="", offset=0):
+    """offset: the device field starts `offset` floats into its allocation
+    (offset 1: not 16-byte aligned)."""
+    _say("oracle-compare", tag, x.shape, eb, r2r, block, offset)
     a_orc, n_orc, _ = po.compress(x, eb, r2r, dict_size, block)
     xt = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    if offset:
+        buf = torch.zeros(x.size + offset, dtype=torch.float32, device="cuda")
+        buf[offset:] = xt.ravel()
+        xt = buf[offset:].view(x.shape)
     a, nout = ctx.compress(xt, eb, "r2r" if r2r else "abs", block, dict_size)
     ab = a.cpu().numpy().tobytes()
     assert nout == n_orc, tag
@@ -149,6 +155,32 @@ def test_four_per_lane_reconstruction(gpu_ctx, shape, blocks):
     for B in blocks:
         _check_vs_oracle(gpu_ctx, x, 1e-3, False, block=B, tag=f"x4 B{B}{shape}")
         _check_vs_oracle(gpu_ctx, x, 1e-4, True, block=B, tag=f"x4 r2r B{B}{shape}")
+
+
+@pytest.mark.parametrize("shape,blocks,offset", [((9, 20, 131), (8, 16), 0), ((7, 13, 257), (8,), 0),
+                                                 ((12, 10, 136), (8, 16), 1), ((40, 129), (8, 16, 32, 64), 0),
+                                                 ((30, 256), (8, 64), 1)])
+def test_unaligned_four_wide_tiles(gpu_ctx, shape, blocks, offset):
+    """Fields at least 128 wide whose x extent is not a multiple of 4, or that
+    start off a 16-byte boundary, take the 4-wide dual-quant tiles with
+    element loads in a batch (a single field this small takes the 64-wide
+    tiles); the partial x group (3D scalar forms, 2D lost columns) falls
+    inside the last lane group.  Both forms against the oracle."""
+    rng = np.random.default_rng(sum(shape) + offset)
+    x = np.cumsum(rng.normal(0, 0.05, int(np.prod(shape)))).astype(np.float32)
+    x[rng.integers(0, x.size, x.size // 100)] += 300.0
+    x = x.reshape(shape)
+    other = torch.from_numpy(np.cumsum(rng.normal(0, 0.05, 4000)).astype(np.float32)).cuda()
+    for B in blocks:
+        for eb, r2r in ((1e-3, False), (1e-4, True)):
+            _check_vs_oracle(gpu_ctx, x, eb, r2r, block=B, tag=f"x4u B{B}{shape}", offset=offset)
+            a_orc, n_orc, _ = po.compress(x, eb, r2r, 4096, B)
+            buf = torch.zeros(x.size + offset, dtype=torch.float32, device="cuda")
+            buf[offset:] = torch.from_numpy(x.ravel()).cuda()
+            xt = buf[offset:].view(x.shape)
+            mode = "r2r" if r2r else "abs"
+            (a, nout), _ = gpu_ctx.compress_batch([xt, other], [eb, 1e-3], [mode, "abs"], block=B)
+            assert a.cpu().numpy().tobytes() == a_orc and nout == n_orc, (shape, B, r2r, offset)
 
 
 def test_c1_sinusoids_full_size(gpu_ctx):

@@ -385,7 +385,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         }
         a.halo = c->halo.as<float>();
     }
-    const bool x4 = pmode == VECSZ_PAD_NONE && dualquant_use_x4(ndim, B, d_in, d0);
+    const bool x4 = pmode == VECSZ_PAD_NONE && dualquant_use_x4(ndim, B, d_in, d0, d1, d2, false);
     const uint64_t tiles_x = x4 ? (d0 + 255) / 256 : a.ntiles_x;
     const uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : tiles_x * a.ntiles_y * a.ntiles_z;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, (uint64_t)c->dq_grid));
@@ -1036,7 +1036,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
     }
     // unit starts: range, chunks (the histogram pass borrows them first), then
     // one dual-quant array per tile form
-    const size_t start_bytes = (2 + 5) * (size_t)(nf + 1) * 8;
+    const size_t start_bytes = (2 + kDqKinds) * (size_t)(nf + 1) * 8;
     if (!c->codes.ensure(ntot * 2 + 64) || !c->sparse.ensure(ntot * 4 + 64) ||
         !c->b_hist.ensure((size_t)nf * dict * 8) || !c->b_range.ensure((size_t)nf * 8) ||
         !c->b_ebd.ensure((size_t)nf * sizeof(EbDev)) || !c->b_desc.ensure((size_t)nf * sizeof(BatchField)) ||
@@ -1049,10 +1049,10 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
     // descriptors and unit starts are built here and copied into pinned memory
     // for each upload (the pinned block may be reallocated in between)
     std::vector<BatchField> fd(nf);
-    std::vector<uint64_t> st(7 * (size_t)(nf + 1), 0);
+    std::vector<uint64_t> st((2 + kDqKinds) * (size_t)(nf + 1), 0);
     uint64_t* rg0 = st.data();
     uint64_t* ck0 = rg0 + (nf + 1);
-    uint64_t* dqk = rg0 + 2 * (nf + 1);  // dqk + kind * (nf + 1), kinds kDq1 .. kDq3x4
+    uint64_t* dqk = rg0 + 2 * (nf + 1);  // dqk + kind * (nf + 1) for each DqKind
     std::vector<char> dev_range(nf, 0);
     BatchField* fdd = c->b_desc.as<BatchField>();
     for (int k = 0; k < nf; k++) {
@@ -1078,7 +1078,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         a.halo = nullptr;
         a.hb0 = ref_nblk(d0, B);
         a.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
-        const bool x4 = dualquant_use_x4(ndim, B, d_in[k], d0);
+        const bool x4 = dualquant_use_x4(ndim, B, d_in[k], d0, d1, d2, true);
         a.ntiles_x = x4 ? (d0 + 255) / 256 : (d0 + 63) / 64;
         a.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
         a.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
@@ -1097,13 +1097,16 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         f.sparse = c->sparse.as<float>() + off[k];
         f.hist = c->b_hist.as<unsigned long long>() + (size_t)k * dict;
         f.eb = q.eb;
-        f.kind = ndim == 1 ? kDq1 : ndim == 2 ? (x4 ? kDq2x4 : kDq2) : (x4 ? kDq3x4 : kDq3);
+        const bool aln = dualquant_x4_aligned(d_in[k], d0);
+        f.kind = ndim == 1 ? kDq1
+                 : ndim == 2 ? (x4 ? (aln ? kDq2x4 : kDq2x4u) : kDq2)
+                             : (x4 ? (aln ? kDq3x4 : kDq3x4u) : kDq3);
         f.dq_tiles = ndim == 1 ? ((nel[k] + 7) / 8 + 63) / 64 : a.ntiles_x * a.ntiles_y * a.ntiles_z;
         f.archive = d_out[k];
         f.out_cap = out_cap[k];
         f.meta_bytes = meta_bytes(ndim);
         rg0[k + 1] = rg0[k] + (dev_range[k] ? range_units(nel[k]) : 0);
-        for (int kind = kDq1; kind <= kDq3x4; kind++) {
+        for (int kind = 0; kind < kDqKinds; kind++) {
             uint64_t* s0 = dqk + (size_t)kind * (nf + 1);
             s0[k + 1] = s0[k] + (kind == f.kind ? (f.dq_tiles + 3) / 4 : 0);
         }
@@ -1159,9 +1162,10 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         launch_range_multi(fdd, nf, rg0_d, rg0[nf], c->stream);
     }
     HIPCHK(c, hipMemsetAsync(c->b_hist.p, 0, (size_t)nf * dict * 8, c->stream));
-    static const char* const kDqStage[] = {"dualquant_1d", "dualquant_2d", "dualquant_2d_x4", "dualquant_3d",
-                                           "dualquant_3d_x4"};
-    for (int kind = kDq1; kind <= kDq3x4; kind++) {
+    static const char* const kDqStage[kDqKinds] = {"dualquant_1d", "dualquant_2d",     "dualquant_2d_x4",
+                                                   "dualquant_3d", "dualquant_3d_x4", "dualquant_2d_x4u",
+                                                   "dualquant_3d_x4u"};
+    for (int kind = 0; kind < kDqKinds; kind++) {
         const uint64_t nu = dqk[(size_t)kind * (nf + 1) + nf];
         if (!nu) continue;
         StageScope s(c, kDqStage[kind]);

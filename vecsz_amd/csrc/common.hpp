@@ -74,7 +74,8 @@ struct EncSummary {
 // One field of a fused batch compress.  A device array of these drives the
 // batched kernels; each kernel walks a queue of work units that spans every
 // field (unit u belongs to field f when start[f] <= u < start[f + 1]).
-enum DqKind { kDq1 = 0, kDq2 = 1, kDq2x4 = 2, kDq3 = 3, kDq3x4 = 4 };
+// x4 forms: 4 elements per lane, 16-byte loads (x4) or element loads (x4u).
+enum DqKind { kDq1 = 0, kDq2 = 1, kDq2x4 = 2, kDq3 = 3, kDq3x4 = 4, kDq2x4u = 5, kDq3x4u = 6, kDqKinds = 7 };
 struct BatchField {
     DQArgs a;                   // dual-quant scalars (a.ebd = ebd when the range is found on the device)
     const float* in;

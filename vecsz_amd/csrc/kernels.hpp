@@ -14,7 +14,12 @@ void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st);
 void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hipStream_t st);
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
                       uint32_t* partials, const DQArgs& a, int grid, hipStream_t st);
-bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0);
+// 4-wide tiles (256 x per wavefront) for 2D/3D fields at least 128 wide;
+// aligned: DIM0 % 4 == 0 and a 16-byte aligned field (vector loads/stores).
+// A single field takes the element-load form only when it has enough
+// 4-wide tiles to fill the chip; a batch (tiles of many fields) always does.
+bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0, uint64_t d1, uint64_t d2, bool batched);
+bool dualquant_x4_aligned(const float* in, uint64_t d0);
 // grid_cap: most workgroups of a launch (a tuned launch shape; the output
 // does not depend on it).
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,

@@ -413,20 +413,52 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
 // ------------------------------------------------------------------ 4-wide tiles
 
 // Same per-element arithmetic as the kernels above, with each lane owning 4
-// consecutive x (one float4 load, one 8-byte code store), so a wavefront covers
-// 256 x and moves 1 KiB per row.  Used when DIM0 % 4 == 0 and the field is
-// 16-byte aligned.  Since 8*floor(DIM0/8) is a multiple of 4, the 4 elements of
-// a lane all take the same prequant/postquant form, and only element 0 can be
-// a block start.
+// consecutive x, so a wavefront covers 256 x and moves 1 KiB per row.  ALN:
+// DIM0 % 4 == 0 and the field is 16-byte aligned, so a lane's 4 elements are
+// one float4 load and one 8-byte code store; otherwise each element is loaded
+// and stored on its own and the lane holding x = DIM0 - 1 has fewer than 4 in
+// the field (the rest read 0 and are neither stored nor counted; they are no
+// element's x-1 neighbour).  Since 8*floor(DIM0/8) is a multiple of 4, the 4
+// elements of a lane all take the same prequant/postquant form, and only
+// element 0 can be a block start.
+template <bool ALN>
+__device__ __forceinline__ float4 load_x4(const float* __restrict__ p, uint32_t nv) {
+    if constexpr (ALN) {
+        return *reinterpret_cast<const float4*>(p);
+    } else {
+        float4 v = make_float4(p[0], 0.f, 0.f, 0.f);
+        if (nv > 1) v.y = p[1];
+        if (nv > 2) v.z = p[2];
+        if (nv > 3) v.w = p[3];
+        return v;
+    }
+}
+template <bool ALN>
+__device__ __forceinline__ void store_x4(uint16_t* __restrict__ p, const uint32_t* c, uint32_t nv) {
+    if constexpr (ALN) {
+        ushort4 cv;
+        cv.x = (uint16_t)c[0];
+        cv.y = (uint16_t)c[1];
+        cv.z = (uint16_t)c[2];
+        cv.w = (uint16_t)c[3];
+        *reinterpret_cast<ushort4*>(p) = cv;
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (k < nv) p[k] = (uint16_t)c[k];
+    }
+}
+
 // One 256 x B x B tile.  SCAL is true only for tiles that contain lanes of
 // the partial x-group (x >= 8*floor(DIM0/8)); every other tile runs the AVX
 // forms alone, without evaluating the scalar ones.
-template <int B, bool SCAL, class Hist>
+template <int B, bool SCAL, bool ALN, class Hist>
 __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                             float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t x,
                                             uint64_t y0, uint64_t z0) {
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
     const bool xin = x < a.d0;
+    const uint32_t nv = ALN ? 4u : (xin ? (uint32_t)min<uint64_t>(4, a.d0 - x) : 0u);
     const bool scal = SCAL && x >= a.x8;
     const bool xstart = (x % B) == 0;
     const float thr = scal ? a.thr_scalar : a.thr_vec;
@@ -441,7 +473,7 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
     float4 vnext[B];
 #pragma unroll
     for (int yy = 0; yy < B; yy++)
-        vnext[yy] = (xin && yy < ny) ? *reinterpret_cast<const float4*>(in + x + (y0 + yy) * sy + z0 * sz)
+        vnext[yy] = (xin && yy < ny) ? load_x4<ALN>(in + x + (y0 + yy) * sy + z0 * sz, nv)
                                      : make_float4(0.f, 0.f, 0.f, 0.f);
     for (int zz = 0; zz < nz; zz++) {
         const uint64_t z = z0 + zz;
@@ -451,9 +483,8 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
         if (zz + 1 < nz) {
 #pragma unroll
             for (int yy = 0; yy < B; yy++)
-                vnext[yy] = (xin && yy < ny)
-                                ? *reinterpret_cast<const float4*>(in + x + (y0 + yy) * sy + (z + 1) * sz)
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
+                vnext[yy] = (xin && yy < ny) ? load_x4<ALN>(in + x + (y0 + yy) * sy + (z + 1) * sz, nv)
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         float qr[4] = {0.f, 0.f, 0.f, 0.f}, xr[4] = {0.f, 0.f, 0.f, 0.f};
         float zr[4] = {0.f, 0.f, 0.f, 0.f}, xzr[4] = {0.f, 0.f, 0.f, 0.f};
@@ -498,16 +529,13 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
                     c[k] = postquant(q[k], pred, thr, a.radius_f);
                 }
                 if (xin) {
-                    ushort4 cv;
-                    cv.x = (uint16_t)c[0];
-                    cv.y = (uint16_t)c[1];
-                    cv.z = (uint16_t)c[2];
-                    cv.w = (uint16_t)c[3];
-                    *reinterpret_cast<ushort4*>(codes + idx) = cv;
+                    store_x4<ALN>(codes + idx, c, nv);
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        if (c[k] == 0) sparse[idx + k] = q[k];
-                        H.add((int)c[k]);
+                        if (ALN || (uint32_t)k < nv) {
+                            if (c[k] == 0) sparse[idx + k] = q[k];
+                            H.add((int)c[k]);
+                        }
                     }
                 }
 #pragma unroll
@@ -524,7 +552,7 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
 }
 
 // Tile t of the 256 x B x B grid.
-template <int B, class Hist>
+template <int B, bool ALN, class Hist>
 __device__ __forceinline__ void dq3_x4_at(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                           float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
                                           uint32_t lane) {
@@ -534,12 +562,12 @@ __device__ __forceinline__ void dq3_x4_at(const float* __restrict__ in, uint16_t
     const uint64_t x = tx * 256 + 4 * lane;
     // the partial x-group only exists in the last tile column
     if (tx * 256 + 256 > a.x8 && a.x8 < a.d0)
-        dq3_x4_tile<B, true>(in, codes, sparse, H, a, x, ty * B, tz * B);
+        dq3_x4_tile<B, true, ALN>(in, codes, sparse, H, a, x, ty * B, tz * B);
     else
-        dq3_x4_tile<B, false>(in, codes, sparse, H, a, x, ty * B, tz * B);
+        dq3_x4_tile<B, false, ALN>(in, codes, sparse, H, a, x, ty * B, tz * B);
 }
 
-template <int B>
+template <int B, bool ALN>
 __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                          float* __restrict__ sparse,
                                                          unsigned long long* __restrict__ hist,
@@ -553,11 +581,11 @@ __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
     for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq3_x4_at<B>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+        dq3_x4_at<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
-template <int B, class Hist>
+template <int B, bool ALN, class Hist>
 __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                             float* __restrict__ sparse, Hist& H, const DQArgs& a, uint64_t t,
                                             uint32_t lane) {
@@ -567,6 +595,7 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
         const uint64_t x = tx * 256 + 4 * lane;
         const uint64_t y0 = ty * B;
         const bool xin = x < a.d0;
+        const uint32_t nv = ALN ? 4u : (xin ? (uint32_t)min<uint64_t>(4, a.d0 - x) : 0u);
         const bool lost = x >= a.x8;
         const bool xstart = (x % B) == 0;
         float qr[4] = {0.f, 0.f, 0.f, 0.f}, xr[4] = {0.f, 0.f, 0.f, 0.f};
@@ -574,7 +603,7 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
             const uint64_t y = y0 + yy;
             if (y >= a.d1) break;
             const uint64_t idx = x + y * a.d0;
-            float4 v = xin ? *reinterpret_cast<const float4*>(in + idx) : make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 v = xin ? load_x4<ALN>(in + idx, nv) : make_float4(0.f, 0.f, 0.f, 0.f);
             float q[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int k = 0; k < 4; k++) q[k] = prequant_vec(q[k], a.r_f);
@@ -592,16 +621,13 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
                 c[k] = lost ? 0u : postquant(q[k], pred, a.thr_vec, a.radius_f);
             }
             if (xin) {
-                ushort4 cv;
-                cv.x = (uint16_t)c[0];
-                cv.y = (uint16_t)c[1];
-                cv.z = (uint16_t)c[2];
-                cv.w = (uint16_t)c[3];
-                *reinterpret_cast<ushort4*>(codes + idx) = cv;
+                store_x4<ALN>(codes + idx, c, nv);
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
-                    if (c[k] == 0) sparse[idx + k] = lost ? 0.f : q[k];
-                    H.add((int)c[k]);
+                    if (ALN || (uint32_t)k < nv) {
+                        if (c[k] == 0) sparse[idx + k] = lost ? 0.f : q[k];
+                        H.add((int)c[k]);
+                    }
                 }
             }
 #pragma unroll
@@ -613,7 +639,7 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
     }
 }
 
-template <int B>
+template <int B, bool ALN>
 __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                          float* __restrict__ sparse,
                                                          unsigned long long* __restrict__ hist,
@@ -627,7 +653,7 @@ __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
     for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq2_x4_tile<B>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+        dq2_x4_tile<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
     hist_flush(H, partials);
 }
 
@@ -1288,12 +1314,12 @@ __global__ void __launch_bounds__(256) k_dualquant_multi(const BatchField* __res
         }
         const uint64_t t = (u - ub) * 4 + wave;
         if (t >= ntiles) continue;
-        if constexpr (KIND == kDq3x4) {
-            if constexpr (B <= 16) dq3_x4_at<B>(in, codes, sparse, H, a, t, lane);
+        if constexpr (KIND == kDq3x4 || KIND == kDq3x4u) {
+            if constexpr (B <= 16) dq3_x4_at<B, KIND == kDq3x4>(in, codes, sparse, H, a, t, lane);
         } else if constexpr (KIND == kDq3) {
             dq3_tile<B, false>(in, codes, sparse, H, a, t, lane);
-        } else if constexpr (KIND == kDq2x4) {
-            dq2_x4_tile<B>(in, codes, sparse, H, a, t, lane);
+        } else if constexpr (KIND == kDq2x4 || KIND == kDq2x4u) {
+            dq2_x4_tile<B, KIND == kDq2x4>(in, codes, sparse, H, a, t, lane);
         } else if constexpr (KIND == kDq2) {
             dq2_tile<B, false>(in, codes, sparse, H, a, t, lane);
         } else {
@@ -1513,33 +1539,51 @@ void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hip
     hipLaunchKernelGGL(k_ebfam, dim3(1), dim3(64), 0, st, mm, x, eb, out);
 }
 
-bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0) {
-    if (ndim == 1) return false;
-    if ((d0 % 4) != 0 || (reinterpret_cast<uintptr_t>(in) & 15) != 0 || d0 < 128) return false;
-    if (ndim == 3 && B > 16) return false;
-    return true;
+bool dualquant_x4_aligned(const float* in, uint64_t d0) {
+    return (d0 % 4) == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
 }
+
+bool dualquant_use_x4(int ndim, int B, const float* in, uint64_t d0, uint64_t d1, uint64_t d2, bool batched) {
+    if (ndim == 1 || d0 < 128) return false;
+    if (ndim == 3 && B > 16) return false;
+    if (batched || dualquant_x4_aligned(in, d0)) return true;
+    // element loads: only when the 4-wide tiles alone keep the chip busy
+    const uint64_t tiles = (d0 + 255) / 256 * ((d1 + B - 1) / B) * (ndim == 3 ? (d2 + B - 1) / B : 1);
+    return tiles >= 2048;
+}
+
 
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
                       uint32_t* partials, const DQArgs& a0, int grid, hipStream_t st) {
     const int lds = dq_lds_bytes(a0.dict);
     DQArgs a = a0;
     const bool pad = a.halo != nullptr;
-    const bool x4 = !pad && dualquant_use_x4(ndim, B, in, a.d0);
+    const bool x4 = !pad && dualquant_use_x4(ndim, B, in, a.d0, a.d1, a.d2, false);
+    const bool aln = dualquant_x4_aligned(in, a.d0);
     if (x4) a.ntiles_x = (a.d0 + 255) / 256;
 #define DQ_CASE(ND, KERN, BB)                                                                              \
     if (ndim == ND && B == BB) {                                                                           \
         hipLaunchKernelGGL(KERN<BB>, dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
         goto launched;                                                                                     \
     }
-    if (x4) {
-        DQ_CASE(3, k_dualquant_3d_x4, 8)
-        DQ_CASE(3, k_dualquant_3d_x4, 16)
-        DQ_CASE(2, k_dualquant_2d_x4, 8)
-        DQ_CASE(2, k_dualquant_2d_x4, 16)
-        DQ_CASE(2, k_dualquant_2d_x4, 32)
-        DQ_CASE(2, k_dualquant_2d_x4, 64)
+#define DQ_X4(ND, KERN, BB)                                                                                          \
+    if (ndim == ND && B == BB) {                                                                                     \
+        if (aln)                                                                                                     \
+            hipLaunchKernelGGL((KERN<BB, true>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
+        else                                                                                                         \
+            hipLaunchKernelGGL((KERN<BB, false>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials,  \
+                               a);                                                                                   \
+        goto launched;                                                                                               \
     }
+    if (x4) {
+        DQ_X4(3, k_dualquant_3d_x4, 8)
+        DQ_X4(3, k_dualquant_3d_x4, 16)
+        DQ_X4(2, k_dualquant_2d_x4, 8)
+        DQ_X4(2, k_dualquant_2d_x4, 16)
+        DQ_X4(2, k_dualquant_2d_x4, 32)
+        DQ_X4(2, k_dualquant_2d_x4, 64)
+    }
+#undef DQ_X4
     if (pad) {
 #define DQ_PAD(ND, KERN, BB)                                                                                       \
     if (ndim == ND && B == BB) {                                                                                   \
@@ -1633,9 +1677,9 @@ void launch_dualquant_multi(int B, int kind, const BatchField* fd, int nf, const
 #define DQM(BB, KK)                                                                                              \
     if (B == BB && kind == KK)                                                                                   \
         hipLaunchKernelGGL((k_dualquant_multi<BB, KK>), dim3(grid), dim3(256), 0, st, fd, start, nf, nunits);
-#define DQM_B(BB) DQM(BB, kDq1) DQM(BB, kDq2) DQM(BB, kDq2x4) DQM(BB, kDq3)
+#define DQM_B(BB) DQM(BB, kDq1) DQM(BB, kDq2) DQM(BB, kDq2x4) DQM(BB, kDq2x4u) DQM(BB, kDq3)
     DQM_B(8) DQM_B(16) DQM_B(32) DQM_B(64)
-    DQM(8, kDq3x4) DQM(16, kDq3x4)
+    DQM(8, kDq3x4) DQM(16, kDq3x4) DQM(8, kDq3x4u) DQM(16, kDq3x4u)
 #undef DQM_B
 #undef DQM
 }
