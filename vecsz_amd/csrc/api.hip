@@ -1204,7 +1204,9 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
                     status[k] = build_codebook(reinterpret_cast<const uint64_t*>(hp) + (size_t)k * dict, dict,
                                                c->b_cb[k]);
         };
-        const int nt = std::max(1, std::min(c->nworkers, nf));
+        // host threads: at least the worker count, up to 16 where the host has them
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        const int nt = std::max(1, std::min(nf, std::max(c->nworkers, std::min(16, hw))));
         std::vector<std::thread> th;
         for (int t = 1; t < nt; t++) th.emplace_back(body);
         body();

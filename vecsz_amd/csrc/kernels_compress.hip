@@ -1418,11 +1418,16 @@ __global__ void __launch_bounds__(256) k_chunk_bits_multi(const BatchField* __re
         __syncthreads();
         const uint16_t* codes = fd[f].codes;
         const uint64_t n = fd[f].a.n, nch = fd[f].nchunks, c0 = fd[f].chunk0, ub = start[f];
-        for (; u < ue; u++) {
-            const uint64_t k = (u - ub) * 4 + wave;
-            if (k >= nch) continue;
+        // the next unit's codes are in flight while this one is summed
+        uint32_t nxt[kSymPerThread];
+        uint64_t k = (u - ub) * 4 + wave;
+        if (k < nch) load_syms(codes, n, k * kChunk + lane * kSymPerThread, nxt);
+        for (; u < ue; u++, k += 4) {
             uint32_t sym[kSymPerThread];
-            load_syms(codes, n, k * kChunk + lane * kSymPerThread, sym);
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) sym[j] = nxt[j];
+            if (u + 1 < ue && k + 4 < nch) load_syms(codes, n, (k + 4) * kChunk + lane * kSymPerThread, nxt);
+            if (k >= nch) continue;
             uint32_t bits = 0, zeros = 0;
 #pragma unroll
             for (int j = 0; j < kSymPerThread; j++) {
@@ -1501,14 +1506,19 @@ __global__ void __launch_bounds__(256) k_pack_multi(const BatchField* __restrict
             uint8_t* archive = fd[f].archive;
             const uint64_t n = fd[f].a.n, nch = fd[f].nchunks, c0 = fd[f].chunk0, ub = start[f];
             const uint64_t bb = bit_off[c0], zb = out_off[c0];
-            for (uint64_t v = u; v < ue; v++) {
-                const uint64_t k = (v - ub) * 4 + wave;
+            // software pipeline: the next unit's chunk is in flight while this one is packed
+            ChunkRaw nxt;
+            uint64_t k = (u - ub) * 4 + wave;
+            if (k < nch) load_chunk(codes, n, nch, bit_off + c0, out_off + c0, zb + S.n_outlier, k, lane, fill, nxt);
+            for (uint64_t v = u; v < ue; v++, k += 4) {
+                const ChunkRaw cur = nxt;
+                if (v + 1 < ue && k + 4 < nch)
+                    load_chunk(codes, n, nch, bit_off + c0, out_off + c0, zb + S.n_outlier, k + 4, lane, fill, nxt);
                 if (k >= nch) continue;
-                ChunkRaw cur;
-                load_chunk(codes, n, nch, bit_off + c0, out_off + c0, zb + S.n_outlier, k, lane, fill, cur);
-                cur.bit_off -= bb;
-                cur.out_off -= zb;
-                pack_chunk<BOOK>(cur, k, nch, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
+                ChunkRaw c = cur;
+                c.bit_off -= bb;
+                c.out_off -= zb;
+                pack_chunk<BOOK>(c, k, nch, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
             }
         }
         u = ue;
