@@ -1,12 +1,15 @@
 // C ABI and host orchestration of the compress / decompress pipelines.
 //
 // Compress (vecsz::interface::Compress, src/vecsz_interface.hh:31-229):
-//   [r2r] k_range -> eb family on the host (src/types.cc:84-93)
+//   [r2r] k_range -> k_ebfam (eb family on the device, src/types.cc:84-93)
+//   [padding] pad values and halos
 //   k_dualquant -> histogram to host -> codebook + tree bytes (codebook.cpp)
 //   header + tree H2D -> k_chunk_bits -> chunk scan -> k_enc_summary -> k_pack
+//   Batches: the same stages once for all fields (k_*_multi launches).
 // Decompress (vecsz::interface::Decompress, src/vecsz_interface.hh:232-375):
-//   header + tree D2H -> decode tables -> k_decode_sync -> k_decode_check
-//   [-> k_decode_redo rounds] -> scan -> k_decode_write -> k_recon_* [-> fallback]
+//   header + tree D2H -> decode tables -> k_decode_sync -> k_decode_pass
+//   (re-decode of listed subsequences, until no list remains) -> scan
+//   -> k_decode_write -> k_recon_* [-> fallback]
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
