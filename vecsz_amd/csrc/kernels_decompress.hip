@@ -224,10 +224,13 @@ constexpr uint32_t kStageWords = kSyncWg * (kSubBits / 32) + 12;  // + shift0 an
 constexpr uint32_t kStageLds = kStageWords + kStageWords / 32 + 1;
 constexpr int kCkStride = kCkpts + 1;                            // odd row stride: one bank per thread
 constexpr uint32_t kNoCkpt = 0xFF;                               // window without a codeword start
-// Codewords a continuation decodes before it gives up and lists the
-// subsequence for a full re-decode: most continuations meet a checkpoint
-// within a few windows, and a wavefront otherwise waits for its slowest lane.
-constexpr int kContCap = 32;
+// Bits a continuation decodes before it gives up and lists the subsequence
+// for a full re-decode: most continuations meet a checkpoint within a few
+// windows (a meeting shows at the first codeword start of the window after
+// the two decodes agree, so the budget must span more than one window
+// whatever the code lengths), and a wavefront otherwise waits for its
+// slowest lane.
+constexpr uint32_t kContBits = 288;
 
 // Stages NW aligned-space words starting at w0 (a multiple of 4) into the
 // padded layout.  Every thread issues all of its 16-byte loads before the
@@ -339,7 +342,7 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         bool met = false, capped = false;
         uint32_t q = q0;
         for (;;) {
-            if (cc == kContCap) {
+            if (q - q0 >= kContBits) {
                 capped = true;
                 break;
             }
