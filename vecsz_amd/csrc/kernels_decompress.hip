@@ -11,8 +11,8 @@
 //                     subsequence until a codeword start meets a checkpoint,
 //                     after which both decodes agree, and writes the next
 //                     subsequence's true start, end and counts
-//                     (after 32 codewords without a meeting point it gives
-//                     up and lists the subsequence)
+//                     (after kContBits bits without a meeting point it
+//                     gives up and lists the subsequence)
 //     k_decode_pass   listed subsequences re-decode in full from their true
 //                     starts, from LDS-staged rows; a moved end lists the
 //                     successor for the next pass, until no list is left
