@@ -168,9 +168,12 @@ int vecsz_ctx_set_launch_shape(vecsz_ctx* ctx, int dq_grid, int enc_grid);
  * queue; a request is admitted when a slot is free and the device bytes it
  * holds while running (its buffers plus its scratch) fit in what is left of
  * hbm_budget, so requests that arrive while others run start as soon as room
- * frees up and overlap on the GPU.  Results are those of vecsz_compress /
- * vecsz_decompress on the same request.  Buffers are the caller's device
- * memory and must stay valid until the request is waited for. */
+ * frees up and overlap on the GPU.  A slot that starts a small compress
+ * request also takes the compatible compress requests queued right behind it
+ * (same block size and dictionary, no padding) and runs them as one fused
+ * batch launch.  Results are those of vecsz_compress / vecsz_decompress on the
+ * same request.  Buffers are the caller's device memory and must stay valid
+ * until the request is waited for. */
 typedef struct vecsz_server vecsz_server;
 typedef struct vecsz_server_result {
     int status;          /* VECSZ_* of the request */

@@ -74,3 +74,34 @@ def test_server_budget_admission(gpu_ctx):
     with pytest.raises(VecszError):
         srv.submit_compress(make_params((1 << 24,), 1e-3, "abs"), x.data_ptr(), out.data_ptr(), out.numel())
     srv.close()
+
+
+def test_server_folds_queued_compress_requests(gpu_ctx):
+    """Small compress requests waiting together are folded into one fused
+    batch launch (continuous batching); each archive is the one the request
+    gives alone."""
+    from vecsz_amd import Server, datagen as dg, make_params
+
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(40):
+        nd = int(rng.integers(1, 4))
+        s = (int(rng.integers(100, 50000)),) if nd == 1 else tuple(int(v) for v in rng.integers(8, 300 if nd == 2 else 48, nd))
+        x = torch.from_numpy(dg.grf(s, seed=50 + i) if nd > 1 else
+                             np.cumsum(rng.normal(0, 0.05, s)).astype(np.float32)).cuda()
+        eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        a, nout = gpu_ctx.compress(x, eb, "abs")
+        cap = int(x.numel() * 12 + 4096)
+        reqs.append({"x": x, "p": make_params(tuple(reversed(s)), eb, "abs"), "a": a.clone(), "nout": nout,
+                     "out": torch.empty(cap, dtype=torch.uint8, device="cuda"), "cap": cap})
+    torch.cuda.synchronize()
+    srv = Server(0, slots=1)
+    print("folded compress requests", flush=True)
+    tickets = [srv.submit_compress(r["p"], r["x"].data_ptr(), r["out"].data_ptr(), r["cap"]) for r in reqs]
+    res = [srv.wait(t) for t in tickets]
+    srv.close()
+    for r, q in zip(reqs, res):
+        assert q["out_bytes"] == r["a"].numel() and q["n_outlier"] == r["nout"]
+        assert torch.equal(r["out"][: q["out_bytes"]], r["a"])
+    # requests that shared a launch started and ended together
+    assert len(set(q["run_ms"] for q in res)) < len(res), [q["run_ms"] for q in res]

@@ -1526,11 +1526,27 @@ struct vecsz_server {
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, Request> reqs;
 
+    // Small compress requests at the head of the queue that can share one
+    // fused batch launch: no padding, the same block size and dictionary
+    // (at most 4096), at most kFoldElems elements each.
+    static constexpr uint64_t kFoldElems = 4ull << 20;   // 16 MiB of f32 per request
+    static constexpr uint64_t kFoldTotal = 32ull << 20;  // elements of one folded batch
+    static constexpr int kFoldMax = 64;
+    static uint64_t elems(const vecsz_params& p) {
+        uint64_t n = 1;
+        for (int i = 0; i < std::max(1, std::min(3, p.ndim)); i++) n *= p.dims[i];
+        return n;
+    }
+    static bool foldable(const Request& r) {
+        return r.kind == 0 && r.p.pad_mode == VECSZ_PAD_NONE && r.p.dict_size <= kMaxLdsBook && elems(r.p) <= kFoldElems;
+    }
+
     void slot_loop(vecsz_ctx* c) {
         (void)hipSetDevice(c->device);
         for (;;) {
             uint64_t t;
             Request r;
+            std::vector<uint64_t> fold;  // further requests folded into r's launch
             {
                 std::unique_lock<std::mutex> lk(m);
                 // FIFO admission: the oldest request starts once a slot is
@@ -1544,6 +1560,26 @@ struct vecsz_server {
                 in_use += q.need;
                 q.t_start = now_ms();
                 r = q;
+                // continuous batching: compress requests waiting right behind
+                // it join its launch while they fit the budget
+                if (foldable(r)) {
+                    uint64_t total = elems(r.p);
+                    while (!queue.empty() && (int)fold.size() + 1 < kFoldMax) {
+                        Request& n = reqs[queue.front()];
+                        if (!foldable(n) || n.p.block != r.p.block || n.p.dict_size != r.p.dict_size ||
+                            total + elems(n.p) > kFoldTotal || n.need > budget - in_use)
+                            break;
+                        total += elems(n.p);
+                        in_use += n.need;
+                        n.t_start = q.t_start;
+                        fold.push_back(queue.front());
+                        queue.pop_front();
+                    }
+                }
+            }
+            if (!fold.empty()) {
+                run_folded(c, t, r, fold);
+                continue;
             }
             int rc;
             if (r.kind == 0) {
@@ -1567,6 +1603,49 @@ struct vecsz_server {
             cv_done.notify_all();
             cv_work.notify_all();
         }
+    }
+
+    // Runs request t (copied in r) and the folded requests as one fused
+    // compress batch; each keeps its own status and sizes.
+    void run_folded(vecsz_ctx* c, uint64_t t, const Request& r, const std::vector<uint64_t>& fold) {
+        const int nf = 1 + (int)fold.size();
+        std::vector<vecsz_params> ps(nf);
+        std::vector<const float*> ins(nf);
+        std::vector<uint8_t*> outs(nf);
+        std::vector<uint64_t> caps(nf), sizes(nf, 0), nouts(nf, 0);
+        std::vector<int> st(nf, 0);
+        {
+            std::lock_guard<std::mutex> lk(m);
+            for (int k = 0; k < nf; k++) {
+                const Request& q = k == 0 ? r : reqs[fold[k - 1]];
+                ps[k] = q.p;
+                ins[k] = static_cast<const float*>(q.in);
+                outs[k] = static_cast<uint8_t*>(q.out);
+                caps[k] = q.out_cap;
+            }
+        }
+        if (batch_fusable(c, nf, ps.data(), ins.data(), outs.data())) {
+            (void)compress_batch_fused(c, nf, ps.data(), ins.data(), outs.data(), caps.data(), sizes.data(),
+                                       nouts.data(), st.data());
+        } else {  // a request with invalid parameters: each alone, for its own status
+            for (int k = 0; k < nf; k++)
+                st[k] = compress_device(c, &ps[k], ins[k], outs[k], caps[k], &sizes[k], &nouts[k]);
+        }
+        {
+            std::lock_guard<std::mutex> lk(m);
+            const double te = now_ms();
+            for (int k = 0; k < nf; k++) {
+                Request& q = reqs[k == 0 ? t : fold[k - 1]];
+                q.status = st[k];
+                q.out_bytes = sizes[k];
+                q.n_outlier = nouts[k];
+                q.t_end = te;
+                q.done = true;
+                in_use -= q.need;
+            }
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
     }
 };
 
