@@ -701,6 +701,8 @@ __device__ __forceinline__ void dq1_tile(const float* __restrict__ in, uint16_t*
             const uint64_t bstart = x0 - (x0 % B);
             const bool vec = bstart + B < a.d0;
             const float thr = vec ? a.thr_vec : a.thr_scalar;
+            const bool whole = x0 + 8 <= a.n;  // the group's 8 codes go out as one 16-byte store
+            uint32_t cw[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const uint64_t idx = x0 + j;
@@ -713,12 +715,15 @@ __device__ __forceinline__ void dq1_tile(const float* __restrict__ in, uint16_t*
                 } else {
                     code = postquant(q[j], pred, thr, a.radius_f);
                 }
+                cw[j >> 1] |= (code & 0xFFFFu) << (16 * (j & 1));
                 if (idx < a.n) {
-                    codes[idx] = (uint16_t)code;
+                    if (!whole) codes[idx] = (uint16_t)code;
                     if (code == 0) sparse[idx] = q[j];
                     H.add((int)code);
                 }
             }
+            // codes is 16-byte aligned and x0 a multiple of 8
+            if (whole) *reinterpret_cast<uint4*>(codes + x0) = make_uint4(cw[0], cw[1], cw[2], cw[3]);
         }
     }
 }
