@@ -408,62 +408,70 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
 constexpr uint32_t kRowWords = 38;   // 32 words + shift0 + the last codeword and window reads
 constexpr uint32_t kRowStride = 39;  // odd: one bank per thread
 constexpr uint32_t kPassGrid = 512;  // persistent workgroups of a pass
+// Entries e0 .. e0 + 255 (those below m) of pass `pass`'s list: staged rows,
+// one entry per thread.
 template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_pass(DecodeArgs a, SubState s, uint32_t pass, DecSummary* sum) {
-    const uint32_t m = s.list_n[pass % 3];
-    if (blockIdx.x == 0 && threadIdx.x == 0) s.list_n[(pass + 2) % 3] = 0;  // the list after next
-    if ((uint64_t)blockIdx.x * 256 >= m) return;  // whole workgroup
+__device__ __forceinline__ void redecode_entries(const DecodeArgs& a, const SubState& s, uint32_t pass,
+                                                 const Decoder<T16>& dec, uint32_t* rows, const StreamView& sv,
+                                                 uint64_t e0, uint32_t m, bool& bad) {
     const uint32_t* lj = s.list_j[pass & 1];
     const uint64_t* ls = s.list_start[pass & 1];
     uint32_t* oj = s.list_j[(pass + 1) & 1];
     uint64_t* os = s.list_start[(pass + 1) & 1];
     uint32_t* on = &s.list_n[(pass + 1) % 3];
+    const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
+    __syncthreads();  // the tables (first time) / the previous rows' readers
+    for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
+        const uint32_t e = p / kRowWords, w = p % kRowWords;
+        const uint64_t j = lj[e0 + e];
+        rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
+    }
+    __syncthreads();
+    if (threadIdx.x < ne) {
+        const uint64_t j = lj[e0 + threadIdx.x];
+        const uint64_t start = ls[e0 + threadIdx.x];
+        const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
+        const RowWin win{rows + threadIdx.x * kRowStride};
+        const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
+        const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
+        uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
+        uint32_t cnt = 0, z = 0;
+        while (lp < tl) {
+            uint32_t len;
+            const uint32_t sym = dec.take_w(win, lp, len);
+            bad |= len == 0;
+            lp += len ? len : 1;
+            cnt++;
+            z += sym == 0;
+        }
+        const uint64_t pos = rowbit + lp - sv.shift0;
+        s.b[j] = start;
+        s.cnt[j] = cnt;
+        s.zer[j] = z;
+        if (pos != s.e[j]) {
+            s.e[j] = pos;
+            if (j + 1 < a.nsub) {
+                const uint32_t k = atomicAdd(on, 1u);
+                oj[k] = (uint32_t)(j + 1);
+                os[k] = pos;
+            }
+        }
+    }
+}
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_pass(DecodeArgs a, SubState s, uint32_t pass, DecSummary* sum) {
+    const uint32_t m = s.list_n[pass % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) s.list_n[(pass + 2) % 3] = 0;  // the list after next
+    if ((uint64_t)blockIdx.x * 256 >= m) return;  // whole workgroup
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     uint32_t* rows = lds + dec.load(lds, a);
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     bool bad = false;
-    for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256) {
-        const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
-        __syncthreads();  // the tables (first time) / the previous rows' readers
-        for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
-            const uint32_t e = p / kRowWords, w = p % kRowWords;
-            const uint64_t j = lj[e0 + e];
-            rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
-        }
-        __syncthreads();
-        if (threadIdx.x < ne) {
-            const uint64_t j = lj[e0 + threadIdx.x];
-            const uint64_t start = ls[e0 + threadIdx.x];
-            const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
-            const RowWin win{rows + threadIdx.x * kRowStride};
-            const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
-            const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
-            uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
-            uint32_t cnt = 0, z = 0;
-            while (lp < tl) {
-                uint32_t len;
-                const uint32_t sym = dec.take_w(win, lp, len);
-                bad |= len == 0;
-                lp += len ? len : 1;
-                cnt++;
-                z += sym == 0;
-            }
-            const uint64_t pos = rowbit + lp - sv.shift0;
-            s.b[j] = start;
-            s.cnt[j] = cnt;
-            s.zer[j] = z;
-            if (pos != s.e[j]) {
-                s.e[j] = pos;
-                if (j + 1 < a.nsub) {
-                    const uint32_t k = atomicAdd(on, 1u);
-                    oj[k] = (uint32_t)(j + 1);
-                    os[k] = pos;
-                }
-            }
-        }
-    }
+    for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256)
+        redecode_entries<T16>(a, s, pass, dec, rows, sv, e0, m, bad);
     if (bad) atomicOr(&sum->error, 1u);
 }
 
@@ -1071,16 +1079,20 @@ static size_t table_lds_bytes(const DecodeArgs& a, bool t16) {
 }
 
 // Lets a kernel take more than the default 64 KiB of dynamic LDS (gfx950
-// has 160 KiB per workgroup); set once per kernel.
-template <typename K>
-static void allow_big_lds(K kernel) {
-    static bool done = false;
-    if (!done) {
-        hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            160 * 1024);
-        done = true;
-    }
+// has 160 KiB per workgroup; the kernels here have no static LDS).  Set once
+// per kernel: each call site keeps its own once-flag (kernels of one
+// signature share a function type, so a flag per type would skip all but
+// the first of them).
+static bool set_big_lds(const void* kernel) {
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+        (void)hipGetLastError();  // the launch then reports what does not fit
+    return true;
 }
+#define allow_big_lds(K)                                                                 \
+    do {                                                                                 \
+        static const bool once_ = set_big_lds(reinterpret_cast<const void*>(&(K)));      \
+        (void)once_;                                                                     \
+    } while (0)
 
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
     allow_big_lds(k_decode_sync<true>);
