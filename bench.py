@@ -29,10 +29,15 @@ def log(*a):
 
 
 def stage_avg(ctx, fn, reps):
+    """Per-call device time of each stage (a stage that runs several times in
+    one call, like the decoder's re-decode passes, is summed), averaged."""
     acc = {}
     for _ in range(reps):
         fn()
+        per = {}
         for name, ms in ctx.timings():
+            per[name] = per.get(name, 0.0) + ms
+        for name, ms in per.items():
             acc.setdefault(name, []).append(ms)
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
@@ -491,6 +496,34 @@ def main():
     torch.cuda.synchronize()
     td = (time.perf_counter() - t0) / args.steps
 
+    # the same with a decode index (a side product of compress kept beside
+    # the archive): compress writing it, decompress using it
+    iw = ctx.index_words(p)
+    idx = torch.empty(iw, dtype=torch.int64, device="cuda")
+    yi = torch.empty_like(x)
+
+    def comp_idx():
+        return ctx.compress_into(p, x.data_ptr(), arch.data_ptr(), cap, idx.data_ptr(), iw)
+
+    def decomp_idx():
+        ctx.decompress_into(pd, arch.data_ptr(), size, yi.data_ptr(), n, idx.data_ptr(), iw)
+
+    for _ in range(args.warmup):
+        comp_idx()
+        decomp_idx()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        comp_idx()
+    torch.cuda.synchronize()
+    tci = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        decomp_idx()
+    torch.cuda.synchronize()
+    tdi = (time.perf_counter() - t0) / args.steps
+    same_idx = bool(torch.equal(yi.view(torch.int32), y.view(torch.int32)))
+
     # correctness: the round trip must respect the error bound up to the
     # reference's fp32 slack (bit-exact parity is tested in tests/).
     err = (x.double() - y.double()).abs()
@@ -503,6 +536,7 @@ def main():
     ctx.enable_timing(True)
     cst = stage_avg(ctx, comp, args.steps)
     dst = stage_avg(ctx, decomp, args.steps)
+    dsti = stage_avg(ctx, decomp_idx, args.steps)
     ctx.enable_timing(False)
     log("compress stages (ms):", {k: round(v, 4) for k, v in cst.items()})
     log("decompress stages (ms):", {k: round(v, 4) for k, v in dst.items()})
@@ -549,6 +583,10 @@ def main():
                    "dict_size": args.dict, "parallelism": "single-gpu"},
         "decompress_gbps": round(dec_gbps, 2),
         "decompress_ms": round(td * 1e3, 4),
+        "decode_index": {"words": iw, "bytes": 8 * iw, "compress_gbps": round(4 * n / tci / 1e9, 2),
+                         "decompress_gbps": round(4 * n / tdi / 1e9, 2), "decompress_ms": round(tdi * 1e3, 4),
+                         "identical_output": same_idx,
+                         "decompress_stages_ms": {k: round(v, 4) for k, v in dsti.items()}},
         "archive_bytes": size,
         "compression_ratio": round(4 * n / size, 4),
         "n_outlier": nout,

@@ -98,6 +98,27 @@ int vecsz_compress(vecsz_ctx* ctx, const vecsz_params* p, const float* d_in, uin
 int vecsz_decompress(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
                      float* d_out, uint64_t out_cap);
 
+/* Decode index (no counterpart in the reference; its decoder walks the
+ * bitstream serially, src/huffman.hh:343-372): a side product of compress,
+ * kept beside the archive, never inside it.  For every 256 symbols it records
+ * the bit where their first codeword starts and the rank of their first
+ * outlier, so decompress can decode each 256-symbol segment independently
+ * instead of synchronising on the bitstream.
+ * vecsz_index_words: size of the index for a field, in u64 words.
+ * vecsz_compress_indexed: vecsz_compress (the same archive, byte for byte)
+ *   that also writes the index to d_index (device, index_words words).
+ * vecsz_decompress_indexed: vecsz_decompress (the same output, bit for bit)
+ *   taking the index; VECSZ_EINVAL when the index was not made for this
+ *   archive (element count, segment count, stream bits or outlier count
+ *   differ), VECSZ_ECORRUPT when a segment does not end where the next one
+ *   starts. */
+int vecsz_index_words(const vecsz_params* p, uint64_t* words);
+int vecsz_compress_indexed(vecsz_ctx* ctx, const vecsz_params* p, const float* d_in, uint8_t* d_out,
+                           uint64_t out_cap, uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index,
+                           uint64_t index_words);
+int vecsz_decompress_indexed(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
+                             const uint64_t* d_index, uint64_t index_words, float* d_out, uint64_t out_cap);
+
 /* Host-memory conveniences: stage through device buffers owned by the
  * context.  Same semantics as the device entry points. */
 int vecsz_compress_host(vecsz_ctx* ctx, const vecsz_params* p, const float* h_in, uint8_t* h_out,

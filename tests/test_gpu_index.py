@@ -1,0 +1,100 @@
+"""Decode index (vecsz_compress_indexed / vecsz_decompress_indexed): the
+archive is the one vecsz_compress writes, byte for byte; decompressing with
+the index gives the self-synchronising decode's output bit for bit (and the
+oracle's, for the small cases); an index that belongs to another archive, or
+a damaged one, is refused."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _check(ctx, x, eb, mode, block=8, dict_size=4096, rng=None, **pad):
+    xt = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    _say("indexed", tuple(xt.shape), eb, mode, block, dict_size, pad)
+    a0, n0 = ctx.compress(xt, eb, mode, block, dict_size, **pad)
+    a, n, idx = ctx.compress_indexed(xt, eb, mode, block, dict_size, **pad)
+    assert n == n0 and torch.equal(a, a0)
+    if rng is None:
+        rng = float((xt.max() - xt.min()).item()) if mode == "r2r" else 0.0
+    y0 = ctx.decompress(a, mode, rng, block, dict_size)
+    y = ctx.decompress(a, mode, rng, block, dict_size, index=idx)
+    assert torch.equal(y.view(torch.int32), y0.view(torch.int32))
+    return a, idx, y
+
+
+def test_indexed_matches_self_sync_and_oracle(gpu_ctx, golden):
+    for c in golden["cases"]:
+        if c["x"].size < 2:
+            continue
+        mode = "r2r" if c["r2r"] else "abs"
+        a, idx, y = _check(gpu_ctx, c["x"], c["eb"], mode, c["block"], c["dict"],
+                           rng=c["value_range"] if c["r2r"] else 0.0)
+        assert a.cpu().numpy().tobytes() == c["archive"], c["name"]
+        assert np.array_equal(y.cpu().numpy().reshape(-1).view(np.uint32), c["out"].reshape(-1).view(np.uint32)), \
+            c["name"]
+
+
+@pytest.mark.parametrize("eb", [1e-2, 1e-3, 1e-4])
+def test_indexed_grf(gpu_ctx, eb):
+    from vecsz_amd import datagen as dg
+
+    x = dg.grf((96, 128, 160), seed=3, device="cuda", pow2=True)
+    _check(gpu_ctx, x, eb, "r2r")
+    _check(gpu_ctx, x, eb, "r2r", block=16)
+
+
+def test_indexed_other_forms(gpu_ctx):
+    from vecsz_amd import datagen as dg
+
+    w = dg.walk_spikes_1d((1 << 22) + 77, seed=4, device="cuda")
+    _check(gpu_ctx, w, 1e-4, "r2r")
+    _check(gpu_ctx, torch.from_numpy(dg.sinusoids_2d(300, 517, seed=2)).cuda(), 1e-4, "abs")
+    g = torch.from_numpy(dg.grf((40, 61, 75), seed=5)).cuda()
+    _check(gpu_ctx, g, 1e-5, "abs", dict_size=65536)  # 32-bit decode tables
+    _check(gpu_ctx, g, 1e-3, "abs", dict_size=1024)
+    _check(gpu_ctx, g, 1e-3, "r2r", pad_mode="block", pad_type="avg")
+    _check(gpu_ctx, torch.zeros(50, 60, device="cuda"), 1e-3, "abs")  # one-symbol tree: no bitstream
+    _check(gpu_ctx, torch.randn(10, 11, 12, device="cuda") * 100, 1e-6, "abs")  # all outliers
+
+
+def test_indexed_full_size_grf(gpu_ctx):
+    from vecsz_amd import datagen as dg
+
+    x = dg.grf((256, 256, 256), seed=0, device="cuda")
+    _check(gpu_ctx, x, 1e-4, "r2r")
+
+
+def test_index_of_another_archive_is_refused(gpu_ctx):
+    from vecsz_amd import VecszError, datagen as dg
+
+    x = torch.from_numpy(dg.grf((32, 40, 48), seed=8)).cuda()
+    a, n, idx = gpu_ctx.compress_indexed(x, 1e-3, "abs")
+    b, m, idx_b = gpu_ctx.compress_indexed(x + 1.0, 1e-3, "abs")
+    assert a.numel() != b.numel() or not torch.equal(a, b)
+    with pytest.raises(VecszError):
+        gpu_ctx.decompress(a, "abs", index=idx[:5])  # shorter than its header
+    bad = idx.clone()
+    bad[1] += 1  # element count
+    with pytest.raises(VecszError):
+        gpu_ctx.decompress(a, "abs", index=bad)
+    if int(idx_b[3]) != int(idx[3]):  # stream bits differ
+        with pytest.raises(VecszError):
+            gpu_ctx.decompress(a, "abs", index=idx_b)
+    # a segment start moved by one bit: the decode notices the misfit
+    bad = idx.clone()
+    bad[6 + 3] += 1
+    with pytest.raises(VecszError):
+        gpu_ctx.decompress(a, "abs", index=bad)
+    # and the context is still good afterwards
+    y0 = gpu_ctx.decompress(a, "abs")
+    y = gpu_ctx.decompress(a, "abs", index=idx)
+    assert torch.equal(y.view(torch.int32), y0.view(torch.int32))

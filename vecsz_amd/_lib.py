@@ -22,7 +22,8 @@ MODE_R2R = 1
 # Every symbol include/vecsz_hip.h declares.
 EXPORTS = (
     "vecsz_ctx_create", "vecsz_ctx_destroy", "vecsz_strerror", "vecsz_ctx_last_error", "vecsz_params_default",
-    "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_compress_host", "vecsz_decompress_host",
+    "vecsz_compress_bound", "vecsz_compress", "vecsz_decompress", "vecsz_index_words", "vecsz_compress_indexed",
+    "vecsz_decompress_indexed", "vecsz_compress_host", "vecsz_decompress_host",
     "vecsz_archive_info", "vecsz_huffman_codebook", "vecsz_compress_batch", "vecsz_decompress_batch",
     "vecsz_ctx_set_workers", "vecsz_ctx_set_batch_fused", "vecsz_autotune", "vecsz_ctx_set_launch_shape",
     "vecsz_server_create", "vecsz_server_destroy", "vecsz_server_submit_compress",
@@ -138,6 +139,9 @@ def load():
     lib.vecsz_decompress_batch.argtypes = [vp, ctypes.c_int, P, vp, u64p, vp, u64p, ctypes.POINTER(ctypes.c_int)]
     lib.vecsz_ctx_set_workers.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_set_batch_fused.argtypes = [vp, ctypes.c_int]
+    lib.vecsz_index_words.argtypes = [P, u64p]
+    lib.vecsz_compress_indexed.argtypes = [vp, P, vp, vp, u64, u64p, u64p, vp, u64]
+    lib.vecsz_decompress_indexed.argtypes = [vp, P, vp, u64, vp, u64, vp, u64]
     lib.vecsz_autotune.argtypes = [vp, P, vp, vp, u64, ctypes.POINTER(Tune)]
     lib.vecsz_ctx_set_launch_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     lib.vecsz_server_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, u64]

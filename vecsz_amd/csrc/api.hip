@@ -295,8 +295,10 @@ struct HostTimer {
     }
 };
 
+// d_index (nullable): decode index of kIdxHead + 2 * idx_segments(n) words,
+// written beside the archive.
 int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
-                    uint64_t* out_bytes, uint64_t* n_outlier) {
+                    uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index = nullptr) {
     int rc = check_params(c, p, true);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
@@ -470,6 +472,17 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint32_t* cw_lo = cw_hi + dict;
     const uint8_t* cw_len = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
     HIPCHK(c, hipMemsetAsync(c->err.p, 0, 16, c->stream));
+    if (!c->pin2.ensure(256)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    if (d_index) {
+        // index header words 0-2 and 5; k_enc_summary writes words 3 and 4
+        uint64_t* ih = reinterpret_cast<uint64_t*>(c->pin2.as<uint8_t>() + 192);
+        ih[0] = kIdxMagic;
+        ih[1] = n;
+        ih[2] = idx_segments(n);
+        ih[5] = kIdxSeg;
+        HIPCHK(c, hipMemcpyAsync(d_index, ih, 24, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_index + 5, ih + 5, 8, hipMemcpyHostToDevice, c->stream));
+    }
     {
         StageScope s(c, "chunk_bits");
         launch_chunk_bits(c->codes.as<uint16_t>(), n, cw_len, dict, c->chunk_bits.as<uint32_t>(),
@@ -481,16 +494,15 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
                           c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
                           c->scan_tot.as<uint64_t>(), c->stream);
         launch_enc_summary(c->scan_tot.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb, pad_bytes, tb, out_cap,
-                           c->err.as<uint32_t>(), c->stream);
+                           c->err.as<uint32_t>(), d_index, c->stream);
     }
     {
         StageScope s(c, "pack");
         launch_pack(c->codes.as<uint16_t>(), n, cw_hi, cw_lo, cw_len, dict, c->bit_off.as<uint64_t>(),
                     c->out_off.as<uint64_t>(), nchunks, c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out,
-                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->enc_grid, c->stream);
+                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
     }
     HIPCHK(c, hipGetLastError());
-    if (!c->pin2.ensure(256)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
     uint8_t* h = c->pin2.as<uint8_t>();
     HIPCHK(c, hipMemcpyAsync(h, c->summary.p, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(h + 128, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
@@ -562,8 +574,11 @@ bool tree_is_full(const DecodeTables& dt) {
     return true;
 }
 
+// d_index (nullable): a decode index written by compress_device for this
+// archive (index_words words); the decode then starts every segment at its
+// first codeword instead of synchronising.
 int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes, float* d_out,
-                      uint64_t out_cap) {
+                      uint64_t out_cap, const uint64_t* d_index = nullptr, uint64_t index_words = 0) {
     int rc = check_params(c, p, false);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
@@ -575,7 +590,13 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     uint8_t* h = c->pin.as<uint8_t>();
     const uint64_t head_avail = std::min<uint64_t>(in_bytes, kHeadRead);
     HIPCHK(c, hipMemcpyAsync(h, d_in, head_avail, hipMemcpyDeviceToHost, c->stream));
+    uint64_t ih[kIdxHead] = {0, 0, 0, 0, 0, 0};
+    if (d_index) {
+        if (index_words < kIdxHead) return fail(c, VECSZ_EINVAL, "decode index too short");
+        HIPCHK(c, hipMemcpyAsync(h + kHeadRead, d_index, 8 * kIdxHead, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (d_index) memcpy(ih, h + kHeadRead, sizeof(ih));
     int ndim;
     uint64_t dims[3], pad_len, huff_size, nout;
     double ebu;
@@ -623,9 +644,16 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
     const uint8_t* d_stream = d_huff + 8 + tb;
     const uint64_t stream_bytes = huff_size - 8 - tb;
+    if (d_index) {
+        const uint64_t nseg = idx_segments(n);
+        if (ih[0] != kIdxMagic || ih[1] != n || ih[2] != nseg || ih[5] != (uint64_t)kIdxSeg ||
+            index_words < kIdxHead + 2 * nseg || (ih[3] + 7) / 8 != stream_bytes || ih[4] != nout)
+            return fail(c, VECSZ_EINVAL, "decode index does not belong to this archive");
+    }
     const uint8_t* d_outl = d_huff + huff_size;
     uint16_t* codes = c->codes.as<uint16_t>();
     float* sparse = c->sparse.as<float>();
+    bool indexed = false;  // decode errors then show at the end
 
     if (dt.root_leaf) {
         StageScope s(c, "decode");
@@ -686,6 +714,12 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         ss.list_start[1] = c->sub_ls.as<uint64_t>() + nsub;
         ss.list_n = c->sub_ln.as<uint32_t>();
         DecSummary* dsum = c->dec_sum.as<DecSummary>();
+        if (d_index) {
+            StageScope s(c, "decode_indexed");
+            launch_decode_indexed(da, d_index, ih[3], codes, sparse, dsum, c->stream);
+            HIPCHK(c, hipGetLastError());
+            indexed = true;
+        } else {
         {
             StageScope s(c, "decode_sync");
             launch_decode_sync(da, ss, dsum, c->stream);
@@ -733,6 +767,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         memcpy(tot, h + 64, 16);
         if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
         if (tot[0] < n) return fail(c, VECSZ_ECORRUPT, "bitstream holds fewer symbols than the field");
+        }
     }
 
     // Lorenzo reconstruction.
@@ -781,8 +816,14 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
                                     c->nflagged.as<uint32_t>(), c->stream);
     }
     HIPCHK(c, hipGetLastError());
+    if (indexed) HIPCHK(c, hipMemcpyAsync(h, c->dec_sum.p, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     timing_end(c);
+    if (indexed) {
+        DecSummary ds;
+        memcpy(&ds, h, sizeof(ds));
+        if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode with its decode index");
+    }
     return VECSZ_OK;
 }
 
@@ -875,6 +916,31 @@ int vecsz_compress(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
                    uint64_t* out_bytes, uint64_t* n_outlier) {
     if (!c || !d_in || !d_out || !out_bytes) return fail(c, VECSZ_EINVAL, "NULL argument");
     return compress_device(c, p, d_in, d_out, out_cap, out_bytes, n_outlier);
+}
+
+int vecsz_index_words(const vecsz_params* p, uint64_t* words) {
+    if (!p || !words || p->ndim < 1 || p->ndim > 3) return VECSZ_EINVAL;
+    uint64_t n = 1;
+    for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
+    *words = kIdxHead + 2 * idx_segments(n);
+    return VECSZ_OK;
+}
+
+int vecsz_compress_indexed(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                           uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index, uint64_t index_words) {
+    if (!c) return VECSZ_EINVAL;
+    if (!d_in || !d_out || !out_bytes || !d_index) return fail(c, VECSZ_EINVAL, "NULL argument");
+    uint64_t need = 0;
+    if (vecsz_index_words(p, &need)) return fail(c, VECSZ_EINVAL, "params");
+    if (index_words < need) return fail(c, VECSZ_ENOMEM, "decode index buffer too small");
+    return compress_device(c, p, d_in, d_out, out_cap, out_bytes, n_outlier, d_index);
+}
+
+int vecsz_decompress_indexed(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
+                             const uint64_t* d_index, uint64_t index_words, float* d_out, uint64_t out_cap) {
+    if (!c) return VECSZ_EINVAL;
+    if (!d_in || !d_out || !d_index) return fail(c, VECSZ_EINVAL, "NULL argument");
+    return decompress_device(c, p, d_in, in_bytes, d_out, out_cap, d_index, index_words);
 }
 
 int vecsz_decompress(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes, float* d_out,

@@ -95,6 +95,17 @@ struct BatchField {
     uint64_t nchunks, chunk0;   // 1024-symbol chunks, and the first one's index in the batch
 };
 
+// Decode index: an optional side product of compress (never part of the
+// archive) that lets decompress start every kIdxSeg-symbol segment at its
+// first codeword instead of synchronising.  u64 words: [0] kIdxMagic, [1] n,
+// [2] segments, [3] stream bits, [4] outliers, [5] kIdxSeg, then the first
+// codeword bit of each segment (from the stream start), then the outlier rank
+// of each segment's first symbol.
+constexpr int kIdxSeg = 256;
+constexpr int kIdxHead = 6;
+constexpr uint64_t kIdxMagic = 0x3158444956535A56ull;  // "VZSVIDX1" little-endian
+__host__ __device__ inline uint64_t idx_segments(uint64_t n) { return (n + kIdxSeg - 1) / kIdxSeg; }
+
 // Decoder summary.
 struct DecSummary {
     uint32_t error;       // nonzero: corrupt stream

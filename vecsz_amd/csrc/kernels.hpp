@@ -24,11 +24,14 @@ bool dualquant_x4_aligned(const float* in, uint64_t d0);
 // does not depend on it).
 void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab, int dict, uint32_t* cb, uint32_t* cz,
                        uint64_t nchunks, int grid_cap, hipStream_t st);
+// index (nullable): decode index whose header words 3 and 4 get the totals.
 void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
-                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st);
+                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, uint64_t* index,
+                        hipStream_t st);
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
-                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap, hipStream_t st);
+                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap,
+                 uint64_t* index, hipStream_t st);
 
 // Batched compress (kernels_compress.hip): every field of a batch per launch.
 // start arrays hold nf + 1 unit offsets (see BatchField).
@@ -91,6 +94,11 @@ void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
                          uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st);
 void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st);
+// Decode with a decode index (index = its kIdxHead header words first): codes
+// and outliers as k_decode_write leaves them.
+// total_bits: the stream's bit count from the index header.
+void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
+                           float* sparse, DecSummary* sum, hipStream_t st);
 void launch_scatter_all_outliers(const uint8_t* outliers, uint64_t n, float* sparse, hipStream_t st);
 
 struct ReconArgs {

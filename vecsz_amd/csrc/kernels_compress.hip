@@ -863,7 +863,12 @@ __global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__
 // (src/utils/datapack.hh:288) written in place.
 __device__ __forceinline__ void enc_summary(uint64_t tb, uint64_t tz, EncSummary* __restrict__ sum,
                                             uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
-                                            uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
+                                            uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err,
+                                            uint64_t* __restrict__ index = nullptr) {
+    if (index) {  // decode-index header: stream bits and outliers
+        index[3] = tb;
+        index[4] = tz;
+    }
     const uint64_t enc = (tb + 7) / 8;
     const uint64_t huff = 8 + tree_bytes + enc;
     const uint64_t stream_byte = meta_bytes + pad_bytes + 8 + tree_bytes;
@@ -887,9 +892,10 @@ __device__ __forceinline__ void enc_summary(uint64_t tb, uint64_t tz, EncSummary
 
 __global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
                               uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
-                              uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err) {
+                              uint64_t tree_bytes, uint64_t out_cap, uint32_t* __restrict__ err,
+                              uint64_t* __restrict__ index) {
     if (threadIdx.x == 0 && blockIdx.x == 0)
-        enc_summary(totals[0], totals[1], sum, archive, meta_bytes, pad_bytes, tree_bytes, out_cap, err);
+        enc_summary(totals[0], totals[1], sum, archive, meta_bytes, pad_bytes, tree_bytes, out_cap, err, index);
 }
 
 // Bit packing, one wavefront per 1024-symbol chunk, no workgroup barriers or
@@ -981,13 +987,15 @@ __host__ __device__ constexpr uint32_t book_words(int book, int dict) {
 }
 
 // Packs chunk k (codes already in `cur`) into the archive; see k_pack.
+// index: decode-index segment entries (first bits, then outlier ranks), or null.
 template <int BOOK>
 __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint64_t nchunks, uint32_t lane,
                                            const uint32_t* tab, const uint8_t* tlen,
                                            const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
                                            const uint8_t* __restrict__ len_tab, const EncSummary& S,
                                            const float* __restrict__ sparse, uint8_t* __restrict__ archive,
-                                           uint32_t* stage) {
+                                           uint32_t* stage, uint64_t* __restrict__ index = nullptr,
+                                           uint64_t nseg = 0) {
     constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint64_t base = k * kChunk + lane * kSymPerThread;
@@ -1032,6 +1040,17 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
     const uint32_t incl = wave_scan_dpp(v);
     const uint32_t excl = incl - v;
     const uint32_t cbits = lane63(incl) & 0x1FFFF;
+
+    // decode index: lanes holding a segment's first symbol record its first
+    // codeword bit and outlier rank
+    constexpr uint32_t kLanesPerSeg = kIdxSeg / kSymPerThread;
+    if (index && (lane % kLanesPerSeg) == 0) {
+        const uint64_t g = k * (kChunk / kIdxSeg) + lane / kLanesPerSeg;
+        if (g < nseg) {
+            index[g] = cur.bit_off + (excl & 0x1FFFF);
+            index[nseg + g] = cur.out_off + (excl >> 17);
+        }
+    }
 
     // outlier compaction, index order
     if (tz) {
@@ -1149,7 +1168,8 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
                                               const uint64_t* __restrict__ bit_off, const uint64_t* __restrict__ out_off,
                                               uint64_t nchunks, const EncSummary* __restrict__ sum,
                                               const float* __restrict__ sparse, uint8_t* __restrict__ archive,
-                                              const uint32_t* __restrict__ err, uint32_t stage_words) {
+                                              const uint32_t* __restrict__ err, uint32_t stage_words,
+                                              uint64_t* __restrict__ index, uint64_t nseg) {
     constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     if (*err) return;
     extern __shared__ uint32_t lds[];
@@ -1182,7 +1202,8 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     for (; k < nchunks; k += nw) {
         const ChunkRaw cur = nxt;
         if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k + nw, lane, fill, nxt);
-        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
+        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage, index,
+                         nseg);
     }
 }
 
@@ -1650,21 +1671,26 @@ void launch_chunk_bits(const uint16_t* codes, uint64_t n, const uint8_t* len_tab
 }
 
 void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
-                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, hipStream_t st) {
+                        uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap, uint32_t* err, uint64_t* index,
+                        hipStream_t st) {
     hipLaunchKernelGGL(k_enc_summary, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, pad_bytes,
-                       tree_bytes, out_cap, err);
+                       tree_bytes, out_cap, err, index);
 }
 
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
-                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap, hipStream_t st) {
+                 const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap,
+                 uint64_t* index, hipStream_t st) {
+    uint64_t* ientries = index ? index + kIdxHead : nullptr;
+    const uint64_t nseg = index ? idx_segments(n) : 0;
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, grid_cap));
     // words one chunk can touch: 1024 codewords of at most max_len bits, plus
     // the partial words at either end
     const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
     const int book = dict > kMaxLdsBook ? kBookGlobal : max_len <= 32 ? kBookSplit : kBookPacked;
     const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;
-#define PACK_ARGS codes, n, cw_hi, cw_lo, len_tab, dict, bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words
+#define PACK_ARGS \
+    codes, n, cw_hi, cw_lo, len_tab, dict, bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words, ientries, nseg
     if (book == kBookSplit)
         hipLaunchKernelGGL(k_pack<kBookSplit>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
     else if (book == kBookPacked)

@@ -582,6 +582,117 @@ __device__ __forceinline__ float read_f32_bytes(const uint8_t* p) {
     return __uint_as_float(v);
 }
 
+// Window over the stream for the indexed decode: 64 bits held in a register
+// (relative bit b0 at its MSB, refilled a word at a time by the caller);
+// windows beyond it (the deeper levels of codewords over 32 bits) are read
+// from memory.  Relative bit 0 is aligned-space bit 32 * w0.
+struct CursorWin {
+    StreamView sv;
+    uint64_t w0;
+    uint64_t buf;
+    uint32_t b0;
+    __device__ __forceinline__ uint32_t operator()(uint32_t lp) const {
+        const uint32_t o = lp - b0;
+        if (o <= 32) return (uint32_t)((buf << o) >> 32);
+        const uint64_t ab = 32 * w0 + lp;
+        const uint64_t wi = ab >> 5;
+        return __builtin_elementwise_fshl(sv.word(wi), sv.word(wi + 1), (uint32_t)(ab & 31));
+    }
+};
+
+// Decode with a decode index: thread g decodes segment g (kIdxSeg symbols)
+// from the segment's first codeword, reading the stream through a 64-bit
+// register window with the next word already loaded, so no LDS staging and
+// no synchronisation is needed; it writes its codes in 16-byte units of 8
+// symbols and its outliers from the segment's outlier rank on.  A segment
+// that does not end where the next one starts (an index that does not
+// belong to this stream) is an error.
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ ibits,
+                                                        const uint64_t* __restrict__ iouts, uint64_t nseg,
+                                                        uint64_t total_bits, uint16_t* __restrict__ codes,
+                                                        float* __restrict__ sparse, DecSummary* sum) {
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    dec.load(lds, a);
+    __syncthreads();
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    if (g >= nseg) return;
+    StreamView sv;
+    sv.setup(a.stream, a.stream_bits);
+    const uint64_t s0 = g * kIdxSeg;
+    const uint32_t ns = (uint32_t)min<uint64_t>(kIdxSeg, a.n - s0);
+    const uint64_t start = ibits[g];
+    const uint64_t end = g + 1 < nseg ? ibits[g + 1] : total_bits;
+    uint64_t rank = iouts[g];
+    bool bad = start > end || end > a.stream_bits;
+    const uint64_t abit = start + sv.shift0;
+    CursorWin cw;
+    cw.sv = sv;
+    cw.w0 = abit >> 5;
+    cw.b0 = 0;
+    cw.buf = ((uint64_t)sv.word(cw.w0) << 32) | sv.word(cw.w0 + 1);
+    // refill words come from 16-byte groups: the current one and the next,
+    // already loaded (about 14 codewords of decode ahead of their use)
+    auto load4 = [&](uint64_t q) -> uint4 {
+        if (16 * (q + 1) <= sv.nbytes) {
+            const uint4 v = reinterpret_cast<const uint4*>(sv.base)[q];
+            return make_uint4(__builtin_bswap32(v.x), __builtin_bswap32(v.y), __builtin_bswap32(v.z),
+                              __builtin_bswap32(v.w));
+        }
+        return make_uint4(sv.word(4 * q), sv.word(4 * q + 1), sv.word(4 * q + 2), sv.word(4 * q + 3));
+    };
+    uint64_t q4 = (cw.w0 + 2) >> 2;
+    uint32_t qi = (uint32_t)((cw.w0 + 2) & 3);
+    uint4 cur4 = load4(q4), nxt4 = load4(q4 + 1);
+    q4 += 2;
+    uint32_t lp = (uint32_t)(abit & 31);
+    for (uint32_t u = 0; u < ns; u += 8) {
+        uint32_t c[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            c[k] = 0;
+            if (u + k < ns) {
+                while (lp - cw.b0 >= 32) {
+                    const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
+                    cw.buf = (cw.buf << 32) | wv;
+                    cw.b0 += 32;
+                    if (++qi == 4) {
+                        cur4 = nxt4;
+                        nxt4 = load4(q4++);
+                        qi = 0;
+                    }
+                }
+                uint32_t len;
+                const uint32_t sym = dec.take_w(cw, lp, len);
+                bad |= len == 0;
+                lp += len ? len : 1;
+                c[k] = sym;
+                if (sym == 0) {
+                    if (rank < a.n_outlier)
+                        sparse[s0 + u + k] = read_f32_bytes(a.outliers + 4 * rank);
+                    else
+                        bad = true;
+                    rank++;
+                }
+            }
+        }
+        if (u + 8 <= ns) {
+            // codes is 16-byte aligned and s0 + u a multiple of 8
+            *reinterpret_cast<uint4*>(codes + s0 + u) =
+                make_uint4(c[0] | (c[1] << 16), c[2] | (c[3] << 16), c[4] | (c[5] << 16), c[6] | (c[7] << 16));
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                if (u + k < ns) codes[s0 + u + k] = (uint16_t)c[k];
+        }
+    }
+    // every segment ends where the next starts, the last at the stream's
+    // last codeword bit
+    bad |= 32 * cw.w0 + lp - sv.shift0 != end;
+    if (bad) atomicOr(&sum->error, 1u);
+}
+
 // Final decode.  Thread j decodes cnt[j] codewords from its true start b[j]
 // out of the workgroup's staged bits: its symbols are [O, E), O = sym_off[j].
 // Output goes out in whole 16-byte units of 8 symbols: the thread whose range
@@ -1116,6 +1227,22 @@ void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSumma
         hipLaunchKernelGGL(k_decode_pass<true>, grid, dim3(256), lds, st, a, s, pass, sum);
     else
         hipLaunchKernelGGL(k_decode_pass<false>, grid, dim3(256), lds, st, a, s, pass, sum);
+}
+void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
+                           float* sparse, DecSummary* sum, hipStream_t st) {
+    allow_big_lds(k_decode_indexed<true>);
+    allow_big_lds(k_decode_indexed<false>);
+    const bool t16 = a.lut16 != nullptr;
+    const size_t lds = table_lds_bytes(a, t16);
+    const uint64_t nseg = idx_segments(a.n);
+    const uint64_t* ibits = index + kIdxHead;
+    const dim3 grid((unsigned)std::max<uint64_t>(1, (nseg + 255) / 256));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_indexed<true>, grid, dim3(256), lds, st, a, ibits, ibits + nseg, nseg, total_bits,
+                           codes, sparse, sum);
+    else
+        hipLaunchKernelGGL(k_decode_indexed<false>, grid, dim3(256), lds, st, a, ibits, ibits + nseg, nseg,
+                           total_bits, codes, sparse, sum);
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {

@@ -308,19 +308,52 @@ class Context:
         self._check(self._lib.vecsz_compress_bound(ctypes.byref(params), ctypes.byref(b)))
         return b.value
 
-    def compress_into(self, params: Params, d_in_ptr: int, d_out_ptr: int, out_cap: int):
-        """Raw device pointers; returns (archive bytes, n_outlier)."""
+    def compress_into(self, params: Params, d_in_ptr: int, d_out_ptr: int, out_cap: int, index_ptr: int = 0,
+                      index_words: int = 0):
+        """Raw device pointers; returns (archive bytes, n_outlier).  With
+        index_ptr, also writes the decode index (index_words u64 words)."""
         size = ctypes.c_uint64()
         nout = ctypes.c_uint64()
-        rc = self._lib.vecsz_compress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr),
-                                      ctypes.c_void_p(d_out_ptr), out_cap, ctypes.byref(size), ctypes.byref(nout))
+        if index_ptr:
+            rc = self._lib.vecsz_compress_indexed(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr),
+                                                  ctypes.c_void_p(d_out_ptr), out_cap, ctypes.byref(size),
+                                                  ctypes.byref(nout), ctypes.c_void_p(index_ptr), index_words)
+        else:
+            rc = self._lib.vecsz_compress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr),
+                                          ctypes.c_void_p(d_out_ptr), out_cap, ctypes.byref(size), ctypes.byref(nout))
         self._check(rc)
         return size.value, nout.value
 
-    def decompress_into(self, params: Params, d_in_ptr: int, in_bytes: int, d_out_ptr: int, out_cap: int):
-        rc = self._lib.vecsz_decompress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr), in_bytes,
-                                        ctypes.c_void_p(d_out_ptr), out_cap)
+    def decompress_into(self, params: Params, d_in_ptr: int, in_bytes: int, d_out_ptr: int, out_cap: int,
+                        index_ptr: int = 0, index_words: int = 0):
+        if index_ptr:
+            rc = self._lib.vecsz_decompress_indexed(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr),
+                                                    in_bytes, ctypes.c_void_p(index_ptr), index_words,
+                                                    ctypes.c_void_p(d_out_ptr), out_cap)
+        else:
+            rc = self._lib.vecsz_decompress(self._ctx, ctypes.byref(params), ctypes.c_void_p(d_in_ptr), in_bytes,
+                                            ctypes.c_void_p(d_out_ptr), out_cap)
         self._check(rc)
+
+    def index_words(self, params: Params) -> int:
+        """u64 words of the decode index for a field of these dims."""
+        w = ctypes.c_uint64()
+        self._check(self._lib.vecsz_index_words(ctypes.byref(params), ctypes.byref(w)))
+        return w.value
+
+    def compress_indexed(self, x, eb: float = 1e-4, mode="abs", block: int = 8, dict_size: int = 4096,
+                         value_range: float = 0.0, pad_mode="none", pad_type="const", pad_constant: float = 0.0):
+        """compress() of a float32 CUDA tensor that also returns the decode
+        index: (archive, n_outlier, index as an int64 CUDA tensor)."""
+        if not x.is_cuda or x.dtype != torch.float32:
+            raise TypeError("expected a float32 CUDA tensor")
+        x = x.contiguous()
+        p = make_params(dims_of(x.shape), eb, mode, block, dict_size, value_range, pad_mode, pad_type, pad_constant)
+        cap = self.compress_bound(p)
+        out = torch.empty(cap, dtype=torch.uint8, device=x.device)
+        idx = torch.empty(self.index_words(p), dtype=torch.int64, device=x.device)
+        size, nout = self.compress_into(p, x.data_ptr(), out.data_ptr(), out.numel(), idx.data_ptr(), idx.numel())
+        return out[:size], nout, idx
 
     def compress_batch_into(self, params, d_in, d_out, caps):
         """Raw device pointers, one entry per field; returns [(bytes, n_outlier)]."""
@@ -410,11 +443,12 @@ class Context:
         return buf[:size.value].tobytes(), nout.value
 
     def decompress(self, archive, mode="abs", value_range: float = 0.0, block: int = 8, dict_size: int = 4096,
-                   out=None):
+                   out=None, index=None):
         """Decompress an archive payload.
 
         uint8 CUDA tensor -> float32 CUDA tensor (C-order shape from the archive);
-        bytes / numpy     -> numpy array."""
+        bytes / numpy     -> numpy array.  index: the decode index that
+        compress_indexed() returned with this archive (CUDA archives only)."""
         p = make_params((1,), 1e-4, mode, block, dict_size, value_range)
         if torch is not None and isinstance(archive, torch.Tensor):
             if not archive.is_cuda or archive.dtype != torch.uint8:
@@ -425,7 +459,11 @@ class Context:
             n = int(np.prod(shape))
             if out is None:
                 out = torch.empty(shape, dtype=torch.float32, device=archive.device)
-            self.decompress_into(p, archive.data_ptr(), archive.numel(), out.data_ptr(), out.numel())
+            if index is not None:
+                self.decompress_into(p, archive.data_ptr(), archive.numel(), out.data_ptr(), out.numel(),
+                                     index.data_ptr(), index.numel())
+            else:
+                self.decompress_into(p, archive.data_ptr(), archive.numel(), out.data_ptr(), out.numel())
             return out.view(shape) if out.numel() == n else out
         b = bytes(archive)
         info = archive_info(b)
