@@ -602,11 +602,17 @@ struct CursorWin {
 
 // Decode with a decode index: thread g decodes segment g (kIdxSeg symbols)
 // from the segment's first codeword, reading the stream through a 64-bit
-// register window with the next word already loaded, so no LDS staging and
-// no synchronisation is needed; it writes its codes in 16-byte units of 8
-// symbols and its outliers from the segment's outlier rank on.  A segment
-// that does not end where the next one starts (an index that does not
-// belong to this stream) is an error.
+// register window refilled from 16-byte groups loaded ahead, so no LDS
+// staging of the bits and no synchronisation is needed.  Codes leave through
+// LDS: every lane puts 32 symbols (four 16-byte units) in its wavefront's
+// stage, then the wavefront writes the stage so that each store instruction
+// covers 16 whole 64-byte pieces instead of 64 scattered 16-byte units.
+// Outliers are written from the segment's outlier rank on.  A segment that
+// does not end where the next one starts (an index that does not belong to
+// this stream) is an error.
+constexpr uint32_t kIdxFlush = 32;                       // symbols per lane between stage flushes
+constexpr uint32_t kIdxLaneWords = kIdxFlush / 2 + 1;    // u32 per lane in the stage (odd: no bank conflicts)
+constexpr uint32_t kIdxStageWords = 64 * kIdxLaneWords;  // per wavefront
 template <bool T16>
 __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ ibits,
                                                         const uint64_t* __restrict__ iouts, uint64_t nseg,
@@ -614,24 +620,27 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
                                                         float* __restrict__ sparse, DecSummary* sum) {
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
-    dec.load(lds, a);
+    const uint32_t tw = dec.load(lds, a);
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t* stage = lds + tw + (threadIdx.x >> 6) * kIdxStageWords;
     __syncthreads();
     const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
-    if (g >= nseg) return;
+    const uint64_t gw = g - lane;  // the wavefront's first segment
+    const bool live = g < nseg;
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     const uint64_t s0 = g * kIdxSeg;
-    const uint32_t ns = (uint32_t)min<uint64_t>(kIdxSeg, a.n - s0);
-    const uint64_t start = ibits[g];
-    const uint64_t end = g + 1 < nseg ? ibits[g + 1] : total_bits;
-    uint64_t rank = iouts[g];
+    const uint32_t ns = live ? (uint32_t)min<uint64_t>(kIdxSeg, a.n - s0) : 0u;
+    const uint64_t start = live ? ibits[g] : 0;
+    const uint64_t end = !live ? 0 : g + 1 < nseg ? ibits[g + 1] : total_bits;
+    uint64_t rank = live ? iouts[g] : 0;
     bool bad = start > end || end > a.stream_bits;
     const uint64_t abit = start + sv.shift0;
     CursorWin cw;
     cw.sv = sv;
     cw.w0 = abit >> 5;
     cw.b0 = 0;
-    cw.buf = ((uint64_t)sv.word(cw.w0) << 32) | sv.word(cw.w0 + 1);
+    cw.buf = live ? ((uint64_t)sv.word(cw.w0) << 32) | sv.word(cw.w0 + 1) : 0;
     // refill words come from 16-byte groups: the current one and the next,
     // already loaded (about 14 codewords of decode ahead of their use)
     auto load4 = [&](uint64_t q) -> uint4 {
@@ -644,52 +653,77 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     };
     uint64_t q4 = (cw.w0 + 2) >> 2;
     uint32_t qi = (uint32_t)((cw.w0 + 2) & 3);
-    uint4 cur4 = load4(q4), nxt4 = load4(q4 + 1);
+    uint4 cur4 = make_uint4(0, 0, 0, 0), nxt4 = make_uint4(0, 0, 0, 0);
+    if (live) {
+        cur4 = load4(q4);
+        nxt4 = load4(q4 + 1);
+    }
     q4 += 2;
     uint32_t lp = (uint32_t)(abit & 31);
-    for (uint32_t u = 0; u < ns; u += 8) {
-        uint32_t c[8];
+    // the wavefront's segments lie kIdxSeg symbols apart; the last field
+    // segment may be short, and only its whole units go through the stage
+    for (uint32_t f0 = 0; f0 < kIdxSeg; f0 += kIdxFlush) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            c[k] = 0;
-            if (u + k < ns) {
-                while (lp - cw.b0 >= 32) {
-                    const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
-                    cw.buf = (cw.buf << 32) | wv;
-                    cw.b0 += 32;
-                    if (++qi == 4) {
-                        cur4 = nxt4;
-                        nxt4 = load4(q4++);
-                        qi = 0;
+        for (uint32_t u = f0; u < f0 + kIdxFlush; u += 8) {
+            uint32_t c[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                c[k] = 0;
+                if (u + k < ns) {
+                    while (lp - cw.b0 >= 32) {
+                        const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
+                        cw.buf = (cw.buf << 32) | wv;
+                        cw.b0 += 32;
+                        if (++qi == 4) {
+                            cur4 = nxt4;
+                            nxt4 = load4(q4++);
+                            qi = 0;
+                        }
+                    }
+                    uint32_t len;
+                    const uint32_t sym = dec.take_w(cw, lp, len);
+                    bad |= len == 0;
+                    lp += len ? len : 1;
+                    c[k] = sym;
+                    if (sym == 0) {
+                        if (rank < a.n_outlier)
+                            sparse[s0 + u + k] = read_f32_bytes(a.outliers + 4 * rank);
+                        else
+                            bad = true;
+                        rank++;
                     }
                 }
-                uint32_t len;
-                const uint32_t sym = dec.take_w(cw, lp, len);
-                bad |= len == 0;
-                lp += len ? len : 1;
-                c[k] = sym;
-                if (sym == 0) {
-                    if (rank < a.n_outlier)
-                        sparse[s0 + u + k] = read_f32_bytes(a.outliers + 4 * rank);
-                    else
-                        bad = true;
-                    rank++;
-                }
+            }
+            uint32_t* d = stage + lane * kIdxLaneWords + (u - f0) / 2;
+#pragma unroll
+            for (int k = 0; k < 4; k++) d[k] = c[2 * k] | (c[2 * k + 1] << 16);
+            if (u < ns && u + 8 > ns) {  // a partial last unit: straight out
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (u + k < ns) codes[s0 + u + k] = (uint16_t)c[k];
             }
         }
-        if (u + 8 <= ns) {
-            // codes is 16-byte aligned and s0 + u a multiple of 8
-            *reinterpret_cast<uint4*>(codes + s0 + u) =
-                make_uint4(c[0] | (c[1] << 16), c[2] | (c[3] << 16), c[4] | (c[5] << 16), c[6] | (c[7] << 16));
-        } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // piece i of this lane's store: lane r's unit j, r = (lane + 64 i) / 4
 #pragma unroll
-            for (int k = 0; k < 8; k++)
-                if (u + k < ns) codes[s0 + u + k] = (uint16_t)c[k];
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t slot = lane + 64 * i, r = slot >> 2, j = slot & 3;
+            const uint64_t gr = gw + r;
+            const uint32_t nr = gr < nseg ? (uint32_t)min<uint64_t>(kIdxSeg, a.n - gr * kIdxSeg) : 0u;
+            if (f0 + 8 * j + 8 <= nr) {
+                const uint32_t* q = stage + r * kIdxLaneWords + 4 * j;
+                *reinterpret_cast<uint4*>(codes + gr * kIdxSeg + f0 + 8 * j) = make_uint4(q[0], q[1], q[2], q[3]);
+            }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // every segment ends where the next starts, the last at the stream's
     // last codeword bit
-    bad |= 32 * cw.w0 + lp - sv.shift0 != end;
+    if (live) bad |= 32 * cw.w0 + lp - sv.shift0 != end;
     if (bad) atomicOr(&sum->error, 1u);
 }
 
@@ -1233,7 +1267,7 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     allow_big_lds(k_decode_indexed<true>);
     allow_big_lds(k_decode_indexed<false>);
     const bool t16 = a.lut16 != nullptr;
-    const size_t lds = table_lds_bytes(a, t16);
+    const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
     const uint64_t nseg = idx_segments(a.n);
     const uint64_t* ibits = index + kIdxHead;
     const dim3 grid((unsigned)std::max<uint64_t>(1, (nseg + 255) / 256));
