@@ -706,10 +706,11 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // piece i of this lane's store: lane r's unit j, r = (lane + 64 i) / 4
+        // piece i of this lane's store: lane r's unit j, slot lane + 64 i
+        constexpr uint32_t kUnits = kIdxFlush / 8;  // 16-byte units per lane and flush
 #pragma unroll
-        for (uint32_t i = 0; i < 4; i++) {
-            const uint32_t slot = lane + 64 * i, r = slot >> 2, j = slot & 3;
+        for (uint32_t i = 0; i < kUnits; i++) {
+            const uint32_t slot = lane + 64 * i, r = slot / kUnits, j = slot % kUnits;
             const uint64_t gr = gw + r;
             const uint32_t nr = gr < nseg ? (uint32_t)min<uint64_t>(kIdxSeg, a.n - gr * kIdxSeg) : 0u;
             if (f0 + 8 * j + 8 <= nr) {
