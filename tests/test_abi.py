@@ -77,3 +77,19 @@ def test_no_gpu_context_fails_loudly():
 
     with pytest.raises(VecszError):
         Context(0)
+
+
+def test_index_words():
+    """Decode index size: a 6-word header plus a first bit and an outlier rank
+    per 256 symbols."""
+    from vecsz_amd import make_params
+
+    lib = _lib.load()
+    w = ctypes.c_uint64()
+    for dims, segs in (((1,), 1), ((256,), 1), ((257,), 2), ((512, 512, 512), 524288), ((3600, 1800), 25313)):
+        p = make_params(dims, 1e-4, "abs")
+        assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(w)) == 0
+        assert w.value == 6 + 2 * segs, dims
+    p = make_params((8,), 1e-4, "abs")
+    p.ndim = 0
+    assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(w)) != 0
