@@ -409,8 +409,8 @@ def main():
                     help="also run one plain device copy of the field (known bytes) to scale HBM counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one field shards by slabs; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (a field would shard by slabs of "
+              "Lorenzo blocks, with one histogram all-reduce). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -10,7 +10,8 @@
  * AVX2 or scalar code for that element, with no contraction and no
  * reassociation (FLT_EVAL_METHOD == 0 on x86-64).
  *
- * Parity status of each part (how it is pinned) is listed in DESIGN.md.
+ * Each part is pinned by the tests under tests/ (the compiled reference's
+ * outputs, golden fixtures, and closed-form and brute-force checks).
  */
 #include "vecsz_oracle.h"
 
@@ -65,7 +66,7 @@ static inline float prequant_scalar(float x, double r_d) { return (float)round((
 
 /* c_lorenzo_{1,2,3}d1l (src/dualquant.hh:18-913), restated per element.
  *
- * Which form each element takes (derivation in DESIGN.md):
+ * Which form each element takes (from the branches cited below):
  *  - 1D: AVX prequant everywhere (:65, :83-93).  Postquant uses the AVX test
  *    sqrt(e*e) < R in blocks with start + B < DIM0 (:114, :157-178), else the
  *    scalar test |e| < R (:190-205).

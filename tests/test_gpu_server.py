@@ -53,9 +53,9 @@ def test_server_results_match_standalone(gpu_ctx):
 
     reqs = _requests(gpu_ctx)
     srv = Server(0, slots=4)
-    print("mixed round 1", flush=True)
+    print("mixed pass 1", flush=True)
     _run_mixed(srv, reqs)
-    print("mixed round 2", flush=True)
+    print("mixed pass 2", flush=True)
     _run_mixed(srv, reqs)  # resident slot scratch reused
     srv.close()
 
