@@ -80,16 +80,18 @@ def test_no_gpu_context_fails_loudly():
 
 
 def test_index_words():
-    """Decode index size: a 6-word header plus a first bit and an outlier rank
-    per 256 symbols."""
+    """Decode index size: a 6-word header, two words per group of 64
+    segments of 128 symbols, then a u32 and a u16 per segment, each array
+    padded to whole u64 words."""
     from vecsz_amd import make_params
 
     lib = _lib.load()
     w = ctypes.c_uint64()
-    for dims, segs in (((1,), 1), ((256,), 1), ((257,), 2), ((512, 512, 512), 524288), ((3600, 1800), 25313)):
+    for dims, segs in (((1,), 1), ((128,), 1), ((129,), 2), ((512, 512, 512), 1 << 20), ((3600, 1800), 50625)):
         p = make_params(dims, 1e-4, "abs")
         assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(w)) == 0
-        assert w.value == 6 + 2 * segs, dims
+        groups = -(-segs // 64)
+        assert w.value == 6 + 2 * groups + -(-segs // 2) + -(-segs // 4), dims
     p = make_params((8,), 1e-4, "abs")
     p.ndim = 0
     assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(w)) != 0

@@ -89,9 +89,9 @@ def test_index_of_another_archive_is_refused(gpu_ctx):
     if int(idx_b[3]) != int(idx[3]):  # stream bits differ
         with pytest.raises(VecszError):
             gpu_ctx.decompress(a, "abs", index=idx_b)
-    # a segment start moved by one bit: the decode notices the misfit
+    # the segments of group 1 moved by one bit: the decode notices the misfit
     bad = idx.clone()
-    bad[6 + 3] += 1
+    bad[6 + 2] += 1
     with pytest.raises(VecszError):
         gpu_ctx.decompress(a, "abs", index=bad)
     # and the context is still good afterwards

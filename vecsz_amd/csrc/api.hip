@@ -479,7 +479,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         ih[0] = kIdxMagic;
         ih[1] = n;
         ih[2] = idx_segments(n);
-        ih[5] = kIdxSeg;
+        ih[5] = (uint64_t)kIdxSeg | ((uint64_t)kIdxGroup << 32);
         HIPCHK(c, hipMemcpyAsync(d_index, ih, 24, hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(d_index + 5, ih + 5, 8, hipMemcpyHostToDevice, c->stream));
     }
@@ -645,9 +645,10 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     const uint8_t* d_stream = d_huff + 8 + tb;
     const uint64_t stream_bytes = huff_size - 8 - tb;
     if (d_index) {
-        const uint64_t nseg = idx_segments(n);
-        if (ih[0] != kIdxMagic || ih[1] != n || ih[2] != nseg || ih[5] != (uint64_t)kIdxSeg ||
-            index_words < kIdxHead + 2 * nseg || (ih[3] + 7) / 8 != stream_bytes || ih[4] != nout)
+        const IdxLayout L = idx_layout(n);
+        if (ih[0] != kIdxMagic || ih[1] != n || ih[2] != L.nseg ||
+            ih[5] != ((uint64_t)kIdxSeg | ((uint64_t)kIdxGroup << 32)) || index_words < L.words ||
+            (ih[3] + 7) / 8 != stream_bytes || ih[4] != nout)
             return fail(c, VECSZ_EINVAL, "decode index does not belong to this archive");
     }
     const uint8_t* d_outl = d_huff + huff_size;
@@ -922,7 +923,7 @@ int vecsz_index_words(const vecsz_params* p, uint64_t* words) {
     if (!p || !words || p->ndim < 1 || p->ndim > 3) return VECSZ_EINVAL;
     uint64_t n = 1;
     for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
-    *words = kIdxHead + 2 * idx_segments(n);
+    *words = idx_layout(n).words;
     return VECSZ_OK;
 }
 

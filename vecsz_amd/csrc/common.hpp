@@ -98,13 +98,30 @@ struct BatchField {
 // Decode index: an optional side product of compress (never part of the
 // archive) that lets decompress start every kIdxSeg-symbol segment at its
 // first codeword instead of synchronising.  u64 words: [0] kIdxMagic, [1] n,
-// [2] segments, [3] stream bits, [4] outliers, [5] kIdxSeg, then the first
-// codeword bit of each segment (from the stream start), then the outlier rank
-// of each segment's first symbol.
-constexpr int kIdxSeg = 256;
+// [2] segments, [3] stream bits, [4] outliers, [5] kIdxSeg | kIdxGroup << 32;
+// then, per group of kIdxGroup segments, the first codeword bit (from the
+// stream start) and the outlier rank of its first segment (two u64); then
+// per segment its first bit relative to its group's (u32, < 2^19), then its
+// outlier rank relative to its group's (u16, < 2^13), each array padded to
+// whole words.
+constexpr int kIdxSeg = 128;
+constexpr int kIdxGroup = 64;
 constexpr int kIdxHead = 6;
-constexpr uint64_t kIdxMagic = 0x3158444956535A56ull;  // "VZSVIDX1" little-endian
+constexpr uint64_t kIdxMagic = 0x3258444956535A56ull;  // "VZSVIDX2" little-endian
+struct IdxLayout {
+    uint64_t nseg, ngrp, grp, rel_bits, rel_outs, words;  // counts and word offsets
+};
 __host__ __device__ inline uint64_t idx_segments(uint64_t n) { return (n + kIdxSeg - 1) / kIdxSeg; }
+__host__ __device__ inline IdxLayout idx_layout(uint64_t n) {
+    IdxLayout L;
+    L.nseg = n ? idx_segments(n) : 0;
+    L.ngrp = (L.nseg + kIdxGroup - 1) / kIdxGroup;
+    L.grp = kIdxHead;
+    L.rel_bits = L.grp + 2 * L.ngrp;
+    L.rel_outs = L.rel_bits + (L.nseg + 1) / 2;
+    L.words = L.rel_outs + (L.nseg + 3) / 4;
+    return L;
+}
 
 // Decoder summary.
 struct DecSummary {

@@ -987,7 +987,8 @@ __host__ __device__ constexpr uint32_t book_words(int book, int dict) {
 }
 
 // Packs chunk k (codes already in `cur`) into the archive; see k_pack.
-// index: decode-index segment entries (first bits, then outlier ranks), or null.
+// index: the decode index (layout L), or null; bit_off / out_off: the
+// field's chunk offsets (read for the index's group bases).
 template <int BOOK>
 __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint64_t nchunks, uint32_t lane,
                                            const uint32_t* tab, const uint8_t* tlen,
@@ -995,7 +996,8 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
                                            const uint8_t* __restrict__ len_tab, const EncSummary& S,
                                            const float* __restrict__ sparse, uint8_t* __restrict__ archive,
                                            uint32_t* stage, uint64_t* __restrict__ index = nullptr,
-                                           uint64_t nseg = 0) {
+                                           IdxLayout L = IdxLayout{}, const uint64_t* __restrict__ bit_off = nullptr,
+                                           const uint64_t* __restrict__ out_off = nullptr) {
     constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint64_t base = k * kChunk + lane * kSymPerThread;
@@ -1042,13 +1044,21 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
     const uint32_t cbits = lane63(incl) & 0x1FFFF;
 
     // decode index: lanes holding a segment's first symbol record its first
-    // codeword bit and outlier rank
+    // codeword bit and outlier rank, relative to its group's (the first
+    // segment of a group also records the group's)
     constexpr uint32_t kLanesPerSeg = kIdxSeg / kSymPerThread;
+    constexpr uint32_t kChunksPerGroup = kIdxGroup * kIdxSeg / kChunk;
     if (index && (lane % kLanesPerSeg) == 0) {
         const uint64_t g = k * (kChunk / kIdxSeg) + lane / kLanesPerSeg;
-        if (g < nseg) {
-            index[g] = cur.bit_off + (excl & 0x1FFFF);
-            index[nseg + g] = cur.out_off + (excl >> 17);
+        if (g < L.nseg) {
+            const uint64_t G = g / kIdxGroup, k0 = G * kChunksPerGroup;  // the group's first chunk
+            const uint64_t bits = cur.bit_off + (excl & 0x1FFFF), outs = cur.out_off + (excl >> 17);
+            if (g % kIdxGroup == 0) {
+                index[L.grp + 2 * G] = bits;
+                index[L.grp + 2 * G + 1] = outs;
+            }
+            reinterpret_cast<uint32_t*>(index + L.rel_bits)[g] = (uint32_t)(bits - bit_off[k0]);
+            reinterpret_cast<uint16_t*>(index + L.rel_outs)[g] = (uint16_t)(outs - out_off[k0]);
         }
     }
 
@@ -1169,7 +1179,7 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
                                               uint64_t nchunks, const EncSummary* __restrict__ sum,
                                               const float* __restrict__ sparse, uint8_t* __restrict__ archive,
                                               const uint32_t* __restrict__ err, uint32_t stage_words,
-                                              uint64_t* __restrict__ index, uint64_t nseg) {
+                                              uint64_t* __restrict__ index, IdxLayout L) {
     constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     if (*err) return;
     extern __shared__ uint32_t lds[];
@@ -1202,8 +1212,8 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     for (; k < nchunks; k += nw) {
         const ChunkRaw cur = nxt;
         if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k + nw, lane, fill, nxt);
-        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage, index,
-                         nseg);
+        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage, index, L,
+                         bit_off, out_off);
     }
 }
 
@@ -1681,8 +1691,7 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
                  int dict, const uint64_t* bit_off, const uint64_t* out_off, uint64_t nchunks, const EncSummary* sum,
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap,
                  uint64_t* index, hipStream_t st) {
-    uint64_t* ientries = index ? index + kIdxHead : nullptr;
-    const uint64_t nseg = index ? idx_segments(n) : 0;
+    const IdxLayout L = idx_layout(index ? n : 0);
     const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((nchunks + 3) / 4, grid_cap));
     // words one chunk can touch: 1024 codewords of at most max_len bits, plus
     // the partial words at either end
@@ -1690,7 +1699,7 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
     const int book = dict > kMaxLdsBook ? kBookGlobal : max_len <= 32 ? kBookSplit : kBookPacked;
     const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;
 #define PACK_ARGS \
-    codes, n, cw_hi, cw_lo, len_tab, dict, bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words, ientries, nseg
+    codes, n, cw_hi, cw_lo, len_tab, dict, bit_off, out_off, nchunks, sum, sparse, archive, err, stage_words, index, L
     if (book == kBookSplit)
         hipLaunchKernelGGL(k_pack<kBookSplit>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
     else if (book == kBookPacked)

@@ -614,9 +614,8 @@ constexpr uint32_t kIdxFlush = 32;                       // symbols per lane bet
 constexpr uint32_t kIdxLaneWords = kIdxFlush / 2 + 1;    // u32 per lane in the stage (odd: no bank conflicts)
 constexpr uint32_t kIdxStageWords = 64 * kIdxLaneWords;  // per wavefront
 template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ ibits,
-                                                        const uint64_t* __restrict__ iouts, uint64_t nseg,
-                                                        uint64_t total_bits, uint16_t* __restrict__ codes,
+__global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ index,
+                                                        IdxLayout L, uint64_t total_bits, uint16_t* __restrict__ codes,
                                                         float* __restrict__ sparse, DecSummary* sum) {
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
@@ -626,14 +625,19 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     __syncthreads();
     const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
     const uint64_t gw = g - lane;  // the wavefront's first segment
+    const uint64_t nseg = L.nseg;
     const bool live = g < nseg;
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     const uint64_t s0 = g * kIdxSeg;
     const uint32_t ns = live ? (uint32_t)min<uint64_t>(kIdxSeg, a.n - s0) : 0u;
-    const uint64_t start = live ? ibits[g] : 0;
-    const uint64_t end = !live ? 0 : g + 1 < nseg ? ibits[g + 1] : total_bits;
-    uint64_t rank = live ? iouts[g] : 0;
+    const uint64_t* grp = index + L.grp;
+    const uint32_t* rel_bits = reinterpret_cast<const uint32_t*>(index + L.rel_bits);
+    const uint16_t* rel_outs = reinterpret_cast<const uint16_t*>(index + L.rel_outs);
+    auto first_bit = [&](uint64_t h) { return grp[2 * (h / kIdxGroup)] + rel_bits[h]; };
+    const uint64_t start = live ? first_bit(g) : 0;
+    const uint64_t end = !live ? 0 : g + 1 < nseg ? first_bit(g + 1) : total_bits;
+    uint64_t rank = live ? grp[2 * (g / kIdxGroup) + 1] + rel_outs[g] : 0;
     bool bad = start > end || end > a.stream_bits;
     const uint64_t abit = start + sv.shift0;
     CursorWin cw;
@@ -1269,15 +1273,14 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     allow_big_lds(k_decode_indexed<false>);
     const bool t16 = a.lut16 != nullptr;
     const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
-    const uint64_t nseg = idx_segments(a.n);
-    const uint64_t* ibits = index + kIdxHead;
-    const dim3 grid((unsigned)std::max<uint64_t>(1, (nseg + 255) / 256));
+    const IdxLayout L = idx_layout(a.n);
+    const dim3 grid((unsigned)std::max<uint64_t>(1, (L.nseg + 255) / 256));
     if (t16)
-        hipLaunchKernelGGL(k_decode_indexed<true>, grid, dim3(256), lds, st, a, ibits, ibits + nseg, nseg, total_bits,
-                           codes, sparse, sum);
+        hipLaunchKernelGGL(k_decode_indexed<true>, grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse,
+                           sum);
     else
-        hipLaunchKernelGGL(k_decode_indexed<false>, grid, dim3(256), lds, st, a, ibits, ibits + nseg, nseg,
-                           total_bits, codes, sparse, sum);
+        hipLaunchKernelGGL(k_decode_indexed<false>, grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse,
+                           sum);
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {
