@@ -98,3 +98,29 @@ def test_index_of_another_archive_is_refused(gpu_ctx):
     y0 = gpu_ctx.decompress(a, "abs")
     y = gpu_ctx.decompress(a, "abs", index=idx)
     assert torch.equal(y.view(torch.int32), y0.view(torch.int32))
+
+
+def test_indexed_1d_reconstructs_in_the_decode(gpu_ctx):
+    """1D fields with an index reconstruct inside the decode (one pass, floats
+    out): every block size, ragged lengths around the 128-symbol segment,
+    padding modes, all outliers, and an output that is not 16-byte aligned
+    (which takes the two-pass path) give the self-synchronising output."""
+    from vecsz_amd import VecszError, datagen as dg
+
+    for n in (2, 127, 128, 129, 1000, (1 << 20) + 13):
+        w = dg.walk_spikes_1d(n, seed=n % 97, device="cuda")
+        for block in (8, 16, 32, 64):
+            _check(gpu_ctx, w, 1e-4, "r2r", block=block)
+    w = dg.walk_spikes_1d(300_001, seed=6, device="cuda")
+    for pm, pt in (("global", "max"), ("block", "avg"), ("edge", "min"), ("block", "const")):
+        _check(gpu_ctx, w, 1e-3, "r2r", block=16, pad_mode=pm, pad_type=pt)
+    _check(gpu_ctx, torch.randn(5003, device="cuda") * 100, 1e-6, "abs")  # all outliers
+    _check(gpu_ctx, w, 1e-5, "abs", dict_size=65536)
+    a, idx, y = _check(gpu_ctx, w, 1e-4, "abs")
+    buf = torch.empty(w.numel() + 1, dtype=torch.float32, device="cuda")
+    y1 = gpu_ctx.decompress(a, "abs", index=idx, out=buf[1:])
+    assert torch.equal(y1.view(torch.int32), y.view(torch.int32))
+    bad = idx.clone()
+    bad[6 + 2] += 1
+    with pytest.raises(VecszError):
+        gpu_ctx.decompress(a, "abs", index=bad)

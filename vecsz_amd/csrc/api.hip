@@ -655,6 +655,31 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     uint16_t* codes = c->codes.as<uint16_t>();
     float* sparse = c->sparse.as<float>();
     bool indexed = false;  // decode errors then show at the end
+    // Pad halo of the reconstruction.
+    const uint64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
+    ReconArgs ra;
+    ra.halo = nullptr;
+    ra.halo_stride = 0;
+    ra.hb0 = ref_nblk(d0, B);
+    ra.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
+    if (pad_len > 0) {
+        // padding mode from the pad value count (src/vecsz_interface.hh:287-298);
+        // 1D without a mode still reads pad_vals[0] (src/dualquant.hh:925-927)
+        const uint64_t nbref = ref_blocks(ndim, dims, B);
+        const int mode = pad_len == 1 ? VECSZ_PAD_GLOBAL
+                         : pad_len == nbref ? VECSZ_PAD_BLOCK
+                         : pad_len == (uint64_t)ndim * nbref ? VECSZ_PAD_EDGE : VECSZ_PAD_NONE;
+        if (mode != VECSZ_PAD_NONE || ndim == 1) {
+            if (!c->padv.ensure(pad_len * 4 + 16) || !c->halo.ensure(nbref * 12 + 16))
+                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+            HIPCHK(c, hipMemcpyAsync(c->padv.p, d_in + mb, 4 * pad_len, hipMemcpyDeviceToDevice, c->stream));
+            launch_pad_halo(ndim, mode, c->padv.as<float>(), pad_len, nbref, c->halo.as<float>(), c->stream);
+            ra.halo = c->halo.as<float>();
+            ra.halo_stride = 3;
+        }
+    }
+    // 1D with a decode index: the reconstruction runs inside the decode
+    const bool fused_1d = d_index && ndim == 1 && !dt.root_leaf && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
 
     if (dt.root_leaf) {
         StageScope s(c, "decode");
@@ -717,7 +742,8 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         DecSummary* dsum = c->dec_sum.as<DecSummary>();
         if (d_index) {
             StageScope s(c, "decode_indexed");
-            launch_decode_indexed(da, d_index, ih[3], codes, sparse, dsum, c->stream);
+            Recon1D r1{d_out, B, R, eb.twoeb, ra.halo, ra.halo_stride};
+            launch_decode_indexed(da, d_index, ih[3], codes, sparse, fused_1d ? &r1 : nullptr, dsum, c->stream);
             HIPCHK(c, hipGetLastError());
             indexed = true;
         } else {
@@ -772,28 +798,6 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     }
 
     // Lorenzo reconstruction.
-    const uint64_t d0 = dims[0], d1 = dims[1], d2 = dims[2];
-    ReconArgs ra;
-    ra.halo = nullptr;
-    ra.halo_stride = 0;
-    ra.hb0 = ref_nblk(d0, B);
-    ra.hb1 = ndim >= 2 ? ref_nblk(d1, B) : 1;
-    if (pad_len > 0) {
-        // padding mode from the pad value count (src/vecsz_interface.hh:287-298);
-        // 1D without a mode still reads pad_vals[0] (src/dualquant.hh:925-927)
-        const uint64_t nbref = ref_blocks(ndim, dims, B);
-        const int mode = pad_len == 1 ? VECSZ_PAD_GLOBAL
-                         : pad_len == nbref ? VECSZ_PAD_BLOCK
-                         : pad_len == (uint64_t)ndim * nbref ? VECSZ_PAD_EDGE : VECSZ_PAD_NONE;
-        if (mode != VECSZ_PAD_NONE || ndim == 1) {
-            if (!c->padv.ensure(pad_len * 4 + 16) || !c->halo.ensure(nbref * 12 + 16))
-                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-            HIPCHK(c, hipMemcpyAsync(c->padv.p, d_in + mb, 4 * pad_len, hipMemcpyDeviceToDevice, c->stream));
-            launch_pad_halo(ndim, mode, c->padv.as<float>(), pad_len, nbref, c->halo.as<float>(), c->stream);
-            ra.halo = c->halo.as<float>();
-            ra.halo_stride = 3;
-        }
-    }
     ra.d0 = d0;
     ra.d1 = d1;
     ra.d2 = d2;
@@ -804,12 +808,12 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     ra.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
     ra.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
     const uint64_t nblocks = ((d0 + B - 1) / B) * (ndim >= 2 ? (d1 + B - 1) / B : 1) * (ndim >= 3 ? (d2 + B - 1) / B : 1);
-    if (!c->flagged.ensure(nblocks * 4 + 16) || !c->nflagged.ensure(16))
-        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-    HIPCHK(c, hipMemsetAsync(c->nflagged.p, 0, 4, c->stream));
     const uint64_t work = ra.ntiles_x * ra.ntiles_y * ra.ntiles_z;
     const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 2048));
-    {
+    if (!fused_1d) {
+        if (!c->flagged.ensure(nblocks * 4 + 16) || !c->nflagged.ensure(16))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        HIPCHK(c, hipMemsetAsync(c->nflagged.p, 0, 4, c->stream));
         StageScope s(c, "reconstruct");
         launch_reconstruct(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(), c->nflagged.as<uint32_t>(),
                            grid, c->stream);

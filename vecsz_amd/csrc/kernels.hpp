@@ -96,9 +96,20 @@ void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_of
 void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st);
 // Decode with a decode index (index = its kIdxHead header words first): codes
 // and outliers as k_decode_write leaves them.
-// total_bits: the stream's bit count from the index header.
+// 1D reconstruction fused into the indexed decode: output (16-byte
+// aligned), block size, radius, 2 eb, and the halo (3 floats per block, x
+// face last) or null for +0.0.
+struct Recon1D {
+    float* out;
+    int B, radius;
+    double twoeb;
+    const float* halo;
+    uint64_t halo_stride;
+};
+// total_bits: the stream's bit count from the index header.  r1d null: codes
+// and outliers for the reconstruction kernels; else the 1D field itself.
 void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
-                           float* sparse, DecSummary* sum, hipStream_t st);
+                           float* sparse, const Recon1D* r1d, DecSummary* sum, hipStream_t st);
 void launch_scatter_all_outliers(const uint8_t* outliers, uint64_t n, float* sparse, hipStream_t st);
 
 struct ReconArgs {

@@ -611,12 +611,18 @@ struct CursorWin {
 // does not end where the next one starts (an index that does not belong to
 // this stream) is an error.
 constexpr uint32_t kIdxFlush = 32;                       // symbols per lane between stage flushes
-constexpr uint32_t kIdxLaneWords = kIdxFlush / 2 + 1;    // u32 per lane in the stage (odd: no bank conflicts)
+constexpr uint32_t kIdxLaneWords = 16 + 1;               // u32 per lane in the stage (odd: no bank conflicts)
 constexpr uint32_t kIdxStageWords = 64 * kIdxLaneWords;  // per wavefront
-template <bool T16>
+// R1D: the 1D Lorenzo reconstruction done in the same pass (a segment holds
+// whole blocks for every B), writing floats instead of codes.
+template <bool T16, bool R1D>
 __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ index,
                                                         IdxLayout L, uint64_t total_bits, uint16_t* __restrict__ codes,
-                                                        float* __restrict__ sparse, DecSummary* sum) {
+                                                        float* __restrict__ sparse, Recon1D r, DecSummary* sum) {
+    // symbols per lane between flushes (16 words of stage either way) and
+    // per 16-byte piece of output
+    constexpr uint32_t kFlush = R1D ? 16 : kIdxFlush;
+    constexpr uint32_t kPer16 = R1D ? 4 : 8;
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     const uint32_t tw = dec.load(lds, a);
@@ -664,15 +670,18 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     }
     q4 += 2;
     uint32_t lp = (uint32_t)(abit & 31);
+    float sprev = 0.f;  // R1D: the previous element's value in the block
     // the wavefront's segments lie kIdxSeg symbols apart; the last field
-    // segment may be short, and only its whole units go through the stage
-    for (uint32_t f0 = 0; f0 < kIdxSeg; f0 += kIdxFlush) {
+    // segment may be short, and only its whole pieces go through the stage
+    for (uint32_t f0 = 0; f0 < kIdxSeg; f0 += kFlush) {
 #pragma unroll
-        for (uint32_t u = f0; u < f0 + kIdxFlush; u += 8) {
+        for (uint32_t u = f0; u < f0 + kFlush; u += 8) {
             uint32_t c[8];
+            float o[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 c[k] = 0;
+                o[k] = 0.f;
                 if (u + k < ns) {
                     while (lp - cw.b0 >= 32) {
                         const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
@@ -689,37 +698,65 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
                     bad |= len == 0;
                     lp += len ? len : 1;
                     c[k] = sym;
+                    float v = 0.f;
                     if (sym == 0) {
                         if (rank < a.n_outlier)
-                            sparse[s0 + u + k] = read_f32_bytes(a.outliers + 4 * rank);
+                            v = read_f32_bytes(a.outliers + 4 * rank);
                         else
                             bad = true;
                         rank++;
+                        if constexpr (!R1D) sparse[s0 + u + k] = v;
+                    }
+                    if constexpr (R1D) {
+                        // x_lorenzo_1d1l (src/dualquant.hh:915-931): a block's
+                        // first element predicts from the halo (+0.0 or the
+                        // block's pad value), the others from their predecessor
+                        const uint64_t x = s0 + u + k;
+                        if ((x % (uint64_t)r.B) == 0) sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
+                        const float sx = sym == 0 ? v : sprev + (float)((int32_t)sym - r.radius);
+                        sprev = sx;
+                        o[k] = (float)((double)sx * r.twoeb);
                     }
                 }
             }
-            uint32_t* d = stage + lane * kIdxLaneWords + (u - f0) / 2;
+            uint32_t* d = stage + lane * kIdxLaneWords + (u - f0) * 16 / kFlush;
+            if constexpr (R1D) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) d[k] = c[2 * k] | (c[2 * k + 1] << 16);
-            if (u < ns && u + 8 > ns) {  // a partial last unit: straight out
+                for (int k = 0; k < 8; k++) d[k] = __float_as_uint(o[k]);
+            } else {
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if (u + k < ns) codes[s0 + u + k] = (uint16_t)c[k];
+                for (int k = 0; k < 4; k++) d[k] = c[2 * k] | (c[2 * k + 1] << 16);
+            }
+            // pieces that end past a short last segment: straight out
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const uint32_t i = u + k;
+                if (i < ns && (i / kPer16 + 1) * kPer16 > ns) {
+                    if constexpr (R1D)
+                        r.out[s0 + i] = o[k];
+                    else
+                        codes[s0 + i] = (uint16_t)c[k];
+                }
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // piece i of this lane's store: lane r's unit j, slot lane + 64 i
-        constexpr uint32_t kUnits = kIdxFlush / 8;  // 16-byte units per lane and flush
+        // piece i of this lane's store: lane q's piece j, slot lane + 64 i
 #pragma unroll
-        for (uint32_t i = 0; i < kUnits; i++) {
-            const uint32_t slot = lane + 64 * i, r = slot / kUnits, j = slot % kUnits;
-            const uint64_t gr = gw + r;
+        for (uint32_t i = 0; i < 4; i++) {
+            const uint32_t slot = lane + 64 * i, q = slot / 4, j = slot % 4;
+            const uint64_t gr = gw + q;
             const uint32_t nr = gr < nseg ? (uint32_t)min<uint64_t>(kIdxSeg, a.n - gr * kIdxSeg) : 0u;
-            if (f0 + 8 * j + 8 <= nr) {
-                const uint32_t* q = stage + r * kIdxLaneWords + 4 * j;
-                *reinterpret_cast<uint4*>(codes + gr * kIdxSeg + f0 + 8 * j) = make_uint4(q[0], q[1], q[2], q[3]);
+            if (f0 + kPer16 * (j + 1) <= nr) {
+                const uint32_t* w = stage + q * kIdxLaneWords + 4 * j;
+                const uint4 val = make_uint4(w[0], w[1], w[2], w[3]);
+                // codes and the 1D output are 16-byte aligned; gr * kIdxSeg + f0 + kPer16 * j is a
+                // multiple of kPer16
+                if constexpr (R1D)
+                    *reinterpret_cast<uint4*>(r.out + gr * kIdxSeg + f0 + kPer16 * j) = val;
+                else
+                    *reinterpret_cast<uint4*>(codes + gr * kIdxSeg + f0 + kPer16 * j) = val;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1268,19 +1305,30 @@ void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSumma
         hipLaunchKernelGGL(k_decode_pass<false>, grid, dim3(256), lds, st, a, s, pass, sum);
 }
 void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
-                           float* sparse, DecSummary* sum, hipStream_t st) {
-    allow_big_lds(k_decode_indexed<true>);
-    allow_big_lds(k_decode_indexed<false>);
+                           float* sparse, const Recon1D* r1d, DecSummary* sum, hipStream_t st) {
+    allow_big_lds((k_decode_indexed<true, false>));
+    allow_big_lds((k_decode_indexed<false, false>));
+    allow_big_lds((k_decode_indexed<true, true>));
+    allow_big_lds((k_decode_indexed<false, true>));
     const bool t16 = a.lut16 != nullptr;
     const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
     const IdxLayout L = idx_layout(a.n);
     const dim3 grid((unsigned)std::max<uint64_t>(1, (L.nseg + 255) / 256));
-    if (t16)
-        hipLaunchKernelGGL(k_decode_indexed<true>, grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse,
-                           sum);
-    else
-        hipLaunchKernelGGL(k_decode_indexed<false>, grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse,
-                           sum);
+    const Recon1D r = r1d ? *r1d : Recon1D{};
+#define DI(T, R) \
+    hipLaunchKernelGGL((k_decode_indexed<T, R>), grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse, r, sum)
+    if (r1d) {
+        if (t16)
+            DI(true, true);
+        else
+            DI(false, true);
+    } else {
+        if (t16)
+            DI(true, false);
+        else
+            DI(false, false);
+    }
+#undef DI
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
                        uint64_t* totals, hipStream_t st) {
