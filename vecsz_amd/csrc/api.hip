@@ -312,7 +312,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint64_t nchunks = (n + kChunk - 1) / kChunk;
 
     if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
-        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) || !c->range.ensure(16) ||
+        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) || !c->range.ensure(8 + 8 * (size_t)kRangeParts) ||
         !c->ebdev.ensure(sizeof(EbDev)) || !c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
         !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
         !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
@@ -341,7 +341,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         // min/max and the eb family stay on the device: no stop before the dual quant
         StageScope s(c, "range");
         launch_range(d_in, n, c->range.as<uint32_t>(), c->stream);
-        launch_ebfam(c->range.as<uint32_t>(), d_in, p->eb, c->ebdev.as<EbDev>(), c->stream);
+        launch_ebfam(c->range.as<uint32_t>(), n, d_in, p->eb, c->ebdev.as<EbDev>(), c->stream);
         a.ebd = c->ebdev.as<EbDev>();
         a.r_f = 0;
         a.r_d = 0;

@@ -10,8 +10,11 @@ constexpr int kMaxLdsLenTab = 16384;    // LDS code-length table up to this dict
 constexpr int kMaxLdsBook = 4096;       // LDS codebook (k_pack) up to this dict size
 
 int dq_lds_bytes(int dict);
+// out: 2 + 2 * kRangeParts words (min/max, then one pair per workgroup).
+constexpr uint32_t kRangeParts = 1024;
+uint32_t range_parts(uint64_t n);
 void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st);
-void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hipStream_t st);
+void launch_ebfam(uint32_t* mm, uint64_t n, const float* x, double eb, EbDev* out, hipStream_t st);
 void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* sparse, unsigned long long* hist,
                       uint32_t* partials, const DQArgs& a, int grid, hipStream_t st);
 // 4-wide tiles (256 x per wavefront) for 2D/3D fields at least 128 wide;

@@ -48,6 +48,11 @@ __device__ __forceinline__ float prequant_scalar(float x, double r_d) {
 
 // Min/max ignoring NaN; the host applies the first-element rule of
 // std::max_element/min_element (a NaN x[0] makes the range NaN).
+// Per-workgroup min/max (ordered-integer form) into out[2 + 2 b], out[3 + 2 b];
+// k_ebfam reduces them.  Partials rather than atomics on one address: a
+// thousand workgroups contending for one L2 line cost more than the reads.
+// (Plain loads: the dual quantization walks the field backwards next and
+// finds its tail still in the caches.)
 __global__ void __launch_bounds__(256) k_range(const float* __restrict__ x, uint64_t n, uint32_t* __restrict__ out) {
     float mn = __builtin_inff(), mx = -__builtin_inff();
     // float4 loads need 16-byte alignment; an unaligned field takes the scalar tail path
@@ -93,8 +98,10 @@ __global__ void __launch_bounds__(256) k_range(const float* __restrict__ x, uint
             mn = fminf(mn, smn[i]);
             mx = fmaxf(mx, smx[i]);
         }
-        if (mn == mn) atomicMin(&out[0], f2ord(mn));
-        if (mx == mx) atomicMax(&out[1], f2ord(mx));
+        // fminf/fmaxf skip NaNs, so a block of NaNs leaves +inf / -inf,
+        // which every other value overrides
+        out[2 + 2 * blockIdx.x] = f2ord(mn);
+        out[3 + 2 * blockIdx.x] = f2ord(mx);
     }
 }
 
@@ -122,8 +129,34 @@ __device__ __forceinline__ void ebfam(const uint32_t* __restrict__ mm, const flo
     *out = e;
 }
 
-__global__ void k_ebfam(const uint32_t* __restrict__ mm, const float* __restrict__ x, double eb, EbDev* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ebfam(mm, x, eb, out);
+// Reduces k_range's nparts partials into mm[0], mm[1], then the eb family.
+__global__ void __launch_bounds__(256) k_ebfam(uint32_t* __restrict__ mm, uint32_t nparts, const float* __restrict__ x,
+                                               double eb, EbDev* __restrict__ out) {
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        lo = min(lo, mm[2 + 2 * i]);
+        hi = max(hi, mm[3 + 2 * i]);
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, d));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, d));
+    }
+    __shared__ uint32_t slo[4], shi[4];
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        slo[w] = lo;
+        shi[w] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+            lo = min(lo, slo[i]);
+            hi = max(hi, shi[i]);
+        }
+        mm[0] = lo;
+        mm[1] = hi;
+        ebfam(mm, x, eb, out);
+    }
 }
 
 // Replaces a's r_f / r_d by the device-computed family when there is one.
@@ -1567,22 +1600,16 @@ int dq_lds_bytes(int dict) {
     return (kHotBins * 64 + (dict <= kMaxLdsHistBins ? dict : 0)) * 4;
 }
 
-__global__ void k_range_init(uint32_t* out) {
-    if (threadIdx.x == 0) {
-        out[0] = 0xFFFFFFFFu;
-        out[1] = 0u;
-    }
+uint32_t range_parts(uint64_t n) {
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n / 4 + 255) / 256, 1), kRangeParts);
 }
 
 void launch_range(const float* x, uint64_t n, uint32_t* out, hipStream_t st) {
-    uint64_t n4 = (n / 4 + 255) / 256;
-    int grid = (int)std::min<uint64_t>(std::max<uint64_t>(n4, 1), 2048);
-    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(64), 0, st, out);
-    hipLaunchKernelGGL(k_range, dim3(grid), dim3(256), 0, st, x, n, out);
+    hipLaunchKernelGGL(k_range, dim3(range_parts(n)), dim3(256), 0, st, x, n, out);
 }
 
-void launch_ebfam(const uint32_t* mm, const float* x, double eb, EbDev* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_ebfam, dim3(1), dim3(64), 0, st, mm, x, eb, out);
+void launch_ebfam(uint32_t* mm, uint64_t n, const float* x, double eb, EbDev* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_ebfam, dim3(1), dim3(256), 0, st, mm, range_parts(n), x, eb, out);
 }
 
 bool dualquant_x4_aligned(const float* in, uint64_t d0) {
