@@ -161,7 +161,11 @@ struct Decoder<true> {
     }
     template <typename W>
     __device__ __forceinline__ uint32_t take_w(const W& win, uint32_t lp, uint32_t& len) const {
-        const uint32_t w = win(lp);
+        return take_v(win, win(lp), lp, len);
+    }
+    // w: the 32 bits from lp (the window is read again only past them)
+    template <typename W>
+    __device__ __forceinline__ uint32_t take_v(const W& win, uint32_t w, uint32_t lp, uint32_t& len) const {
         uint32_t e = lut[w >> (32 - kLutBits)];
         uint32_t l = e >> 12;
         if (l == 0) {
@@ -202,7 +206,10 @@ struct Decoder<false> {
     }
     template <typename W>
     __device__ __forceinline__ uint32_t take_w(const W& win, uint32_t lp, uint32_t& len) const {
-        const uint32_t w = win(lp);
+        return take_v(win, win(lp), lp, len);
+    }
+    template <typename W>
+    __device__ __forceinline__ uint32_t take_v(const W& win, uint32_t w, uint32_t lp, uint32_t& len) const {
         uint32_t e = lut[w >> (32 - kLutBits)];
         if (e & kLutSlow) {
             uint32_t used = kLutBits;
@@ -670,53 +677,73 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     }
     q4 += 2;
     uint32_t lp = (uint32_t)(abit & 31);
+    uint32_t lp_end = lp;  // lp after the segment's ns codewords
     float sprev = 0.f;  // R1D: the previous element's value in the block
     // the wavefront's segments lie kIdxSeg symbols apart; the last field
     // segment may be short, and only its whole pieces go through the stage
     for (uint32_t f0 = 0; f0 < kIdxSeg; f0 += kFlush) {
 #pragma unroll
         for (uint32_t u = f0; u < f0 + kFlush; u += 8) {
+            // Every lane decodes 8 codewords; a lane whose (last, short)
+            // segment has ended decodes on past it harmlessly (bounded reads,
+            // nothing kept): its errors, outliers and end position count only
+            // up to ns.  Outliers wait for the end of the 8.
             uint32_t c[8];
             float o[8];
+            uint32_t omask = 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                c[k] = 0;
-                o[k] = 0.f;
-                if (u + k < ns) {
-                    while (lp - cw.b0 >= 32) {
-                        const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
-                        cw.buf = (cw.buf << 32) | wv;
-                        cw.b0 += 32;
-                        if (++qi == 4) {
-                            cur4 = nxt4;
-                            nxt4 = load4(q4++);
-                            qi = 0;
-                        }
+                while (lp - cw.b0 >= 32) {
+                    const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
+                    cw.buf = (cw.buf << 32) | wv;
+                    cw.b0 += 32;
+                    if (++qi == 4) {
+                        cur4 = nxt4;
+                        nxt4 = load4(q4++);
+                        qi = 0;
                     }
-                    uint32_t len;
-                    const uint32_t sym = dec.take_w(cw, lp, len);
-                    bad |= len == 0;
-                    lp += len ? len : 1;
-                    c[k] = sym;
-                    float v = 0.f;
-                    if (sym == 0) {
+                }
+                // lp - cw.b0 < 32 here: the codeword's first 32 bits are in buf
+                const uint32_t w = (uint32_t)((cw.buf << (lp - cw.b0)) >> 32);
+                uint32_t len;
+                const uint32_t sym = dec.take_v(cw, w, lp, len);
+                const bool in = u + k < ns;
+                bad |= in & (len == 0);
+                lp += len ? len : 1;
+                lp_end = u + k + 1 == ns ? lp : lp_end;
+                c[k] = sym;
+                omask |= (uint32_t)(in & (sym == 0)) << k;
+            }
+            float ov[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) ov[k] = 0.f;
+            if (omask) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    if ((omask >> k) & 1) {
+                        float v = 0.f;
                         if (rank < a.n_outlier)
                             v = read_f32_bytes(a.outliers + 4 * rank);
                         else
                             bad = true;
                         rank++;
+                        ov[k] = v;
                         if constexpr (!R1D) sparse[s0 + u + k] = v;
                     }
-                    if constexpr (R1D) {
-                        // x_lorenzo_1d1l (src/dualquant.hh:915-931): a block's
-                        // first element predicts from the halo (+0.0 or the
-                        // block's pad value), the others from their predecessor
-                        const uint64_t x = s0 + u + k;
-                        if ((x % (uint64_t)r.B) == 0) sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
-                        const float sx = sym == 0 ? v : sprev + (float)((int32_t)sym - r.radius);
-                        sprev = sx;
-                        o[k] = (float)((double)sx * r.twoeb);
-                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                o[k] = 0.f;
+                if constexpr (R1D) {
+                    // x_lorenzo_1d1l (src/dualquant.hh:915-931): a block's
+                    // first element predicts from the halo (+0.0 or the
+                    // block's pad value), the others from their predecessor
+                    const uint64_t x = s0 + u + k;
+                    if ((x % (uint64_t)r.B) == 0) sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
+                    const float sx = c[k] == 0 ? ov[k] : sprev + (float)((int32_t)c[k] - r.radius);
+                    sprev = sx;
+                    o[k] = (float)((double)sx * r.twoeb);
                 }
             }
             uint32_t* d = stage + lane * kIdxLaneWords + (u - f0) * 16 / kFlush;
@@ -728,14 +755,16 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
                 for (int k = 0; k < 4; k++) d[k] = c[2 * k] | (c[2 * k + 1] << 16);
             }
             // pieces that end past a short last segment: straight out
+            if (ns != kIdxSeg) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t i = u + k;
-                if (i < ns && (i / kPer16 + 1) * kPer16 > ns) {
-                    if constexpr (R1D)
-                        r.out[s0 + i] = o[k];
-                    else
-                        codes[s0 + i] = (uint16_t)c[k];
+                for (int k = 0; k < 8; k++) {
+                    const uint32_t i = u + k;
+                    if (i < ns && (i / kPer16 + 1) * kPer16 > ns) {
+                        if constexpr (R1D)
+                            r.out[s0 + i] = o[k];
+                        else
+                            codes[s0 + i] = (uint16_t)c[k];
+                    }
                 }
             }
         }
@@ -765,7 +794,7 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     }
     // every segment ends where the next starts, the last at the stream's
     // last codeword bit
-    if (live) bad |= 32 * cw.w0 + lp - sv.shift0 != end;
+    if (live) bad |= 32 * cw.w0 + lp_end - sv.shift0 != end;
     if (bad) atomicOr(&sum->error, 1u);
 }
 
