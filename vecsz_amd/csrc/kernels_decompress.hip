@@ -693,6 +693,21 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
             uint32_t omask = 0;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
+                // one refill covers codewords of at most 32 bits (selects, no
+                // branch); the 16-byte reload follows every fourth
+                {
+                    const bool need = lp - cw.b0 >= 32;
+                    const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
+                    cw.buf = need ? (cw.buf << 32) | wv : cw.buf;
+                    cw.b0 += need ? 32u : 0u;
+                    qi += need ? 1u : 0u;
+                    if (qi == 4) {
+                        cur4 = nxt4;
+                        nxt4 = load4(q4++);
+                        qi = 0;
+                    }
+                }
+                // after a longer codeword, more
                 while (lp - cw.b0 >= 32) {
                     const uint32_t wv = qi == 0 ? cur4.x : qi == 1 ? cur4.y : qi == 2 ? cur4.z : cur4.w;
                     cw.buf = (cw.buf << 32) | wv;
