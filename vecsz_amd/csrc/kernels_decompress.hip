@@ -862,16 +862,21 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
     if (W0 < E) {
         const uint64_t W1 = min((E + 7) & ~7ull, a.n);
         for (; i + 8 <= W1; i += 8) {
-            uint32_t w[4];
+            // outliers are rare: one test per unit, then in symbol order
+            uint32_t w[4], omask = 0;
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const uint32_t lo = next();
-                if (lo == 0) outlier(i + 2 * m);
                 const uint32_t hi = next();
-                if (hi == 0) outlier(i + 2 * m + 1);
+                omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (2 * m);
                 w[m] = lo | (hi << 16);
             }
             *reinterpret_cast<uint4*>(codes + i) = make_uint4(w[0], w[1], w[2], w[3]);
+            if (omask) {
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if ((omask >> k) & 1) outlier(i + k);
+            }
         }
         for (; i < W1; i++) {  // the field's last, partial unit
             const uint32_t sym = next();
