@@ -124,3 +124,14 @@ def test_indexed_1d_reconstructs_in_the_decode(gpu_ctx):
     bad[6 + 2] += 1
     with pytest.raises(VecszError):
         gpu_ctx.decompress(a, "abs", index=bad)
+
+
+def test_indexed_1d_walk_16m(gpu_ctx):
+    """A 1D random walk with spikes of 16M + 3 elements (the 2^28 benchmark
+    field's form at 1/16 of its size), r2r 1e-4: the one-pass indexed decode
+    matches the self-synchronising decode bit for bit."""
+    from vecsz_amd import datagen as dg
+
+    w = dg.walk_spikes_1d((1 << 24) + 3, seed=9, device="cuda")
+    _check(gpu_ctx, w, 1e-4, "r2r")
+    _check(gpu_ctx, w, 1e-4, "r2r", block=64)
