@@ -677,7 +677,6 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     }
     q4 += 2;
     uint32_t lp = (uint32_t)(abit & 31);
-    uint32_t lp_end = lp;  // lp after the segment's ns codewords
     float sprev = 0.f;  // R1D: the previous element's value in the block
     // the wavefront's segments lie kIdxSeg symbols apart; the last field
     // segment may be short, and only its whole pieces go through the stage
@@ -685,9 +684,9 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
 #pragma unroll
         for (uint32_t u = f0; u < f0 + kFlush; u += 8) {
             // Every lane decodes 8 codewords; a lane whose (last, short)
-            // segment has ended decodes on past it harmlessly (bounded reads,
-            // nothing kept): its errors, outliers and end position count only
-            // up to ns.  Outliers wait for the end of the 8.
+            // segment has ended keeps decoding at its end position (nothing
+            // kept): its errors and outliers count only up to ns.  Outliers
+            // wait for the end of the 8.
             uint32_t c[8];
             float o[8];
             uint32_t omask = 0;
@@ -722,10 +721,11 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
                 const uint32_t w = (uint32_t)((cw.buf << (lp - cw.b0)) >> 32);
                 uint32_t len;
                 const uint32_t sym = dec.take_v(cw, w, lp, len);
+                // past ns (or at an invalid codeword, an error) the lane
+                // stays where it is: lp ends at the segment's end
                 const bool in = u + k < ns;
                 bad |= in & (len == 0);
-                lp += len ? len : 1;
-                lp_end = u + k + 1 == ns ? lp : lp_end;
+                lp += in ? len : 0u;
                 c[k] = sym;
                 omask |= (uint32_t)(in & (sym == 0)) << k;
             }
@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     }
     // every segment ends where the next starts, the last at the stream's
     // last codeword bit
-    if (live) bad |= 32 * cw.w0 + lp_end - sv.shift0 != end;
+    if (live) bad |= 32 * cw.w0 + lp - sv.shift0 != end;
     if (bad) atomicOr(&sum->error, 1u);
 }
 
