@@ -841,11 +841,13 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
     const uint64_t E = min(O + s.cnt[j], a.n);
     uint64_t rank = zero_off[j];
     bool bad = false;
+    // the loops below count symbols, so an invalid codeword (len 0, an
+    // error) needs no forced progress
     auto next = [&]() -> uint32_t {
         uint32_t len;
         const uint32_t sym = dec.take_at(st, lp, len);
         bad |= len == 0;
-        lp += len ? len : 1;
+        lp += len;
         return sym;
     };
     auto outlier = [&](uint64_t i) {
