@@ -315,11 +315,13 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
         for (int k = 0; k < kCkpts; k++) ck[t * kCkStride + k] = kNoCkpt;
         uint32_t lp = s0, cnt = 0, z = 0, last_k = 0xFFFFFFFFu;
         while (lp < e0) {
+            // the first codeword start in a window is its checkpoint; the
+            // other codewords write the row's pad slot (never read), so the
+            // record needs no branch
             const uint32_t rel = lp - s0;
-            if (rel / kCkptBits != last_k) {
-                last_k = rel / kCkptBits;
-                ck[t * kCkStride + last_k] = ckpt(rel % kCkptBits, cnt, z);
-            }
+            const uint32_t kk = rel / kCkptBits;
+            ck[t * kCkStride + (kk != last_k ? kk : (uint32_t)kCkpts)] = ckpt(rel % kCkptBits, cnt, z);
+            last_k = kk;
             uint32_t len;
             const uint32_t sym = dec.take_at(st, lp, len);
             bad |= len == 0;
