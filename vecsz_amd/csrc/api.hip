@@ -762,7 +762,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         }
         HIPCHK(c, hipGetLastError());
         DecSummary ds;
-        uint32_t left = 0;
+        uint32_t left = 0, batch = 2;
         for (;;) {
             HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipMemcpyAsync(h + 64, ss.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
@@ -772,8 +772,12 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
             if (left == 0) break;
             if (pass > nsub + 2) return fail(c, VECSZ_ECORRUPT, "subsequence boundaries do not converge");
+            // Long chains (slowly self-synchronising streams) take many
+            // passes: run them in growing batches between host checks (a pass
+            // over an empty list does nothing but clear the list after next).
             StageScope s(c, "decode_redo");
-            launch_decode_pass(da, ss, pass++, dsum, c->stream);
+            for (uint32_t b = 0; b < batch; b++) launch_decode_pass(da, ss, pass++, dsum, c->stream);
+            batch = std::min<uint32_t>(2 * batch, 16);
         }
         {
             StageScope s(c, "decode_scan");
