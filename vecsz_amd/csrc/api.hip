@@ -752,8 +752,9 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             launch_decode_sync(da, ss, dsum, c->stream);
         }
         // Re-decode passes, two without a host check (most streams need one or
-        // none), then as many as the lists ask for: each pass resolves at
-        // least the first subsequence of every chain, so nsub passes suffice.
+        // none), then batches of them while the lists ask for more: each pass
+        // resolves at least the first subsequence of every chain, so nsub
+        // passes suffice.
         constexpr uint32_t kPassesAhead = 2;
         uint32_t pass = 0;
         {
