@@ -100,9 +100,9 @@ int vecsz_decompress(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in,
 
 /* Decode index (no counterpart in the reference; its decoder walks the
  * bitstream serially, src/huffman.hh:343-372): a side product of compress,
- * kept beside the archive, never inside it.  For every 256 symbols it records
+ * kept beside the archive, never inside it.  For every 128 symbols it records
  * the bit where their first codeword starts and the rank of their first
- * outlier, so decompress can decode each 256-symbol segment independently
+ * outlier, so decompress can decode each 128-symbol segment independently
  * instead of synchronising on the bitstream.
  * vecsz_index_words: size of the index for a field, in u64 words.
  * vecsz_compress_indexed: vecsz_compress (the same archive, byte for byte)

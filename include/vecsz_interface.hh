@@ -15,7 +15,9 @@
 //    (datapack::pack layout), increments *num_outlier by the outlier count,
 //    and delete[]s data_in (src/vecsz_interface.hh:222).
 //  - Decompress sets ap->ndim, ap->dim4 and ap->eb from the archive
-//    (:262-268) and returns `new T[len]`.
+//    (:262-268), then ap->len, ap->nblk4 and ap->stride4 as InitializeDims
+//    does (:270, src/dimensions.hh:31-45) and the padding flags from the
+//    archive's pad count (:287-298), and returns `new T[len]`.
 //  - r2r: Compress takes the value range from data_in itself (the reference
 //    re-reads ap->files.input_file, which holds the same floats, :49-50);
 //    Decompress re-reads ap->files.input_file like the reference (:280-284).
@@ -44,6 +46,14 @@
 #include <vector>
 
 #include "vecsz_hip.h"
+
+// The reference's header also brings its analysis helpers into every file
+// that includes it (src/vecsz_interface.hh:13); its main calls
+// analysis::histogram (src/vecsz.cc:112-113).  Keep that when the
+// reference's source tree is on the include path, so main compiles unchanged.
+#if __has_include("utils/analysis.hh")
+#include "utils/analysis.hh"
+#endif
 
 namespace vecsz {
 namespace interface {
@@ -95,6 +105,44 @@ struct has_padding<AP, std::void_t<decltype(std::declval<const AP&>().szwf.globa
                                    decltype(std::declval<const AP&>().szwf.edge_padding),
                                    decltype(std::declval<const AP&>().pad_type),
                                    decltype(std::declval<const AP&>().pad_constant)>> : std::true_type {};
+
+// InitializeDims (src/dimensions.hh:11-64) also fills ap->len, ap->nblk4 and
+// ap->stride4; the reference's main reads ap->len after Decompress
+// (src/vecsz.cc:62).  Set where the argparse type has them.
+template <typename AP, typename = void>
+struct has_len : std::false_type {};
+template <typename AP>
+struct has_len<AP, std::void_t<decltype(std::declval<AP&>().len = size_t(1))>> : std::true_type {};
+template <typename AP, typename = void>
+struct has_nblk4 : std::false_type {};
+template <typename AP>
+struct has_nblk4<AP, std::void_t<decltype(std::declval<AP&>().nblk4._0), decltype(std::declval<AP&>().nblk4._3)>>
+    : std::true_type {};
+template <typename AP, typename = void>
+struct has_stride4 : std::false_type {};
+template <typename AP>
+struct has_stride4<AP, std::void_t<decltype(std::declval<AP&>().stride4._0), decltype(std::declval<AP&>().stride4._3)>>
+    : std::true_type {};
+
+template <typename AP>
+void initialize_dims(AP* ap) {
+    const int d0 = ap->dim4._0, d1 = ap->dim4._1, d2 = ap->dim4._2, d3 = ap->dim4._3;
+    if constexpr (has_nblk4<AP>::value) {
+        // the reference's block count, (d + B + 1) / B (src/dimensions.hh:31)
+        const int B = ap->block_size;
+        ap->nblk4._0 = (d0 + B + 1) / B;
+        ap->nblk4._1 = (d1 + B + 1) / B;
+        ap->nblk4._2 = (d2 + B + 1) / B;
+        ap->nblk4._3 = (d3 + B + 1) / B;
+    }
+    if constexpr (has_len<AP>::value) ap->len = (size_t)d0 * (size_t)d1 * (size_t)d2 * (size_t)d3;
+    if constexpr (has_stride4<AP>::value) {
+        ap->stride4._0 = 1;
+        ap->stride4._1 = d0;
+        ap->stride4._2 = d0 * d1;
+        ap->stride4._3 = d0 * d1 * d2;
+    }
+}
 
 template <typename AP>
 vecsz_params params_of(const AP* ap) {
@@ -178,6 +226,18 @@ void* Decompress(AP* ap, unsigned char* tmp_data, size_t target_size, unsigned l
     ap->dim4._2 = ndim >= 3 ? (int)dims[2] : 1;
     ap->dim4._3 = 1;
     ap->eb = eb;
+    detail::initialize_dims(ap);
+    if constexpr (detail::has_padding<AP>::value) {
+        // padding mode from the pad value count, as the reference decides it
+        // (src/vecsz_interface.hh:287-298): 1 global, one per block, ndim per block
+        uint64_t pad_len = 0;
+        std::memcpy(&pad_len, tmp_data + 4 * (ndim + 1) + 8, 8);
+        uint64_t nblk = 1;
+        for (int i = 0; i < ndim; i++) nblk *= (dims[i] + (uint64_t)ap->block_size + 1) / (uint64_t)ap->block_size;
+        ap->szwf.global_padding = pad_len == 1;
+        ap->szwf.block_padding = pad_len != 1 && pad_len == nblk;
+        ap->szwf.edge_padding = pad_len != 1 && pad_len != nblk && pad_len == (uint64_t)ndim * nblk;
+    }
     vecsz_params p = detail::params_of(ap);
     const size_t len = (size_t)dims[0] * (size_t)dims[1] * (size_t)dims[2];
     if (p.mode == VECSZ_MODE_R2R) p.value_range = detail::value_range_of_file(ap->files.input_file, len);

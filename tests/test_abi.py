@@ -95,3 +95,41 @@ def test_index_words():
     p = make_params((8,), 1e-4, "abs")
     p.ndim = 0
     assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(w)) != 0
+
+
+def _header(ndim, dims, huff=0, nout=0, pad=0):
+    """Archive metadata (datapack layout): [i32 ndim][i32 dims][f64 eb][u64 pad][u64 huff][u64 nout]."""
+    return (np.array([ndim, *dims], dtype=np.int32).tobytes() + np.array([1e-4], dtype=np.float64).tobytes()
+            + np.array([pad, huff, nout], dtype=np.uint64).tobytes())
+
+
+def test_archive_info_rejects_wrapping_element_count():
+    """Extents whose product wraps 64 bits (2^25 each gives 2^75) or exceeds
+    the library's 2^37-element cap are a corrupt header, never a tiny field."""
+    from vecsz_amd import VecszError, archive_info
+
+    for dims in ((1 << 25, 1 << 25, 1 << 25), (0x7FFFFFFF, 0x7FFFFFFF, 4), ((1 << 19), (1 << 19))):
+        h = _header(len(dims), dims) + bytes(64)
+        with pytest.raises(VecszError):
+            archive_info(h)
+    # the largest accepted count parses
+    h = _header(3, (1 << 13, 1 << 12, 1 << 12)) + bytes(64)
+    assert archive_info(h)["dims"] == (1 << 13, 1 << 12, 1 << 12)
+
+
+def test_archive_info_rejects_more_outliers_than_elements():
+    from vecsz_amd import VecszError, archive_info
+
+    h = _header(1, (4,), huff=8, nout=5) + bytes(64)
+    with pytest.raises(VecszError):
+        archive_info(h)
+
+
+def test_sizes_reject_wrapping_element_count():
+    from vecsz_amd import make_params
+
+    lib = _lib.load()
+    b = ctypes.c_uint64()
+    p = make_params((1 << 25, 1 << 25, 1 << 25), 1e-4, "abs")
+    assert lib.vecsz_compress_bound(ctypes.byref(p), ctypes.byref(b)) != 0
+    assert lib.vecsz_index_words(ctypes.byref(p), ctypes.byref(b)) != 0

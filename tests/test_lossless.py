@@ -69,3 +69,11 @@ def test_sz_container(golden):
     assert sz_unpack(d8) == (z, len(z), len(a))
     with pytest.raises(VecszError):
         sz_unpack(f[:20])
+
+
+def test_sz_container_body_size_near_2_64_is_corrupt():
+    """A body size read from the file near 2^64 must not wrap the bounds check."""
+    for ls, ds in ((2 ** 64 - 8, 0), (0, 2 ** 64 - 15), (2 ** 64 - 1, 2 ** 64 - 1)):
+        f = np.array([ls, ds], dtype=np.uint64).tobytes() + bytes(32)
+        with pytest.raises(VecszError):
+            sz_unpack(f)

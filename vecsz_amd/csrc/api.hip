@@ -225,6 +225,26 @@ int vec_threshold(int R) {
 
 bool valid_block(int B) { return B == 8 || B == 16 || B == 32 || B == 64; }
 
+// Largest field the library accepts: 2^37 elements (512 GiB of f32, beyond
+// any one GPU's memory), so that every per-element size it computes (at most
+// 16 bytes per element) stays far below 2^64.
+constexpr uint64_t kMaxElems = 1ull << 37;
+
+// Element count of an ndim field with the given extents (x first), each in
+// [1, 2^31); false when an extent is out of range or the product exceeds
+// kMaxElems (it cannot wrap: every partial product is checked).
+bool field_elems(int ndim, const uint64_t* dims, uint64_t* n) {
+    if (ndim < 1 || ndim > 3) return false;
+    uint64_t m = 1;
+    for (int i = 0; i < ndim; i++) {
+        const uint64_t d = dims[i];
+        if (d == 0 || d > 0x7FFFFFFFull) return false;
+        if (__builtin_mul_overflow(m, d, &m) || m > kMaxElems) return false;
+    }
+    *n = m;
+    return true;
+}
+
 int check_params(vecsz_ctx* c, const vecsz_params* p, bool need_dims) {
     if (!p) return fail(c, VECSZ_EINVAL, "params is NULL");
     if (!valid_block(p->block)) return fail(c, VECSZ_EINVAL, "block must be 8, 16, 32 or 64");
@@ -237,6 +257,8 @@ int check_params(vecsz_ctx* c, const vecsz_params* p, bool need_dims) {
             uint64_t d = i < p->ndim ? p->dims[i] : 1;
             if (d == 0 || d > 0x7FFFFFFFull) return fail(c, VECSZ_EINVAL, "dims must be in [1, 2^31)");
         }
+        uint64_t n;
+        if (!field_elems(p->ndim, p->dims, &n)) return fail(c, VECSZ_EINVAL, "field has more than 2^37 elements");
     }
     int R = p->dict_size / 2;
     if (vec_threshold(R) != R) return fail(c, VECSZ_EINVAL, "dict_size not supported (AVX test differs from |e|<R)");
@@ -545,6 +567,9 @@ int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint
         if (d <= 0) return VECSZ_ECORRUPT;
         dims[i] = (uint64_t)d;
     }
+    // the element count must not wrap (every later size is derived from it)
+    uint64_t n;
+    if (!field_elems(nd, dims, &n)) return VECSZ_ECORRUPT;
     memcpy(eb, h + 4 * (nd + 1), 8);
     uint64_t lens[3];
     memcpy(lens, h + 4 * (nd + 1) + 8, 24);
@@ -553,6 +578,8 @@ int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint
     *n_outlier = lens[2];
     if (lens[0] > total || lens[1] > total || lens[2] > total) return VECSZ_ECORRUPT;
     if (mb + 4 * lens[0] + lens[1] + 4 * lens[2] > total) return VECSZ_ECORRUPT;
+    // at most one outlier per element
+    if (lens[2] > n) return VECSZ_ECORRUPT;
     return VECSZ_OK;
 }
 
@@ -912,8 +939,8 @@ const char* vecsz_ctx_last_error(vecsz_ctx* c) { return c ? c->last_error.c_str(
 
 int vecsz_compress_bound(const vecsz_params* p, uint64_t* bytes) {
     if (!p || !bytes || p->ndim < 1 || p->ndim > 3 || p->dict_size < 2 || p->dict_size > 65536) return VECSZ_EINVAL;
-    uint64_t n = 1;
-    for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
+    uint64_t n;
+    if (!field_elems(p->ndim, p->dims, &n)) return VECSZ_EINVAL;
     const uint64_t nsym = std::min<uint64_t>((uint64_t)p->dict_size, n);
     const uint64_t tree = tree_byte_size(2 * nsym - 1);
     // every symbol at most 64 bits, every element possibly an outlier
@@ -929,9 +956,9 @@ int vecsz_compress(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
 }
 
 int vecsz_index_words(const vecsz_params* p, uint64_t* words) {
-    if (!p || !words || p->ndim < 1 || p->ndim > 3) return VECSZ_EINVAL;
-    uint64_t n = 1;
-    for (int i = 0; i < p->ndim; i++) n *= p->dims[i];
+    if (!p || !words) return VECSZ_EINVAL;
+    uint64_t n;
+    if (!field_elems(p->ndim, p->dims, &n)) return VECSZ_EINVAL;
     *words = idx_layout(n).words;
     return VECSZ_OK;
 }
@@ -1096,9 +1123,11 @@ static bool batch_fusable(const vecsz_ctx* c, int nf, const vecsz_params* p, con
     return p[0].dict_size <= kMaxLdsBook;
 }
 
-static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
-                                uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes,
-                                uint64_t* n_outlier, int* status) {
+// *reported is set once every field's own status is final; a failure before
+// that point (scratch allocation, a HIP error) belongs to the whole batch.
+static int compress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
+                                    uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes,
+                                    uint64_t* n_outlier, int* status, bool* reported) {
     HIPCHK(c, hipSetDevice(c->device));
     timing_begin(c);
     const int B = p[0].block, dict = p[0].dict_size, R = dict / 2;
@@ -1107,8 +1136,8 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
     std::vector<uint64_t> off(nf), nel(nf);
     uint64_t ntot = 0;
     for (int k = 0; k < nf; k++) {
-        uint64_t n = 1;
-        for (int i = 0; i < p[k].ndim; i++) n *= p[k].dims[i];
+        uint64_t n = 0;
+        (void)field_elems(p[k].ndim, p[k].dims, &n);  // batch_fusable checked every field
         nel[k] = n;
         off[k] = ntot;
         ntot += (n + 63) / 64 * 64;
@@ -1401,6 +1430,7 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
         }
     }
     timing_end(c);
+    *reported = true;
     for (int k = 0; k < nf; k++)
         if (status[k]) {
             const char* why = status[k] == VECSZ_ERANGE0   ? "r2r value range is zero"
@@ -1410,6 +1440,20 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
             return fail(c, status[k], "batch field " + std::to_string(k) + " failed: " + why);
         }
     return VECSZ_OK;
+}
+
+// Every field gets a status: on a failure of the whole batch, the fields
+// not already failed on their own take the batch's status.
+static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
+                                uint8_t* const* d_out, const uint64_t* out_cap, uint64_t* out_bytes,
+                                uint64_t* n_outlier, int* status) {
+    for (int k = 0; k < nf; k++) status[k] = VECSZ_OK;
+    bool reported = false;
+    const int rc = compress_batch_fused_run(c, nf, p, d_in, d_out, out_cap, out_bytes, n_outlier, status, &reported);
+    if (rc && !reported)
+        for (int k = 0; k < nf; k++)
+            if (status[k] == VECSZ_OK) status[k] = rc;
+    return rc;
 }
 
 extern "C" {
@@ -1512,9 +1556,8 @@ int vecsz_compress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, const
         return fail(c, VECSZ_EINVAL, "NULL argument");
     std::vector<uint64_t> weight(nfields);
     for (int k = 0; k < nfields; k++) {
-        uint64_t n = 1;
-        for (int i = 0; i < std::max(1, std::min(3, p[k].ndim)); i++) n *= p[k].dims[i];
-        weight[k] = n;
+        uint64_t n = 0;  // a field with invalid extents fails on its own in compress_device
+        weight[k] = field_elems(p[k].ndim, p[k].dims, &n) ? n : 0;
     }
     if (c->fused_batch && batch_fusable(c, nfields, p, d_in, d_out))
         return compress_batch_fused(c, nfields, p, d_in, d_out, out_cap, out_bytes, n_outlier, status);
@@ -1563,9 +1606,12 @@ namespace {
 // Device bytes a request holds while it runs: its input and output buffers
 // plus the scratch its slot's context needs for it.
 uint64_t request_bytes(int kind, const vecsz_params* p, uint64_t in_bytes, uint64_t out_cap) {
-    uint64_t n = 1;
-    for (int i = 0; i < std::max(1, std::min(3, p->ndim)); i++) n *= p->dims[i];
+    // a buffer larger than any device counts as 2^48 bytes (the sums below cannot wrap)
+    in_bytes = std::min<uint64_t>(in_bytes, 1ull << 48);
+    out_cap = std::min<uint64_t>(out_cap, 1ull << 48);
     if (kind == 0) {
+        uint64_t n = 0;
+        if (!field_elems(p->ndim, p->dims, &n)) return ~0ull;  // refused at submit
         const uint64_t nch = (n + kChunk - 1) / kChunk;
         return 4 * n + out_cap + 6 * n + 24 * nch + (uint64_t)768 * p->dict_size * 4 + (1 << 20);
     }
@@ -1608,10 +1654,10 @@ struct vecsz_server {
     static constexpr uint64_t kFoldElems = 4ull << 20;   // 16 MiB of f32 per request
     static constexpr uint64_t kFoldTotal = 32ull << 20;  // elements of one folded batch
     static constexpr int kFoldMax = 64;
+    // (submit has checked the extents: the product does not wrap)
     static uint64_t elems(const vecsz_params& p) {
-        uint64_t n = 1;
-        for (int i = 0; i < std::max(1, std::min(3, p.ndim)); i++) n *= p.dims[i];
-        return n;
+        uint64_t n = 0;
+        return field_elems(p.ndim, p.dims, &n) ? n : 0;
     }
     static bool foldable(const Request& r) {
         return r.kind == 0 && r.p.pad_mode == VECSZ_PAD_NONE && r.p.dict_size <= kMaxLdsBook && elems(r.p) <= kFoldElems;
@@ -1701,8 +1747,12 @@ struct vecsz_server {
             }
         }
         if (batch_fusable(c, nf, ps.data(), ins.data(), outs.data())) {
-            (void)compress_batch_fused(c, nf, ps.data(), ins.data(), outs.data(), caps.data(), sizes.data(),
-                                       nouts.data(), st.data());
+            // every request gets a status: the fused call gives a failure of
+            // the whole launch to each field not already failed on its own
+            const int rc = compress_batch_fused(c, nf, ps.data(), ins.data(), outs.data(), caps.data(), sizes.data(),
+                                                nouts.data(), st.data());
+            if (rc && std::all_of(st.begin(), st.end(), [](int v) { return v == VECSZ_OK; }))
+                std::fill(st.begin(), st.end(), rc);
         } else {  // a request with invalid parameters: each alone, for its own status
             for (int k = 0; k < nf; k++)
                 st[k] = compress_device(c, &ps[k], ins[k], outs[k], caps[k], &sizes[k], &nouts[k]);
@@ -1771,6 +1821,9 @@ static int server_submit(vecsz_server* s, int kind, const vecsz_params* p, const
     r.out = out;
     r.in_bytes = in_bytes;
     r.out_cap = out_cap;
+    uint64_t n = 0;
+    if (kind == 0 && !field_elems(p->ndim, p->dims, &n)) return VECSZ_EINVAL;
+    if (kind == 1 && (out_cap > kMaxElems || in_bytes > 16 * kMaxElems)) return VECSZ_EINVAL;
     r.need = request_bytes(kind, p, in_bytes, out_cap);
     if (r.need > s->budget) return VECSZ_ENOMEM;  // could never be admitted
     r.t_submit = now_ms();

@@ -4,7 +4,7 @@
 //   k_dualquant_*  prequant + blocked Lorenzo postquant + outlier capture +
 //                  quant-code histogram (c_lorenzo_{1,2,3}d1l, src/dualquant.hh:18-913;
 //                  initHuffman's frequency loop, src/huffman.hh:239-242)
-//   k_chunk_bits   bits and outliers per 4096-symbol chunk (HuffmanEncode sizes, src/huffman.hh:268-341)
+//   k_chunk_bits   bits and outliers per 1024-symbol chunk (HuffmanEncode sizes, src/huffman.hh:268-341)
 //   k_enc_summary  archive sizes from the scanned chunk totals (src/vecsz_interface.hh:189-202)
 //   k_pack         MSB-first bit packing into the archive + outlier compaction in index order
 //                  (HuffmanEncode; outlier gather, src/vecsz_interface.hh:166-170)

@@ -174,7 +174,7 @@ int vecsz_sz_pack(const uint8_t* body, uint64_t body_bytes, uint64_t lossless_si
                   uint64_t out_cap, uint64_t* out_bytes) {
     if (!body || !out || !out_bytes) return VECSZ_EINVAL;
     if (body_bytes != (lossless_size ? lossless_size : data_size)) return VECSZ_EINVAL;
-    if (out_cap < 16 + body_bytes) return VECSZ_ENOMEM;
+    if (out_cap < 16 || body_bytes > out_cap - 16) return VECSZ_ENOMEM;
     memcpy(out, &lossless_size, 8);
     memcpy(out + 8, &data_size, 8);
     memcpy(out + 16, body, body_bytes);
@@ -193,7 +193,7 @@ int vecsz_sz_unpack(const uint8_t* file, uint64_t bytes, uint64_t* lossless_size
     // the reference's lossless files carry data_size bytes after the header
     // (D8: written from a lossless_size-byte buffer); the body is the first
     // lossless_size of them
-    if (16 + *body_bytes > bytes) return VECSZ_ECORRUPT;
+    if (*body_bytes > bytes - 16) return VECSZ_ECORRUPT;  // (bytes >= 16: no wrap)
     return VECSZ_OK;
 }
 
