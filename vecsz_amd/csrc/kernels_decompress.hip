@@ -756,8 +756,12 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
                     // x_lorenzo_1d1l (src/dualquant.hh:915-931): a block's
                     // first element predicts from the halo (+0.0 or the
                     // block's pad value), the others from their predecessor
+                    // (only for elements of the field: lanes past its end,
+                    // whose values are dropped, must not index the halo,
+                    // which has one entry per block of the field)
                     const uint64_t x = s0 + u + k;
-                    if ((x % (uint64_t)r.B) == 0) sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
+                    if ((x % (uint64_t)r.B) == 0 && u + k < ns)
+                        sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
                     const float sx = c[k] == 0 ? ov[k] : sprev + (float)((int32_t)c[k] - r.radius);
                     sprev = sx;
                     o[k] = (float)((double)sx * r.twoeb);

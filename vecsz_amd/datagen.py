@@ -136,40 +136,73 @@ def walk_spikes_1d(n: int = 1 << 28, seed: int = 0, sigma: float = 1e-2, spike_f
     return x.to(torch.float32).contiguous()
 
 
+# Largest fields of the batched mix: 2D 4096 x 4096, 3D 256 x 384 x 384.
+_MIX_MAX_2D = 4096 * 4096
+_MIX_MAX_3D = 256 * 384 * 384
+
+
+def _loguniform_mean(lo: float, hi: float) -> float:
+    return (hi - lo) / math.log(hi / lo)
+
+
+def mix_lower_fraction(n_fields: int = 96, target_bytes: float = 8 * 2**30) -> float:
+    """f such that element counts drawn log-uniformly on [f * max, max] (half
+    the fields 2D, half 3D, maxima as above) have an expected total of
+    target_bytes of f32.  The stated ranges (2D from 64^2, 3D from 32^3)
+    average about 16 MiB a field, 1.6 GiB for 96; a mix of ~8 GiB needs the
+    lower ends raised, and this raises both by the same fraction of their
+    maximum."""
+    want = target_bytes / 4.0 / n_fields  # mean elements per field
+    lo, hi = 1e-9, 1.0 - 1e-12
+    for _ in range(200):
+        f = 0.5 * (lo + hi)
+        mean = 0.5 * (_loguniform_mean(f * _MIX_MAX_2D, _MIX_MAX_2D) + _loguniform_mean(f * _MIX_MAX_3D, _MIX_MAX_3D))
+        if mean < want:
+            lo = f
+        else:
+            hi = f
+    return 0.5 * (lo + hi)
+
+
 def batch_mix(n_fields: int = 96, seed: int = 0, target_bytes: float = 8 * 2**30, tol: float = 0.10):
-    """Shapes and error bounds of the batched mix: 2D sizes log-uniform from
-    64^2 to 4096^2 and 3D from 32^3 to 256x384x384, redrawn with successive
-    seeds until the total is within `tol` of `target_bytes`.  Returns a list
-    of dicts {shape (C order), eb, r2r} and the seed that was accepted."""
-    for attempt in range(10000):
+    """Shapes (C order) and r2r error bounds of the ~8 GiB batched mix.
+
+    Each field is 2D or 3D with equal odds; its element count is drawn
+    log-uniformly on [f * max, max] with f = mix_lower_fraction(n_fields,
+    target_bytes) (about 0.66 for 96 fields and 8 GiB) and split into axes
+    with a log-uniform aspect, each axis clamped to the stated per-axis range
+    (2D 64..4096, 3D z 32..256, y and x 32..384); the error bound is drawn from
+    {1e-2, 1e-3, 1e-4}.  Draws are repeated with successive seeds until the
+    total is within `tol` of target_bytes.  Returns ([(shape, eb)], the seed
+    that was accepted)."""
+    f = mix_lower_fraction(n_fields, target_bytes)
+    for attempt in range(1000):
         rng = np.random.default_rng(seed + attempt)
-        fields = []
+        out = []
         total = 0
         for _ in range(n_fields):
-            if rng.random() < 0.5:
-                e = rng.uniform(math.log(64 * 64), math.log(4096 * 4096))
-                side = int(round(math.sqrt(math.exp(e))))
-                ny = max(64, min(4096, int(side * rng.uniform(0.8, 1.25))))
-                nx = max(64, min(4096, int(math.exp(e) / ny)))
-                shape = (ny, nx)
-            else:
-                e = rng.uniform(math.log(32 ** 3), math.log(256 * 384 * 384))
-                n = math.exp(e)
-                nz = max(32, min(256, int(round(n ** (1 / 3) * rng.uniform(0.7, 1.0)))))
-                rest = n / nz
-                ny = max(32, min(384, int(round(math.sqrt(rest) * rng.uniform(0.85, 1.15)))))
-                nx = max(32, min(384, int(round(rest / ny))))
-                shape = (nz, ny, nx)
             eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
-            fields.append({"shape": shape, "eb": eb, "r2r": True})
+            if rng.random() < 0.5:
+                n = np.exp(rng.uniform(np.log(f * _MIX_MAX_2D), np.log(_MIX_MAX_2D)))
+                y = int(np.clip(round(np.sqrt(n) * np.exp(rng.uniform(np.log(0.8), np.log(1.25)))), 64, 4096))
+                x = int(np.clip(round(n / y), 64, 4096))
+                shape = (y, x)
+            else:
+                n = np.exp(rng.uniform(np.log(f * _MIX_MAX_3D), np.log(_MIX_MAX_3D)))
+                z = int(np.clip(round(np.cbrt(n) * np.exp(rng.uniform(np.log(0.7), np.log(1.0)))), 32, 256))
+                y = int(np.clip(round(np.sqrt(n / z) * np.exp(rng.uniform(np.log(0.85), np.log(1.15)))), 32, 384))
+                x = int(np.clip(round(n / (z * y)), 32, 384))
+                shape = (z, y, x)
+            out.append((shape, eb))
             total += 4 * int(np.prod(shape))
         if abs(total - target_bytes) <= tol * target_bytes:
-            return fields, seed + attempt
+            return out, seed + attempt
     raise RuntimeError("no draw within tolerance")
 
 
 def batch_mix_shapes(nfields: int = 96, seed: int = 0):
-    """Shapes (C order) of the batched-form mix: half 2D, half 3D (seeded);
+    """Shapes (C order) of the batched mix at its stated ranges (about
+    1.6 GiB for 96 fields; batch_mix draws the ~8 GiB one): half 2D, half 3D (seeded);
     element count log-uniform over 64x64 .. 4096x4096 (2D) and
     32^3 .. 256x384x384 (3D), split into axes with log-uniform aspect and
     clamped to those per-axis ranges.  Returns [(shape, eb)] with the r2r
