@@ -98,6 +98,21 @@ int vecsz_compress(vecsz_ctx* ctx, const vecsz_params* p, const float* d_in, uin
 int vecsz_decompress(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in, uint64_t in_bytes,
                      float* d_out, uint64_t out_cap);
 
+/* Allocation-free compress: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same archive as
+ * vecsz_compress, byte for byte, with every device scratch buffer inside a
+ * static array of the library (about 28 MiB per GPU, held until the process
+ * ends) and no host allocation during the call (the context reserves its
+ * staging when it is created).  Limits: at most 2^22 elements (a 16 MiB
+ * field), dict_size at most 4096, no padding; larger requests return
+ * VECSZ_EINVAL.  Calls in one process run one at a time (they share their
+ * GPU's array and the host codebook arrays, so overlapping calls would
+ * race; the library serialises them).  The call still stops once on the
+ * host for the codebook, so it cannot itself be captured in a graph. */
+int vecsz_compress_noalloc(vecsz_ctx* ctx, const vecsz_params* p, const float* d_in, uint8_t* d_out,
+                           uint64_t out_cap, uint64_t* out_bytes, uint64_t* n_outlier);
+
 /* Decode index (no counterpart in the reference; its decoder walks the
  * bitstream serially, src/huffman.hh:343-372): a side product of compress,
  * kept beside the archive, never inside it.  For every 128 symbols it records

@@ -30,6 +30,7 @@ EXPORTS = (
     "vecsz_server_submit_decompress", "vecsz_server_wait", "vecsz_lossless_bound", "vecsz_lossless_compress",
     "vecsz_lossless_kind", "vecsz_lossless_decompress", "vecsz_sz_pack", "vecsz_sz_unpack", "vecsz_ctx_last_sidecar",
     "vecsz_params_from_sidecar", "vecsz_verify", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
+    "vecsz_compress_noalloc",
 )
 
 
@@ -129,6 +130,7 @@ def load():
     lib.vecsz_params_default.restype = None
     lib.vecsz_compress_bound.argtypes = [P, u64p]
     lib.vecsz_compress.argtypes = [vp, P, vp, vp, u64, u64p, u64p]
+    lib.vecsz_compress_noalloc.argtypes = [vp, P, vp, vp, u64, u64p, u64p]
     lib.vecsz_decompress.argtypes = [vp, P, vp, u64, vp, u64]
     lib.vecsz_compress_host.argtypes = [vp, P, vp, vp, u64, u64p, u64p]
     lib.vecsz_decompress_host.argtypes = [vp, P, vp, u64, vp, u64]

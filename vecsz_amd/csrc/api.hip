@@ -150,6 +150,12 @@ int fail(vecsz_ctx* c, int code, const std::string& msg) {
     if (c) c->last_error = msg;
     return code;
 }
+// (a literal message is copied without building a std::string: with the
+// capacity a context reserves at creation, reporting it allocates nothing)
+int fail(vecsz_ctx* c, int code, const char* msg) {
+    if (c) c->last_error.assign(msg);
+    return code;
+}
 
 #define HIPCHK(ctx, expr)                                                                       \
     do {                                                                                        \
@@ -317,13 +323,139 @@ struct HostTimer {
     }
 };
 
+// Device scratch of one compress: the context's buffers (grown on demand) or
+// slices of the allocation-free entry point's static array.
+struct CScratch {
+    uint16_t* codes = nullptr;
+    float* sparse = nullptr;
+    unsigned long long* hist = nullptr;
+    uint32_t* partials = nullptr;
+    uint32_t *chunk_bits = nullptr, *chunk_zeros = nullptr;
+    uint64_t *bit_off = nullptr, *out_off = nullptr, *scan_tmp = nullptr, *scan_tot = nullptr;
+    EncSummary* summary = nullptr;
+    uint32_t* err = nullptr;
+    uint32_t* range = nullptr;
+    EbDev* ebdev = nullptr;
+    uint32_t* book = nullptr;
+};
+
+// ---- allocation-free compress (vecsz_compress_noalloc)
+//
+// Its device scratch is the file-scope array `scratch`: one copy per GPU,
+// loaded with the code object and held until the process ends.  Host code
+// never passes the array's name to a kernel (on the host that name is only a
+// stand-in); it passes the device address hipGetSymbolAddress returned for
+// the current device, kept in g_na_ptr at context creation.
+constexpr uint64_t kNaMaxElems = 1ull << 22;  // 4 Mi elements: a 16 MiB field
+constexpr int kNaMaxDict = 4096;
+constexpr int kNaDqGrid = 256;                 // dual-quant workgroups (partial histogram rows)
+constexpr uint64_t kNaChunks = kNaMaxElems / kChunk;
+constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaCodes = 0;
+constexpr size_t kNaSparse = kNaCodes + na_align(kNaMaxElems * 2 + 64);
+constexpr size_t kNaHist = kNaSparse + na_align(kNaMaxElems * 4 + 64);
+constexpr size_t kNaPartials = kNaHist + na_align((size_t)kNaMaxDict * 8);
+constexpr size_t kNaChunkBits = kNaPartials + na_align((size_t)kNaDqGrid * kNaMaxDict * 4);
+constexpr size_t kNaChunkZeros = kNaChunkBits + na_align(kNaChunks * 4);
+constexpr size_t kNaBitOff = kNaChunkZeros + na_align(kNaChunks * 4);
+constexpr size_t kNaOutOff = kNaBitOff + na_align(kNaChunks * 8);
+constexpr size_t kNaScanTmp = kNaOutOff + na_align(kNaChunks * 8);
+constexpr size_t kNaScanTot = kNaScanTmp + na_align(((kNaChunks + 4095) / 4096 + 1) * 16);
+constexpr size_t kNaSummary = kNaScanTot + 256;
+constexpr size_t kNaErr = kNaSummary + na_align(sizeof(EncSummary));
+constexpr size_t kNaRange = kNaErr + 256;
+constexpr size_t kNaEbDev = kNaRange + na_align(8 + 8 * (size_t)kRangeParts);
+constexpr size_t kNaBook = kNaEbDev + na_align(sizeof(EbDev));
+constexpr size_t kNaScratchBytes = kNaBook + na_align((size_t)kNaMaxDict * 9 + 64);  // about 28.3 MiB
+
+}  // namespace
+
+__device__ __align__(256) uint8_t scratch[kNaScratchBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;                  // guards g_na_ptr / g_na_state and g_na_host
+void* g_na_ptr[kNaMaxDevices] = {};     // scratch's device address on each GPU
+int g_na_state[kNaMaxDevices] = {};     // 0 not looked up, 1 ok, -1 lookup failed
+// Host side of an allocation-free call: the codebook and its work arrays,
+// reserved once for kNaMaxDict bins.  Allocation-free calls take g_na_mutex
+// for their whole run, so they never overlap in one process.
+struct NaHost {
+    bool reserved = false;
+    Codebook cb;
+    CodebookWork work;
+} g_na_host;
+
+// Looks up scratch's address on `device` (current) once per device.
+void na_setup(int device) {
+    if (device < 0 || device >= kNaMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (!g_na_host.reserved) {
+        codebook_reserve(g_na_host.cb, g_na_host.work, kNaMaxDict);
+        g_na_host.reserved = true;
+    }
+    if (g_na_state[device]) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        g_na_state[device] = -1;
+        return;
+    }
+    g_na_ptr[device] = ptr;
+    g_na_state[device] = 1;
+}
+
+CScratch na_scratch(void* base) {
+    uint8_t* b = static_cast<uint8_t*>(base);
+    CScratch s;
+    s.codes = reinterpret_cast<uint16_t*>(b + kNaCodes);
+    s.sparse = reinterpret_cast<float*>(b + kNaSparse);
+    s.hist = reinterpret_cast<unsigned long long*>(b + kNaHist);
+    s.partials = reinterpret_cast<uint32_t*>(b + kNaPartials);
+    s.chunk_bits = reinterpret_cast<uint32_t*>(b + kNaChunkBits);
+    s.chunk_zeros = reinterpret_cast<uint32_t*>(b + kNaChunkZeros);
+    s.bit_off = reinterpret_cast<uint64_t*>(b + kNaBitOff);
+    s.out_off = reinterpret_cast<uint64_t*>(b + kNaOutOff);
+    s.scan_tmp = reinterpret_cast<uint64_t*>(b + kNaScanTmp);
+    s.scan_tot = reinterpret_cast<uint64_t*>(b + kNaScanTot);
+    s.summary = reinterpret_cast<EncSummary*>(b + kNaSummary);
+    s.err = reinterpret_cast<uint32_t*>(b + kNaErr);
+    s.range = reinterpret_cast<uint32_t*>(b + kNaRange);
+    s.ebdev = reinterpret_cast<EbDev*>(b + kNaEbDev);
+    s.book = reinterpret_cast<uint32_t*>(b + kNaBook);
+    return s;
+}
+
+// Failure without a heap allocation: the message is copied into the
+// context's error string, whose capacity was reserved at creation.
+int fail_fixed(vecsz_ctx* c, int code, const char* msg, const char* detail = "") {
+    c->last_error.assign(msg);
+    c->last_error.append(detail);
+    return code;
+}
+
+// The compress pipeline.  NA (allocation-free): scratch is `fixed` (slices of
+// the static array), nothing is allocated on the device or the host, errors
+// are reported through fail_fixed, stage timing is off, and padding and the
+// decode index are not available.  Otherwise scratch comes from the context.
 // d_index (nullable): decode index of kIdxHead + 2 * idx_segments(n) words,
 // written beside the archive.
-int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
-                    uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index = nullptr) {
+template <bool NA>
+int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                 uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index, const CScratch* fixed) {
+#define CFAIL(code, msg) return NA ? fail_fixed(c, code, msg) : fail(c, code, msg)
+#define CHIP(expr)                                                                                          \
+    do {                                                                                                    \
+        const hipError_t _e = (expr);                                                                       \
+        if (_e != hipSuccess)                                                                               \
+            return NA ? fail_fixed(c, VECSZ_EHIP, #expr ": ", hipGetErrorString(_e))                        \
+                      : fail(c, VECSZ_EHIP, std::string(#expr) + ": " + hipGetErrorString(_e));             \
+    } while (0)
     int rc = check_params(c, p, true);
     if (rc) return rc;
-    HIPCHK(c, hipSetDevice(c->device));
+    CHIP(hipSetDevice(c->device));
     timing_begin(c);
     const int ndim = p->ndim;
     const uint64_t d0 = p->dims[0], d1 = ndim >= 2 ? p->dims[1] : 1, d2 = ndim >= 3 ? p->dims[2] : 1;
@@ -333,14 +465,34 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const bool device_range = r2r && !(p->value_range > 0);
     const uint64_t nchunks = (n + kChunk - 1) / kChunk;
 
-    if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
-        !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) || !c->range.ensure(8 + 8 * (size_t)kRangeParts) ||
-        !c->ebdev.ensure(sizeof(EbDev)) || !c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
-        !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
-        !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
-        !c->book.ensure((size_t)dict * 9 + 64))
-        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-    if (!c->pin.ensure((size_t)dict * 8 + 4096)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    CScratch S;
+    if (NA) {
+        S = *fixed;
+    } else {
+        if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
+            !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) ||
+            !c->range.ensure(8 + 8 * (size_t)kRangeParts) || !c->ebdev.ensure(sizeof(EbDev)) ||
+            !c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
+            !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
+            !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
+            !c->book.ensure((size_t)dict * 9 + 64))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        if (!c->pin.ensure((size_t)dict * 8 + 4096)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+        S.codes = c->codes.as<uint16_t>();
+        S.sparse = c->sparse.as<float>();
+        S.hist = c->hist.as<unsigned long long>();
+        S.chunk_bits = c->chunk_bits.as<uint32_t>();
+        S.chunk_zeros = c->chunk_zeros.as<uint32_t>();
+        S.bit_off = c->bit_off.as<uint64_t>();
+        S.out_off = c->out_off.as<uint64_t>();
+        S.scan_tmp = c->scan_tmp.as<uint64_t>();
+        S.scan_tot = c->scan_tot.as<uint64_t>();
+        S.summary = c->summary.as<EncSummary>();
+        S.err = c->err.as<uint32_t>();
+        S.range = c->range.as<uint32_t>();
+        S.ebdev = c->ebdev.as<EbDev>();
+        S.book = c->book.as<uint32_t>();
+    }
 
     DQArgs a;
     a.d0 = d0;
@@ -362,9 +514,9 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     if (device_range) {
         // min/max and the eb family stay on the device: no stop before the dual quant
         StageScope s(c, "range");
-        launch_range(d_in, n, c->range.as<uint32_t>(), c->stream);
-        launch_ebfam(c->range.as<uint32_t>(), n, d_in, p->eb, c->ebdev.as<EbDev>(), c->stream);
-        a.ebd = c->ebdev.as<EbDev>();
+        launch_range(d_in, n, S.range, c->stream);
+        launch_ebfam(S.range, n, d_in, p->eb, S.ebdev, c->stream);
+        a.ebd = S.ebdev;
         a.r_f = 0;
         a.r_d = 0;
     } else {
@@ -381,7 +533,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint64_t hb2 = ndim >= 3 ? ref_nblk(d2, B) : 1;
     const int pmode = p->pad_mode;
     const uint64_t pad_len = pad_len_for(ndim, p->dims, B, pmode);
-    if (pmode != VECSZ_PAD_NONE) {
+    if (!NA && pmode != VECSZ_PAD_NONE) {
         const uint64_t nbref = a.hb0 * a.hb1 * hb2;
         if (!c->padv.ensure(pad_len * 4 + 16) || !c->halo.ensure((pmode == VECSZ_PAD_GLOBAL ? 1 : nbref) * 12 + 16) ||
             !c->pad_scratch.ensure(64))
@@ -400,7 +552,7 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         pa.hb0 = a.hb0;
         pa.hb1 = a.hb1;
         pa.hb2 = hb2;
-        pa.ebd = device_range ? c->ebdev.as<EbDev>() : nullptr;
+        pa.ebd = device_range ? S.ebdev : nullptr;
         pa.r_d = device_range ? 0.0 : eb.r_d;
         StageScope s(c, "pad");
         if (pmode == VECSZ_PAD_GLOBAL) {
@@ -415,25 +567,28 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const bool x4 = pmode == VECSZ_PAD_NONE && dualquant_use_x4(ndim, B, d_in, d0, d1, d2, false);
     const uint64_t tiles_x = x4 ? (d0 + 255) / 256 : a.ntiles_x;
     const uint64_t work = ndim == 1 ? ((n + 7) / 8 + 63) / 64 : tiles_x * a.ntiles_y * a.ntiles_z;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, (uint64_t)c->dq_grid));
-    if (!c->partials.ensure((size_t)grid * dict * 4)) return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    const int grid_cap = NA ? std::min(c->dq_grid, kNaDqGrid) : c->dq_grid;
+    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, (uint64_t)grid_cap));
+    if (!NA) {
+        if (!c->partials.ensure((size_t)grid * dict * 4)) return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        S.partials = c->partials.as<uint32_t>();
+    }
 
-    HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)dict * 8, c->stream));
+    launch_zero_u32(reinterpret_cast<uint32_t*>(S.hist), (uint64_t)dict * 2, c->stream);
     {
         StageScope s(c, "dualquant");
-        launch_dualquant(ndim, B, d_in, c->codes.as<uint16_t>(), c->sparse.as<float>(),
-                         c->hist.as<unsigned long long>(), c->partials.as<uint32_t>(), a, grid, c->stream);
+        launch_dualquant(ndim, B, d_in, S.codes, S.sparse, S.hist, S.partials, a, grid, c->stream);
     }
-    HIPCHK(c, hipGetLastError());
+    CHIP(hipGetLastError());
     uint8_t* hp = c->pin.as<uint8_t>();
-    HIPCHK(c, hipMemcpyAsync(hp, c->hist.p, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
+    CHIP(hipMemcpyAsync(hp, S.hist, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
     if (device_range)
-        HIPCHK(c, hipMemcpyAsync(hp + (size_t)dict * 8, c->ebdev.p, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+        CHIP(hipMemcpyAsync(hp + (size_t)dict * 8, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
+    CHIP(hipStreamSynchronize(c->stream));
     if (device_range) {
         EbDev e;
         memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
-        if (e.zero_range) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
+        if (e.zero_range) CFAIL(VECSZ_ERANGE0, "r2r value range is zero");
         c->sc_range = e.range;
         c->sc_eb_final = e.eb_final;
     } else {
@@ -441,13 +596,13 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         c->sc_eb_final = eb.eb_final;
     }
 
-    Codebook& cb = c->cb;
+    Codebook& cb = NA ? g_na_host.cb : c->cb;
     {
         HostTimer ht(c, "host_codebook");
-        rc = build_codebook(reinterpret_cast<const uint64_t*>(hp), dict, cb);
+        rc = build_codebook(reinterpret_cast<const uint64_t*>(hp), dict, cb, NA ? &g_na_host.work : nullptr);
     }
-    if (rc == VECSZ_ECODELEN) return fail(c, rc, "Huffman code longer than 64 bits");
-    if (rc) return fail(c, rc, "codebook construction failed");
+    if (rc == VECSZ_ECODELEN) CFAIL(rc, "Huffman code longer than 64 bits");
+    if (rc) CFAIL(rc, "codebook construction failed");
 
     // Header: datapack metadata (huff_size / n_outlier filled by k_enc_summary),
     // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and
@@ -458,10 +613,11 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     const uint64_t pad_bytes = 4 * pad_len;
     if (head + pad_bytes > out_cap) {
         *out_bytes = head + pad_bytes;
-        return fail(c, VECSZ_ENOMEM, "output buffer too small");
+        CFAIL(VECSZ_ENOMEM, "output buffer too small");
     }
     const size_t book_bytes = (size_t)dict * 9;
-    if (!c->pin.ensure(head + book_bytes + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    if (NA ? c->pin.cap < head + book_bytes + 64 : !c->pin.ensure(head + book_bytes + 64))
+        CFAIL(VECSZ_ENOMEM, "pinned buffer");
     uint8_t* hdr = c->pin.as<uint8_t>();
     {
         memset(hdr, 0, mb);
@@ -482,19 +638,19 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
     }
     if (pad_bytes == 0) {
-        HIPCHK(c, hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
     } else {
         // [metadata][pad values][Huffman header + tree] (src/utils/datapack.hh:17-29)
-        HIPCHK(c, hipMemcpyAsync(d_out, hdr, mb, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_out, hdr, mb, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
     }
-    HIPCHK(c, hipMemcpyAsync(c->book.p, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
-    const uint32_t* cw_hi = c->book.as<uint32_t>();
+    CHIP(hipMemcpyAsync(S.book, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
+    const uint32_t* cw_hi = S.book;
     const uint32_t* cw_lo = cw_hi + dict;
     const uint8_t* cw_len = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
-    HIPCHK(c, hipMemsetAsync(c->err.p, 0, 16, c->stream));
-    if (!c->pin2.ensure(256)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    launch_zero_u32(S.err, 4, c->stream);
+    if (NA ? c->pin2.cap < 256 : !c->pin2.ensure(256)) CFAIL(VECSZ_ENOMEM, "pinned buffer");
     if (d_index) {
         // index header words 0-2 and 5; k_enc_summary writes words 3 and 4
         uint64_t* ih = reinterpret_cast<uint64_t*>(c->pin2.as<uint8_t>() + 192);
@@ -502,41 +658,37 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
         ih[1] = n;
         ih[2] = idx_segments(n);
         ih[5] = (uint64_t)kIdxSeg | ((uint64_t)kIdxGroup << 32);
-        HIPCHK(c, hipMemcpyAsync(d_index, ih, 24, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(d_index + 5, ih + 5, 8, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_index, ih, 24, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(d_index + 5, ih + 5, 8, hipMemcpyHostToDevice, c->stream));
     }
     {
         StageScope s(c, "chunk_bits");
-        launch_chunk_bits(c->codes.as<uint16_t>(), n, cw_len, dict, c->chunk_bits.as<uint32_t>(),
-                          c->chunk_zeros.as<uint32_t>(), nchunks, c->enc_grid, c->stream);
+        launch_chunk_bits(S.codes, n, cw_len, dict, S.chunk_bits, S.chunk_zeros, nchunks, c->enc_grid, c->stream);
     }
     {
         StageScope s(c, "scan");
-        launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), nchunks,
-                          c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
-                          c->scan_tot.as<uint64_t>(), c->stream);
-        launch_enc_summary(c->scan_tot.as<uint64_t>(), c->summary.as<EncSummary>(), d_out, mb, pad_bytes, tb, out_cap,
-                           c->err.as<uint32_t>(), d_index, c->stream);
+        launch_scan_u32x2(S.chunk_bits, S.chunk_zeros, nchunks, S.bit_off, S.out_off, S.scan_tmp, S.scan_tot,
+                          c->stream);
+        launch_enc_summary(S.scan_tot, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index, c->stream);
     }
     {
         StageScope s(c, "pack");
-        launch_pack(c->codes.as<uint16_t>(), n, cw_hi, cw_lo, cw_len, dict, c->bit_off.as<uint64_t>(),
-                    c->out_off.as<uint64_t>(), nchunks, c->summary.as<EncSummary>(), c->sparse.as<float>(), d_out,
-                    c->err.as<uint32_t>(), std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
+        launch_pack(S.codes, n, cw_hi, cw_lo, cw_len, dict, S.bit_off, S.out_off, nchunks, S.summary, S.sparse, d_out,
+                    S.err, std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
     }
-    HIPCHK(c, hipGetLastError());
+    CHIP(hipGetLastError());
     uint8_t* h = c->pin2.as<uint8_t>();
-    HIPCHK(c, hipMemcpyAsync(h, c->summary.p, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(h + 128, c->err.p, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    EncSummary S;
-    memcpy(&S, h, sizeof(S));
+    CHIP(hipMemcpyAsync(h, S.summary, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
+    CHIP(hipMemcpyAsync(h + 128, S.err, 4, hipMemcpyDeviceToHost, c->stream));
+    CHIP(hipStreamSynchronize(c->stream));
+    EncSummary ES;
+    memcpy(&ES, h, sizeof(ES));
     uint32_t e;
     memcpy(&e, h + 128, 4);
     timing_end(c);
-    *out_bytes = S.archive_size;
-    if (n_outlier) *n_outlier = S.n_outlier;
-    if (e) return fail(c, VECSZ_ENOMEM, "output buffer too small");
+    *out_bytes = ES.archive_size;
+    if (n_outlier) *n_outlier = ES.n_outlier;
+    if (e) CFAIL(VECSZ_ENOMEM, "output buffer too small");
     vecsz_sidecar& sc = c->sidecar;
     sc.magic = VECSZ_SIDECAR_MAGIC;
     sc.version = 1;
@@ -549,6 +701,13 @@ int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint
     sc.eb_final = c->sc_eb_final;
     c->have_sidecar = true;
     return VECSZ_OK;
+#undef CHIP
+#undef CFAIL
+}
+
+int compress_device(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                    uint64_t* out_bytes, uint64_t* n_outlier, uint64_t* d_index = nullptr) {
+    return compress_run<false>(c, p, d_in, d_out, out_cap, out_bytes, n_outlier, d_index, nullptr);
 }
 
 int parse_meta(const uint8_t* h, uint64_t avail, uint64_t total, int* ndim, uint64_t dims[3], double* eb,
@@ -919,6 +1078,15 @@ int vecsz_ctx_create(vecsz_ctx** out, int device, void* stream) {
         }
         c->own_stream = true;
     }
+    // set-up for vecsz_compress_noalloc: error text capacity, the pinned
+    // staging it uses, and (once per device, this device current) the
+    // address of the static scratch array
+    c->last_error.reserve(512);
+    if (!c->pin.ensure(1 << 20) || !c->pin2.ensure(256)) {
+        vecsz_ctx_destroy(c);
+        return VECSZ_ENOMEM;
+    }
+    na_setup(device);
     *out = c;
     return VECSZ_OK;
 }
@@ -953,6 +1121,30 @@ int vecsz_compress(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
                    uint64_t* out_bytes, uint64_t* n_outlier) {
     if (!c || !d_in || !d_out || !out_bytes) return fail(c, VECSZ_EINVAL, "NULL argument");
     return compress_device(c, p, d_in, d_out, out_cap, out_bytes, n_outlier);
+}
+
+int vecsz_compress_noalloc(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t* d_out, uint64_t out_cap,
+                           uint64_t* out_bytes, uint64_t* n_outlier) {
+    if (!c) return VECSZ_EINVAL;
+    if (!p || !d_in || !d_out || !out_bytes) return fail(c, VECSZ_EINVAL, "NULL argument");
+    int rc = check_params(c, p, true);
+    if (rc) return rc;
+    uint64_t n = 0;
+    (void)field_elems(p->ndim, p->dims, &n);
+    if (n > kNaMaxElems) return fail(c, VECSZ_EINVAL, "allocation-free compress: field above 2^22 elements");
+    if (p->dict_size > kNaMaxDict) return fail(c, VECSZ_EINVAL, "allocation-free compress: dict_size above 4096");
+    if (p->pad_mode != VECSZ_PAD_NONE) return fail(c, VECSZ_EINVAL, "allocation-free compress: no padding");
+    // one allocation-free call at a time in the process: they share the
+    // scratch array of their GPU and the host codebook arrays
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (c->device < 0 || c->device >= kNaMaxDevices || g_na_state[c->device] != 1)
+        return fail(c, VECSZ_EHIP, "allocation-free compress: no scratch address on this device");
+    const CScratch S = na_scratch(g_na_ptr[c->device]);
+    const bool was_timing = c->timing;
+    c->timing = false;  // stage timing would create events
+    rc = compress_run<true>(c, p, d_in, d_out, out_cap, out_bytes, n_outlier, nullptr, &S);
+    c->timing = was_timing;
+    return rc;
 }
 
 int vecsz_index_words(const vecsz_params* p, uint64_t* words) {

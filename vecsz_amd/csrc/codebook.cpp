@@ -34,10 +34,7 @@ struct PackedQueue {
     static int32_t node(E e) { return (int32_t)(e & ((1u << kNodeBits) - 1)); }
 };
 struct PairQueue {
-    struct E {
-        uint64_t f;
-        int32_t n;
-    };
+    using E = CodebookWork::Pair;
     static E make(uint64_t f, int32_t n) { return E{f, n}; }
     static uint64_t f(const E& e) { return e.f; }
     static int32_t node(const E& e) { return e.n; }
@@ -45,28 +42,22 @@ struct PairQueue {
 
 // Per-thread work arrays, kept between calls so a codebook build allocates
 // nothing once the first field of a given symbol count has been seen.
-struct Work {
-    std::vector<int32_t> sym, kid, len, size, idx;
-    std::vector<uint64_t> bits;
-    std::vector<PackedQueue::E> heap_packed;
-    std::vector<PairQueue::E> heap_pair;
-};
-thread_local Work t_work;
+thread_local CodebookWork t_work_default;
 
 template <typename Q>
-std::vector<typename Q::E>& heap_of();
+std::vector<typename Q::E>& heap_of(CodebookWork& w);
 template <>
-std::vector<PackedQueue::E>& heap_of<PackedQueue>() { return t_work.heap_packed; }
+std::vector<PackedQueue::E>& heap_of<PackedQueue>(CodebookWork& w) { return w.heap_packed; }
 template <>
-std::vector<PairQueue::E>& heap_of<PairQueue>() { return t_work.heap_pair; }
+std::vector<PairQueue::E>& heap_of<PairQueue>(CodebookWork& w) { return w.heap_pair; }
 
 // The reference's queue (src/huffman.hh:146-192) and merge loop (:244-262):
 // leaves enter in ascending symbol order; each merge removes b, then a, and
 // inserts node(left = a, right = b) with freq a + b.
 template <typename Q>
-void merge_all(const uint64_t* freq, int dict, int nsym, int32_t* sym, int32_t* kid) {
+void merge_all(const uint64_t* freq, int dict, int nsym, int32_t* sym, int32_t* kid, CodebookWork& w) {
     using E = typename Q::E;
-    std::vector<E>& qq = heap_of<Q>();
+    std::vector<E>& qq = heap_of<Q>(w);
     qq.resize(nsym + 2);
     int qend = 1;
     auto qinsert = [&](E e) {
@@ -127,7 +118,24 @@ uint64_t tree_byte_size(uint64_t nc) {
     return 1 + nc * (2 * w + 4 + 1);
 }
 
-int build_codebook(const uint64_t* freq, int dict, Codebook& cb) {
+void codebook_reserve(Codebook& cb, CodebookWork& w, int dict) {
+    const size_t nc = 2 * (size_t)dict;
+    cb.cw_hi.reserve(dict);
+    cb.cw_lo.reserve(dict);
+    cb.len.reserve(dict);
+    cb.tree.reserve(tree_byte_size(nc));
+    w.sym.reserve(dict);
+    w.kid.reserve(nc);
+    w.len.reserve(nc);
+    w.size.reserve(nc);
+    w.idx.reserve(nc);
+    w.bits.reserve(nc);
+    w.heap_packed.reserve(dict + 2);
+    w.heap_pair.reserve(dict + 2);
+}
+
+int build_codebook(const uint64_t* freq, int dict, Codebook& cb, CodebookWork* work) {
+    CodebookWork& t_work = work ? *work : t_work_default;
     cb.dict = dict;
     cb.cw_hi.assign(dict, 0);
     cb.cw_lo.assign(dict, 0);
@@ -148,9 +156,9 @@ int build_codebook(const uint64_t* freq, int dict, Codebook& cb) {
     uint64_t total = 0;
     for (int s2 = 0; s2 < dict; s2++) total += freq[s2];
     if (total < (1ull << (64 - kNodeBits)))
-        merge_all<PackedQueue>(freq, dict, nsym, sym.data(), kid.data());
+        merge_all<PackedQueue>(freq, dict, nsym, sym.data(), kid.data(), t_work);
     else
-        merge_all<PairQueue>(freq, dict, nsym, sym.data(), kid.data());
+        merge_all<PairQueue>(freq, dict, nsym, sym.data(), kid.data(), t_work);
 
     // Codewords top-down in descending node order (parents before children).
     std::vector<uint64_t>& bits = t_work.bits;

@@ -16,9 +16,27 @@ struct Codebook {
     std::vector<uint8_t> tree;           // serialized tree (src/huffman.hh:425-502)
 };
 
+// Work arrays of a codebook build.  build_codebook keeps one set per host
+// thread; a caller may pass its own, reserved beforehand with
+// codebook_reserve, so that the build allocates nothing.
+struct CodebookWork {
+    struct Pair {
+        uint64_t f;
+        int32_t n;
+    };
+    std::vector<int32_t> sym, kid, len, size, idx;
+    std::vector<uint64_t> bits;
+    std::vector<uint64_t> heap_packed;
+    std::vector<Pair> heap_pair;
+};
+
+// Capacity for any histogram of at most `dict` bins in cb and w.
+void codebook_reserve(Codebook& cb, CodebookWork& w, int dict);
+
 // Builds the reference's tree from a symbol histogram of `dict` bins.
-// Returns a VECSZ_* status.
-int build_codebook(const uint64_t* freq, int dict, Codebook& cb);
+// Returns a VECSZ_* status.  With cb and *work reserved for dict bins
+// (codebook_reserve), it allocates nothing.
+int build_codebook(const uint64_t* freq, int dict, Codebook& cb, CodebookWork* work = nullptr);
 
 struct DecodeTables {
     int node_count = 0;

@@ -10,6 +10,8 @@ constexpr int kMaxLdsLenTab = 16384;    // LDS code-length table up to this dict
 constexpr int kMaxLdsBook = 4096;       // LDS codebook (k_pack) up to this dict size
 
 int dq_lds_bytes(int dict);
+// Zeroes n 32-bit words with a kernel.
+void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st);
 // out: 2 + 2 * kRangeParts words (min/max, then one pair per workgroup).
 constexpr uint32_t kRangeParts = 1024;
 uint32_t range_parts(uint64_t n);

@@ -1600,6 +1600,18 @@ int dq_lds_bytes(int dict) {
     return (kHotBins * 64 + (dict <= kMaxLdsHistBins ? dict : 0)) * 4;
 }
 
+// Clears n words with stores from a kernel (not hipMemsetAsync, so that a
+// caller's graph capture records only kernel nodes).
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t st) {
+    const unsigned g = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_zero_u32, dim3(g), dim3(256), 0, st, p, n);
+}
+
 uint32_t range_parts(uint64_t n) {
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>((n / 4 + 255) / 256, 1), kRangeParts);
 }
