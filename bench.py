@@ -535,6 +535,9 @@ def main():
     # a second run of the same K compress steps
     ctx.enable_timing(True)
     cst = stage_avg(ctx, comp, args.steps)
+    ctx.set_fused_encode(True)  # the one-pass encode, for comparison (same archive)
+    cst1 = stage_avg(ctx, comp, args.steps)
+    ctx.set_fused_encode(False)
     dst = stage_avg(ctx, decomp, args.steps)
     dsti = stage_avg(ctx, decomp_idx, args.steps)
     ctx.enable_timing(False)
@@ -546,7 +549,8 @@ def main():
     # element); the packer writes the archive (A bytes).  The u16 code array
     # they pass along is scratch, counted in `traffic`, not in `achieved`.
     kern = {"range": ("k_range", 4 * n), "dualquant": (dq_kernel_name(len(dims), args.block), 4 * n),
-            "chunk_bits": ("k_chunk_bits<true>", 0), "scan": ("k_scan_local", 0), "pack": ("k_pack", size)}
+            "chunk_bits": ("k_chunk_bits<true>", 0), "scan": ("k_scan_local", 0), "pack": ("k_pack", size),
+            "encode": ("k_encode", size)}
     dom = max((k for k in cst if k in kern), key=lambda k: cst[k])
     kname, abytes = kern[dom]
     achieved = abytes / (cst[dom] * 1e-3) / 1e9
@@ -593,6 +597,7 @@ def main():
         "error_bound_ok": bound_ok,
         "latency_ms": {"p50": round(lat[len(lat) // 2] * 1e3, 4), "p99": round(lat[-1] * 1e3, 4)},
         "stages_ms": {"compress": {k: round(v, 4) for k, v in cst.items()},
+                      "compress_one_pass_encode": {k: round(v, 4) for k, v in cst1.items()},
                       "decompress": {k: round(v, 4) for k, v in dst.items()}},
         "roofline": roofline,
         "launch_shape": tune,

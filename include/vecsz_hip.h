@@ -170,6 +170,14 @@ int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
  * fused path when the batch qualifies.  Both write identical archives. */
 int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
 
+/* 1: a single-field compress encodes in one pass (bit counts, prefix by a
+ * decoupled look-back over partitions, and packing in one kernel; the codes
+ * are read from HBM once) when dict_size is at most 4096; 0 (default): chunk
+ * bit counts, a scan of them and the packing as three kernels, which is
+ * faster on an MI355X (0.221 against 0.243 ms at 512^3).  Both write
+ * identical archives. */
+int vecsz_ctx_set_fused_encode(vecsz_ctx* ctx, int on);
+
 /* GPU launch-shape autotuner; replaces autotune_vector_len /
  * autotune_block_sizes (src/autotune.hh:80-127, called from
  * src/vecsz_interface.hh:60-68).  Compresses the field repeatedly (d_out is

@@ -253,7 +253,7 @@ def test_stage_timing(gpu_ctx):
     gpu_ctx.enable_timing(True)
     a, _ = gpu_ctx.compress(x, 1e-4, "r2r")
     names = [n for n, _ in gpu_ctx.timings()]
-    assert "dualquant" in names and "pack" in names
+    assert "dualquant" in names and "pack" in names  # the three-kernel encode (the default)
     gpu_ctx.decompress(a, "r2r", float(x.max() - x.min()))
     assert "reconstruct" in [n for n, _ in gpu_ctx.timings()]
     gpu_ctx.enable_timing(False)

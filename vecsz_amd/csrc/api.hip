@@ -18,6 +18,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <deque>
+#include <initializer_list>
 #include <mutex>
 #include <unordered_map>
 #include <cstdio>
@@ -106,11 +107,13 @@ struct vecsz_ctx {
     std::vector<vecsz_ctx*> workers;
     int nworkers = 8;
     bool fused_batch = true;  // compress batches as one launch per stage when they qualify
+    bool fused_encode = false; // single-field encode as one pass (k_encode): measured slower, off by default
     // launch shapes (workgroup caps of the grid-stride compress kernels), set
     // by vecsz_autotune; the output does not depend on them
     int dq_grid = 768, enc_grid = 2048;
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
+    DevBuf enc_status, outl;  // fused encode: partition status words + counter, compact outliers
     DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
     // fused batch compress: field descriptors, unit starts, per-field histograms,
     // range words, eb families, summaries, status words, codebooks, headers
@@ -337,6 +340,9 @@ struct CScratch {
     uint32_t* range = nullptr;
     EbDev* ebdev = nullptr;
     uint32_t* book = nullptr;
+    uint64_t* enc_status = nullptr;  // fused encode (null: that stage takes the three-kernel path)
+    uint32_t* outl = nullptr;
+    uint64_t outl_cap = 0;           // outliers outl holds
 };
 
 // ---- allocation-free compress (vecsz_compress_noalloc)
@@ -366,7 +372,12 @@ constexpr size_t kNaErr = kNaSummary + na_align(sizeof(EncSummary));
 constexpr size_t kNaRange = kNaErr + 256;
 constexpr size_t kNaEbDev = kNaRange + na_align(8 + 8 * (size_t)kRangeParts);
 constexpr size_t kNaBook = kNaEbDev + na_align(sizeof(EbDev));
-constexpr size_t kNaScratchBytes = kNaBook + na_align((size_t)kNaMaxDict * 9 + 64);  // about 28.3 MiB
+constexpr uint64_t kNaParts = (kNaChunks + kEncPartChunks - 1) / kEncPartChunks;  // fused-encode partitions
+constexpr uint64_t kNaOutl = 1ull << 20;              // outliers of the fused encode; more: three-kernel path
+constexpr size_t kNaEncStatus = kNaBook + na_align((size_t)kNaMaxDict * 9 + 64);
+constexpr size_t kNaOutlOff = kNaEncStatus + na_align((2 * kNaParts + 1) * 8);
+constexpr size_t kNaScratchBytes = kNaOutlOff + na_align(kNaOutl * 4);  // about 32.3 MiB
+static_assert(kNaParts >= kNaChunks / kEncPartChunks, "status words for every partition");
 
 }  // namespace
 
@@ -425,6 +436,9 @@ CScratch na_scratch(void* base) {
     s.range = reinterpret_cast<uint32_t*>(b + kNaRange);
     s.ebdev = reinterpret_cast<EbDev*>(b + kNaEbDev);
     s.book = reinterpret_cast<uint32_t*>(b + kNaBook);
+    s.enc_status = reinterpret_cast<uint64_t*>(b + kNaEncStatus);
+    s.outl = reinterpret_cast<uint32_t*>(b + kNaOutlOff);
+    s.outl_cap = kNaOutl;
     return s;
 }
 
@@ -472,21 +486,12 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
         if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
             !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) ||
             !c->range.ensure(8 + 8 * (size_t)kRangeParts) || !c->ebdev.ensure(sizeof(EbDev)) ||
-            !c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
-            !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
-            !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
             !c->book.ensure((size_t)dict * 9 + 64))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
         if (!c->pin.ensure((size_t)dict * 8 + 4096)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
         S.codes = c->codes.as<uint16_t>();
         S.sparse = c->sparse.as<float>();
         S.hist = c->hist.as<unsigned long long>();
-        S.chunk_bits = c->chunk_bits.as<uint32_t>();
-        S.chunk_zeros = c->chunk_zeros.as<uint32_t>();
-        S.bit_off = c->bit_off.as<uint64_t>();
-        S.out_off = c->out_off.as<uint64_t>();
-        S.scan_tmp = c->scan_tmp.as<uint64_t>();
-        S.scan_tot = c->scan_tot.as<uint64_t>();
         S.summary = c->summary.as<EncSummary>();
         S.err = c->err.as<uint32_t>();
         S.range = c->range.as<uint32_t>();
@@ -585,6 +590,7 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     if (device_range)
         CHIP(hipMemcpyAsync(hp + (size_t)dict * 8, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
     CHIP(hipStreamSynchronize(c->stream));
+    const uint64_t hist0 = reinterpret_cast<const uint64_t*>(hp)[0];  // code 0: the outliers
     if (device_range) {
         EbDev e;
         memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
@@ -661,20 +667,53 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
         CHIP(hipMemcpyAsync(d_index, ih, 24, hipMemcpyHostToDevice, c->stream));
         CHIP(hipMemcpyAsync(d_index + 5, ih + 5, 8, hipMemcpyHostToDevice, c->stream));
     }
-    {
-        StageScope s(c, "chunk_bits");
-        launch_chunk_bits(S.codes, n, cw_len, dict, S.chunk_bits, S.chunk_zeros, nchunks, c->enc_grid, c->stream);
+    // Encode: one pass (k_encode + k_enc_finish) when the codebook fits in LDS,
+    // else chunk bits, a scan of them, the sizes and the packing.
+    const uint64_t nparts = encode_parts(nchunks);
+    bool fused = c->fused_encode && encode_fused_ok(dict);
+    if (fused && !NA) {
+        if (!c->enc_status.ensure((2 * nparts + 1) * 8) || !c->outl.ensure(hist0 * 4 + 64))
+            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        S.enc_status = c->enc_status.as<uint64_t>();
+        S.outl = c->outl.as<uint32_t>();
+        S.outl_cap = hist0;
     }
-    {
-        StageScope s(c, "scan");
-        launch_scan_u32x2(S.chunk_bits, S.chunk_zeros, nchunks, S.bit_off, S.out_off, S.scan_tmp, S.scan_tot,
+    fused = fused && hist0 <= S.outl_cap;
+    if (fused) {
+        StageScope s(c, "encode");
+        launch_zero_u32(reinterpret_cast<uint32_t*>(S.enc_status), (2 * nparts + 1) * 2, c->stream);
+        launch_encode(S.codes, n, cw_hi, cw_lo, cw_len, dict, cb.max_len, S.sparse, d_out, (head + pad_bytes) * 8,
+                      out_cap, cb.max_len == 0, S.outl, S.enc_status, S.err, d_index, c->enc_grid, c->stream);
+        launch_enc_finish(S.enc_status, n, S.outl, hist0, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index,
                           c->stream);
-        launch_enc_summary(S.scan_tot, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index, c->stream);
-    }
-    {
-        StageScope s(c, "pack");
-        launch_pack(S.codes, n, cw_hi, cw_lo, cw_len, dict, S.bit_off, S.out_off, nchunks, S.summary, S.sparse, d_out,
-                    S.err, std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
+    } else {
+        if (!NA) {
+            if (!c->chunk_bits.ensure(nchunks * 4) || !c->chunk_zeros.ensure(nchunks * 4) ||
+                !c->bit_off.ensure(nchunks * 8) || !c->out_off.ensure(nchunks * 8) ||
+                !c->scan_tmp.ensure(((nchunks + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16))
+                return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+            S.chunk_bits = c->chunk_bits.as<uint32_t>();
+            S.chunk_zeros = c->chunk_zeros.as<uint32_t>();
+            S.bit_off = c->bit_off.as<uint64_t>();
+            S.out_off = c->out_off.as<uint64_t>();
+            S.scan_tmp = c->scan_tmp.as<uint64_t>();
+            S.scan_tot = c->scan_tot.as<uint64_t>();
+        }
+        {
+            StageScope s(c, "chunk_bits");
+            launch_chunk_bits(S.codes, n, cw_len, dict, S.chunk_bits, S.chunk_zeros, nchunks, c->enc_grid, c->stream);
+        }
+        {
+            StageScope s(c, "scan");
+            launch_scan_u32x2(S.chunk_bits, S.chunk_zeros, nchunks, S.bit_off, S.out_off, S.scan_tmp, S.scan_tot,
+                              c->stream);
+            launch_enc_summary(S.scan_tot, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index, c->stream);
+        }
+        {
+            StageScope s(c, "pack");
+            launch_pack(S.codes, n, cw_hi, cw_lo, cw_len, dict, S.bit_off, S.out_off, nchunks, S.summary, S.sparse,
+                        d_out, S.err, std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
+        }
     }
     CHIP(hipGetLastError());
     uint8_t* h = c->pin2.as<uint8_t>();
@@ -1656,6 +1695,12 @@ int vecsz_ctx_set_workers(vecsz_ctx* c, int nworkers) {
     return VECSZ_OK;
 }
 
+int vecsz_ctx_set_fused_encode(vecsz_ctx* c, int on) {
+    if (!c) return VECSZ_EINVAL;
+    c->fused_encode = on != 0;
+    return VECSZ_OK;
+}
+
 int vecsz_ctx_set_batch_fused(vecsz_ctx* c, int on) {
     if (!c) return VECSZ_EINVAL;
     c->fused_batch = on != 0;
@@ -1682,14 +1727,15 @@ int vecsz_autotune(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
     c->timing = true;
     uint64_t bytes = 0, nout = 0;
     // best (smallest) time of two runs of the named stages
-    auto stage_ms = [&](const vecsz_params* q, const char* a, const char* b, double* ms) -> int {
+    auto stage_ms = [&](const vecsz_params* q, std::initializer_list<const char*> names, double* ms) -> int {
         *ms = 1e30;
         for (int rep = 0; rep < 2; rep++) {
             const int r = compress_device(c, q, d_in, d_out, out_cap, &bytes, &nout);
             if (r) return r;
             double sum = 0;
             for (size_t i = 0; i < c->tnames.size(); i++)
-                if (c->tnames[i] == a || (b && c->tnames[i] == b)) sum += c->tms[i];
+                for (const char* nm : names)
+                    if (c->tnames[i] == nm) sum += c->tms[i];
             *ms = std::min(*ms, sum);
         }
         return VECSZ_OK;
@@ -1697,12 +1743,12 @@ int vecsz_autotune(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
     do {
         if ((rc = compress_device(c, p, d_in, d_out, out_cap, &bytes, &nout))) break;  // warm-up
         static const int kDqGrid[] = {256, 512, 768, 1024, 1536, 2048, 4096};
-        static const int kEncGrid[] = {512, 1024, 2048, 4096, 8192};
+        static const int kEncGrid[] = {256, 512, 768, 1024, 2048, 4096, 8192};
         double best = 1e30, ms = 0;
         int pick = c->dq_grid;
         for (int g : kDqGrid) {
             c->dq_grid = g;
-            if ((rc = stage_ms(p, "dualquant", nullptr, &ms))) break;
+            if ((rc = stage_ms(p, {"dualquant"}, &ms))) break;
             if (ms < best) best = ms, pick = g;
         }
         if (rc) break;
@@ -1713,7 +1759,8 @@ int vecsz_autotune(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8
         pick = c->enc_grid;
         for (int g : kEncGrid) {
             c->enc_grid = g;
-            if ((rc = stage_ms(p, "chunk_bits", "pack", &ms))) break;
+            // the one-pass encode, or chunk bits and packing
+            if ((rc = stage_ms(p, {"encode", "chunk_bits", "pack"}, &ms))) break;
             if (ms < best) best = ms, pick = g;
         }
         if (rc) break;

@@ -38,6 +38,23 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap,
                  uint64_t* index, hipStream_t st);
 
+// Fused single-pass encode (bits, prefix by decoupled look-back, packing) for
+// dictionaries whose codebook fits in LDS (encode_fused_ok).  status: 2 *
+// encode_parts(nchunks) + 1 words, zeroed before the launch; outl: 4 bytes per
+// outlier (the histogram's bin 0); the header must be in place.  Then
+// launch_enc_finish writes the sizes and the outlier section.
+constexpr int kEncWaveChunks = 8;                   // chunks per wavefront
+constexpr int kEncPartChunks = 4 * kEncWaveChunks;  // chunks per partition (one workgroup)
+uint64_t encode_parts(uint64_t nchunks);
+bool encode_fused_ok(int dict);
+void launch_encode(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo,
+                   const uint8_t* len_tab, int dict, int max_len, const float* sparse, uint8_t* archive,
+                   uint64_t stream_bit_off, uint64_t out_cap, bool no_stream, uint32_t* outl, uint64_t* status,
+                   uint32_t* err, uint64_t* index, int grid_cap, hipStream_t st);
+void launch_enc_finish(const uint64_t* status, uint64_t n, const uint32_t* outl, uint64_t n_outlier_max,
+                       EncSummary* sum, uint8_t* archive, uint64_t meta_bytes, uint64_t pad_bytes, uint64_t tree_bytes,
+                       uint64_t out_cap, uint32_t* err, uint64_t* index, hipStream_t st);
+
 // Batched compress (kernels_compress.hip): every field of a batch per launch.
 // start arrays hold nf + 1 unit offsets (see BatchField).
 uint64_t range_units(uint64_t n);

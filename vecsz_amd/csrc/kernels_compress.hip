@@ -901,6 +901,14 @@ __device__ __forceinline__ void enc_summary(uint64_t tb, uint64_t tz, EncSummary
     if (index) {  // decode-index header: stream bits and outliers
         index[3] = tb;
         index[4] = tz;
+        // the per-segment arrays' padding slots (past the last segment, up to
+        // a whole word), so that an index is the same byte for byte on every
+        // run; the packer writes only the slots of existing segments
+        const IdxLayout L = idx_layout(index[1]);
+        uint32_t* rb = reinterpret_cast<uint32_t*>(index + L.rel_bits);
+        for (uint64_t g = L.nseg; g < (L.nseg + 1) / 2 * 2; g++) rb[g] = 0;
+        uint16_t* ro = reinterpret_cast<uint16_t*>(index + L.rel_outs);
+        for (uint64_t g = L.nseg; g < (L.nseg + 3) / 4 * 4; g++) ro[g] = 0;
     }
     const uint64_t enc = (tb + 7) / 8;
     const uint64_t huff = 8 + tree_bytes + enc;
@@ -1019,18 +1027,47 @@ __host__ __device__ constexpr uint32_t book_words(int book, int dict) {
     return book == kBookPacked ? (uint32_t)dict + 1 : book == kBookSplit ? (uint32_t)dict + 1 + ((uint32_t)dict + 4) / 4 : 0;
 }
 
+// Chunks of one decode-index group (its segments' bit offsets are stored
+// relative to the group's first chunk).
+constexpr uint64_t kChunksPerGroup = kIdxGroup * kIdxSeg / kChunk;
+static_assert(kChunksPerGroup >= 1 && kIdxGroup * kIdxSeg % kChunk == 0, "an index group spans whole chunks");
+
+// Where pack_chunk writes: outlier r goes to outl + 4 r (little-endian
+// bytes); the bitstream starts at archive bit stream_bit_off; bytes at or past
+// stream_end_byte are not the stream's (~0: the end of the field's last chunk,
+// known only to that chunk); words that end past cap are not written (err is
+// set instead); no_stream: a one-symbol tree, no bitstream.
+struct PackTarget {
+    uint8_t* outl;
+    uint64_t stream_bit_off;
+    uint64_t stream_end_byte;
+    uint64_t cap;
+    uint32_t* err;
+    bool no_stream;
+};
+
+__device__ __forceinline__ PackTarget pack_target(const EncSummary& S, uint8_t* archive) {
+    PackTarget T;
+    T.outl = archive + S.outlier_byte_off;
+    T.stream_bit_off = S.stream_bit_off;
+    T.stream_end_byte = S.stream_end_byte;
+    T.cap = ~0ull;
+    T.err = nullptr;
+    T.no_stream = S.total_bits == 0;
+    return T;
+}
+
 // Packs chunk k (codes already in `cur`) into the archive; see k_pack.
-// index: the decode index (layout L), or null; bit_off / out_off: the
-// field's chunk offsets (read for the index's group bases).
+// index: the decode index (layout L), or null; gbits / gouts: the first bit
+// and outlier rank of the chunk's index group (its first chunk's offsets).
 template <int BOOK>
 __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint64_t nchunks, uint32_t lane,
                                            const uint32_t* tab, const uint8_t* tlen,
                                            const uint32_t* __restrict__ cw_hi, const uint32_t* __restrict__ cw_lo,
-                                           const uint8_t* __restrict__ len_tab, const EncSummary& S,
+                                           const uint8_t* __restrict__ len_tab, const PackTarget& T,
                                            const float* __restrict__ sparse, uint8_t* __restrict__ archive,
                                            uint32_t* stage, uint64_t* __restrict__ index = nullptr,
-                                           IdxLayout L = IdxLayout{}, const uint64_t* __restrict__ bit_off = nullptr,
-                                           const uint64_t* __restrict__ out_off = nullptr) {
+                                           IdxLayout L = IdxLayout{}, uint64_t gbits = 0, uint64_t gouts = 0) {
     constexpr bool LDS_BOOK = BOOK != kBookGlobal;
     uint32_t* out32 = reinterpret_cast<uint32_t*>(archive);
     const uint64_t base = k * kChunk + lane * kSymPerThread;
@@ -1080,18 +1117,17 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
     // codeword bit and outlier rank, relative to its group's (the first
     // segment of a group also records the group's)
     constexpr uint32_t kLanesPerSeg = kIdxSeg / kSymPerThread;
-    constexpr uint32_t kChunksPerGroup = kIdxGroup * kIdxSeg / kChunk;
     if (index && (lane % kLanesPerSeg) == 0) {
         const uint64_t g = k * (kChunk / kIdxSeg) + lane / kLanesPerSeg;
         if (g < L.nseg) {
-            const uint64_t G = g / kIdxGroup, k0 = G * kChunksPerGroup;  // the group's first chunk
+            const uint64_t G = g / kIdxGroup;
             const uint64_t bits = cur.bit_off + (excl & 0x1FFFF), outs = cur.out_off + (excl >> 17);
             if (g % kIdxGroup == 0) {
                 index[L.grp + 2 * G] = bits;
                 index[L.grp + 2 * G + 1] = outs;
             }
-            reinterpret_cast<uint32_t*>(index + L.rel_bits)[g] = (uint32_t)(bits - bit_off[k0]);
-            reinterpret_cast<uint16_t*>(index + L.rel_outs)[g] = (uint16_t)(outs - out_off[k0]);
+            reinterpret_cast<uint32_t*>(index + L.rel_bits)[g] = (uint32_t)(bits - gbits);
+            reinterpret_cast<uint16_t*>(index + L.rel_outs)[g] = (uint16_t)(outs - gouts);
         }
     }
 
@@ -1103,7 +1139,7 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
             const uint32_t sj = (cur.w[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
             if ((uint32_t)j < cur.nvalid && sj == 0) {
                 const uint32_t bv = __float_as_uint(sparse[base + j]);
-                uint8_t* dst = archive + S.outlier_byte_off + 4 * r;
+                uint8_t* dst = T.outl + 4 * r;
                 dst[0] = (uint8_t)bv;
                 dst[1] = (uint8_t)(bv >> 8);
                 dst[2] = (uint8_t)(bv >> 16);
@@ -1112,9 +1148,11 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
             }
         }
     }
-    if (S.total_bits == 0) return;  // one-symbol tree: no bitstream
+    if (T.no_stream) return;  // one-symbol tree: no bitstream
 
-    const uint64_t Ak = S.stream_bit_off + cur.bit_off;
+    const uint64_t Ak = T.stream_bit_off + cur.bit_off;
+    const uint64_t end_byte =
+        T.stream_end_byte != ~0ull ? T.stream_end_byte : k + 1 == nchunks ? (Ak + cbits + 7) / 8 : ~0ull;
     const uint64_t wbase = Ak >> 5;
     const uint32_t a0 = (uint32_t)(Ak & 31);
     const uint32_t nwords = (a0 + cbits + 31) >> 5;
@@ -1193,10 +1231,14 @@ __device__ __forceinline__ void pack_chunk(const ChunkRaw& cur, uint64_t k, uint
             const uint32_t mask = ~0u << (32 - a0);
             wv = (__builtin_bswap32(out32[Wg]) & mask) | (wv & ~mask);
         }
-        if ((Wg + 1) * 4 > S.stream_end_byte) {
+        // bytes at or past the stream end are not the stream's; bytes at or
+        // past cap are outside the buffer (a stream byte there: too small)
+        const uint64_t lim = end_byte < T.cap ? end_byte : T.cap;
+        if ((Wg + 1) * 4 > lim) {
             uint8_t* bp = archive + Wg * 4;
             for (int j = 0; j < 4; j++)
-                if (Wg * 4 + j < S.stream_end_byte) bp[j] = (uint8_t)(wv >> (24 - 8 * j));
+                if (Wg * 4 + j < lim) bp[j] = (uint8_t)(wv >> (24 - 8 * j));
+            if (T.cap < min(end_byte, (Wg + 1) * 4)) atomicOr(T.err, 1u);
         } else {
             out32[Wg] = __builtin_bswap32(wv);
         }
@@ -1242,11 +1284,301 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
     const uint32_t fill = LDS_BOOK ? (uint32_t)dict : 0u;
     ChunkRaw nxt;
     load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k, lane, fill, nxt);
+    const PackTarget T = pack_target(S, archive);
     for (; k < nchunks; k += nw) {
         const ChunkRaw cur = nxt;
         if (k + nw < nchunks) load_chunk(codes, n, nchunks, bit_off, out_off, S.n_outlier, k + nw, lane, fill, nxt);
-        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage, index, L,
-                         bit_off, out_off);
+        const uint64_t k0 = k / kChunksPerGroup * kChunksPerGroup;  // the index group's first chunk
+        pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, T, sparse, archive, stage, index, L,
+                         index ? bit_off[k0] : 0, index ? out_off[k0] : 0);
+    }
+}
+
+// ------------------------------------------------------------------ fused single-pass encode
+//
+// k_encode does the work of k_chunk_bits, the chunk scan, k_enc_summary and
+// k_pack in one pass, so the field's codes are read from HBM once (a
+// partition's codes are read again for the packing, from the caches).  The
+// chunks are cut into partitions of kEncPartChunks (kEncWaveChunks per
+// wavefront, four wavefronts per workgroup).  A workgroup takes the next
+// partition from a counter, sums every chunk's
+// bits and outliers, and finds the partition's first bit and outlier rank by
+// a decoupled look-back over the partitions before it: a partition publishes
+// its own totals (aggregate) as soon as it has summed them, and its
+// inclusive prefix once it knows it.  Each published value shares one 64-bit
+// word with its 2-bit flag and is written and read with agent-scope atomics,
+// so a value never arrives without its flag and no cache maintenance is
+// needed between the XCDs.  Partitions are handed out in counter order, so
+// every partition a workgroup waits for was taken earlier by a running
+// workgroup, which publishes its aggregate without waiting on anyone: the
+// look-back always ends.  Then each chunk is packed exactly as k_pack packs
+// it.  The outlier section's offset depends on the total stream length, so
+// outliers go to a compact scratch array by rank, and k_enc_finish writes
+// the sizes and moves them into the archive.
+// WC chunks per wavefront (a divisor or multiple of an index group)
+constexpr uint64_t kStAgg = 1ull << 62, kStIncl = 2ull << 62, kStVal = (1ull << 62) - 1;
+
+__device__ __forceinline__ uint64_t ld_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+    return v;
+}
+
+// Exclusive prefixes (bits, outliers) of partition `part` from the status
+// words of the partitions before it, found by the whole workgroup (every
+// thread must call it; control flow is uniform, from LDS values all threads
+// read alike): in each round thread t looks at partition base - t, so 256
+// predecessors are examined per round trip to memory.  Partition -1 counts
+// as an inclusive prefix of 0.  sh: 24 words of LDS.
+__device__ void lookback_wg(const uint64_t* st_b, const uint64_t* st_z, uint64_t part, uint64_t* sh, uint64_t& pre_b,
+                            uint64_t& pre_z) {
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint64_t* s_mask = sh;     // 4 waves x {incl_b, ready_b, incl_z, ready_z}
+    uint64_t* s_sum = sh + 16; // 4 waves x {bits, zeros}
+    uint64_t sum_b = 0, sum_z = 0;
+    bool done_b = false, done_z = false;
+    int64_t base = (int64_t)part - 1;
+    for (;;) {
+        const int64_t q = base - (int64_t)t;
+        const uint64_t wb = done_b ? 0 : q >= 0 ? ld_status(st_b + q) : kStIncl;
+        const uint64_t wz = done_z ? 0 : q >= 0 ? ld_status(st_z + q) : kStIncl;
+        const uint64_t ib = __ballot((wb >> 62) == 2), rb = __ballot((wb >> 62) != 0);
+        const uint64_t iz = __ballot((wz >> 62) == 2), rz = __ballot((wz >> 62) != 0);
+        if (lane == 0) {
+            s_mask[4 * wave] = ib;
+            s_mask[4 * wave + 1] = rb;
+            s_mask[4 * wave + 2] = iz;
+            s_mask[4 * wave + 3] = rz;
+        }
+        __syncthreads();
+        // nearest inclusive prefix (j = 256: none in this window) and whether
+        // every partition up to it has published
+        int jb = 256, jz = 256;
+        bool okb = true, okz = true;
+#pragma unroll
+        for (int w = 3; w >= 0; w--) {
+            if (s_mask[4 * w]) jb = 64 * w + __builtin_ctzll(s_mask[4 * w]);
+            if (s_mask[4 * w + 2]) jz = 64 * w + __builtin_ctzll(s_mask[4 * w + 2]);
+        }
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const int lo = 64 * w;
+            const uint64_t needb = jb >= lo + 63 ? ~0ull : jb < lo ? 0ull : (2ull << (jb - lo)) - 1;
+            const uint64_t needz = jz >= lo + 63 ? ~0ull : jz < lo ? 0ull : (2ull << (jz - lo)) - 1;
+            okb = okb && (s_mask[4 * w + 1] & needb) == needb;
+            okz = okz && (s_mask[4 * w + 3] & needz) == needz;
+        }
+        const bool ok = (done_b || okb) && (done_z || okz);
+        if (ok) {
+            const uint64_t vb = (!done_b && (int)t <= jb) ? (wb & kStVal) : 0ull;
+            const uint64_t vz = (!done_z && (int)t <= jz) ? (wz & kStVal) : 0ull;
+            const uint64_t sb = wave_sum_u64(vb), sz = wave_sum_u64(vz);
+            if (lane == 0) {
+                s_sum[2 * wave] = sb;
+                s_sum[2 * wave + 1] = sz;
+            }
+        }
+        __syncthreads();
+        if (ok) {
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                sum_b += s_sum[2 * w];
+                sum_z += s_sum[2 * w + 1];
+            }
+            done_b = done_b || jb < 256;
+            done_z = done_z || jz < 256;
+            base -= 256;
+        } else {
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __syncthreads();  // the masks and sums are read before the next round writes them
+        if (done_b && done_z) break;
+    }
+    pre_b = sum_b;
+    pre_z = sum_z;
+}
+
+// Codes of chunk k as lane `lane` holds them (16 symbols, two per word);
+// symbols past the field read as `fill`.
+__device__ __forceinline__ void enc_load_raw(const uint16_t* __restrict__ codes, uint64_t n, uint64_t k, uint32_t lane,
+                                             uint32_t fill, uint32_t* raw) {
+    const uint64_t base = k * kChunk + lane * kSymPerThread;
+    if (base + kSymPerThread <= n) {
+        const uint4* p = reinterpret_cast<const uint4*>(codes + base);
+        const uint4 u0 = p[0], u1 = p[1];
+        raw[0] = u0.x, raw[1] = u0.y, raw[2] = u0.z, raw[3] = u0.w;
+        raw[4] = u1.x, raw[5] = u1.y, raw[6] = u1.z, raw[7] = u1.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t a = base + 2 * j < n ? (uint32_t)codes[base + 2 * j] : fill;
+            const uint32_t b = base + 2 * j + 1 < n ? (uint32_t)codes[base + 2 * j + 1] : fill;
+            raw[j] = a | (b << 16);
+        }
+    }
+}
+
+template <int BOOK, int WC>
+__global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ codes, uint64_t n, uint64_t nchunks,
+                                                uint64_t nparts, const uint32_t* __restrict__ cw_hi,
+                                                const uint32_t* __restrict__ cw_lo,
+                                                const uint8_t* __restrict__ len_tab, int dict,
+                                                const float* __restrict__ sparse, uint8_t* __restrict__ archive,
+                                                PackTarget T, uint64_t* __restrict__ st_bits,
+                                                uint64_t* __restrict__ st_zeros, uint32_t* __restrict__ part_ctr,
+                                                uint32_t stage_words, uint64_t* __restrict__ index, IdxLayout L) {
+    static_assert(BOOK != kBookGlobal, "the fused encode keeps the codebook in LDS");
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_part;
+    __shared__ uint32_t s_wb[4], s_wz[4];
+    __shared__ uint64_t s_lb[24];  // look-back masks and sums
+    uint32_t* tab = lds;
+    uint8_t* tlen = reinterpret_cast<uint8_t*>(lds + (uint32_t)dict + 1);
+    if (BOOK == kBookPacked) {
+        for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+            const uint32_t l = i < dict ? len_tab[i] : 0u;
+            tab[i] = l == 0 ? 0u : l <= 26 ? (cw_lo[i] << 6) | l : kLongCode;
+        }
+    } else {
+        for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
+            tab[i] = i < dict ? cw_lo[i] : 0u;
+            tlen[i] = i < dict ? len_tab[i] : 0u;
+        }
+    }
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint32_t* stage = lds + book_words(BOOK, dict) + wave * stage_words;
+    const uint32_t lane = lane_id();
+    const uint32_t fill = (uint32_t)dict;  // symbols past the field: an LDS slot holding 0
+    for (;;) {
+        if (threadIdx.x == 0) s_part = atomicAdd(part_ctr, 1u);
+        __syncthreads();  // (also orders the codebook fill before the first lookup)
+        const uint64_t part = s_part;
+        if (part >= nparts) break;
+        const uint64_t kw = part * (4 * WC) + (uint64_t)wave * WC;
+
+        // pass 1: bits and outliers of the wavefront's chunks (their codes are
+        // read again by pass 2, from the caches)
+        uint32_t ctot[WC];  // chunk bits (low 17) and outliers (high 15), uniform
+        uint32_t wb = 0, wz = 0;
+#pragma unroll 2
+        for (int i = 0; i < WC; i++) {
+            uint32_t raw[8];
+            enc_load_raw(codes, n, kw + i, lane, fill, raw);
+            uint32_t tbits = 0, tz = 0;
+#pragma unroll
+            for (int j = 0; j < kSymPerThread; j++) {
+                const uint32_t sj = (raw[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+                uint32_t l;
+                if constexpr (BOOK == kBookSplit) {
+                    l = tlen[sj];
+                } else {
+                    l = tab[sj] & 63;
+                    if (l == kLongCode) l = len_tab[sj];
+                }
+                tbits += l;
+                tz += sj == 0;
+            }
+            ctot[i] = lane63(wave_scan_dpp(tbits | (tz << 17)));
+            wb += ctot[i] & 0x1FFFF;
+            wz += ctot[i] >> 17;
+        }
+        if (lane == 0) {
+            s_wb[wave] = wb;
+            s_wz[wave] = wz;
+        }
+        __syncthreads();
+
+        // the partition's prefix: publish the aggregate, look back (the whole
+        // workgroup), publish the inclusive prefix
+        uint64_t pb = 0, pz = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            pb += s_wb[w];
+            pz += s_wz[w];
+        }
+        if (threadIdx.x == 0) {
+            st_status(st_bits + part, (part == 0 ? kStIncl : kStAgg) | pb);
+            st_status(st_zeros + part, (part == 0 ? kStIncl : kStAgg) | pz);
+        }
+        uint64_t eb = 0, ez = 0;
+        if (part > 0) {
+            lookback_wg(st_bits, st_zeros, part, s_lb, eb, ez);
+            if (threadIdx.x == 0) {
+                st_status(st_bits + part, kStIncl | (eb + pb));
+                st_status(st_zeros + part, kStIncl | (ez + pz));
+            }
+        }
+
+        // pass 2: pack each chunk from its first bit and outlier rank
+        uint64_t cb = eb, cz = ez;
+        for (uint32_t w = 0; w < wave; w++) {
+            cb += s_wb[w];
+            cz += s_wz[w];
+        }
+        // the index group's first bit and outlier rank (a group is
+        // kChunksPerGroup chunks: this wavefront's, or it and the one before)
+        uint64_t gbits = cb, gouts = cz;
+        if constexpr (WC < (int)kChunksPerGroup) {
+            static_assert((int)kChunksPerGroup == 2 * WC, "two wavefronts per index group");
+            if (wave & 1) {
+                gbits -= s_wb[wave - 1];
+                gouts -= s_wz[wave - 1];
+            }
+        }
+#pragma unroll 1
+        for (int i = 0; i < WC; i++) {
+            const uint64_t k = kw + i;
+            if (k >= nchunks) break;
+            ChunkRaw cur;
+            enc_load_raw(codes, n, k, lane, fill, cur.w);
+            const uint64_t base = k * kChunk + lane * kSymPerThread;
+            cur.nvalid = base >= n ? 0u : (uint32_t)min<uint64_t>(kSymPerThread, n - base);
+            const uint64_t nx = (k + 1) * kChunk + lane;
+            cur.next = (lane < 32 && k + 1 < nchunks && nx < n) ? (uint32_t)codes[nx] : 0xFFFFFFFFu;
+            cur.bit_off = cb;
+            cur.out_off = cz;
+            cur.nz = ctot[0] >> 17;
+            pack_chunk<BOOK>(cur, k, nchunks, lane, tab, tlen, cw_hi, cw_lo, len_tab, T, sparse, archive, stage,
+                             index, L, gbits, gouts);
+            cb += ctot[0] & 0x1FFFF;
+            cz += ctot[0] >> 17;
+            // the next chunk's totals move to the front (registers, not an indexed array)
+#pragma unroll
+            for (int q = 0; q + 1 < WC; q++) ctot[q] = ctot[q + 1];
+        }
+    }
+}
+
+// Sizes of the fused encode (the last partition's inclusive prefix is the
+// stream's total bits and outliers): the summary and the two length fields of
+// the payload header, then the outliers from the compact scratch into the
+// archive's outlier section, which follows the stream.
+__global__ void __launch_bounds__(256) k_enc_finish(const uint64_t* __restrict__ st_bits,
+                                                    const uint64_t* __restrict__ st_zeros, uint64_t nparts,
+                                                    const uint32_t* __restrict__ outl, EncSummary* __restrict__ sum,
+                                                    uint8_t* __restrict__ archive, uint64_t meta_bytes,
+                                                    uint64_t pad_bytes, uint64_t tree_bytes, uint64_t out_cap,
+                                                    uint32_t* __restrict__ err, uint64_t* __restrict__ index) {
+    const uint64_t tb = nparts ? st_bits[nparts - 1] & kStVal : 0;
+    const uint64_t tz = nparts ? st_zeros[nparts - 1] & kStVal : 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        enc_summary(tb, tz, sum, archive, meta_bytes, pad_bytes, tree_bytes, out_cap, err, index);
+    const uint64_t off = meta_bytes + pad_bytes + 8 + tree_bytes + (tb + 7) / 8;
+    if (off + 4 * tz > out_cap) return;  // enc_summary reports it
+    for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < tz; r += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = outl[r];
+        uint8_t* d = archive + off + 4 * r;
+        d[0] = (uint8_t)v;
+        d[1] = (uint8_t)(v >> 8);
+        d[2] = (uint8_t)(v >> 16);
+        d[3] = (uint8_t)(v >> 24);
     }
 }
 
@@ -1587,7 +1919,8 @@ __global__ void __launch_bounds__(256) k_pack_multi(const BatchField* __restrict
                 ChunkRaw c = cur;
                 c.bit_off -= bb;
                 c.out_off -= zb;
-                pack_chunk<BOOK>(c, k, nch, lane, tab, tlen, cw_hi, cw_lo, len_tab, S, sparse, archive, stage);
+                pack_chunk<BOOK>(c, k, nch, lane, tab, tlen, cw_hi, cw_lo, len_tab, pack_target(S, archive), sparse,
+                                 archive, stage);
             }
         }
         u = ue;
@@ -1746,6 +2079,48 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
     else
         hipLaunchKernelGGL(k_pack<kBookGlobal>, dim3(grid), dim3(256), lds, st, PACK_ARGS);
 #undef PACK_ARGS
+}
+
+uint64_t encode_parts(uint64_t nchunks) { return (nchunks + kEncPartChunks - 1) / kEncPartChunks; }
+
+bool encode_fused_ok(int dict) { return dict <= kMaxLdsBook; }
+
+void launch_encode(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo,
+                   const uint8_t* len_tab, int dict, int max_len, const float* sparse, uint8_t* archive,
+                   uint64_t stream_bit_off, uint64_t out_cap, bool no_stream, uint32_t* outl, uint64_t* status,
+                   uint32_t* err, uint64_t* index, int grid_cap, hipStream_t st) {
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+    const uint64_t nparts = encode_parts(nchunks);
+    const IdxLayout L = idx_layout(index ? n : 0);
+    PackTarget T;
+    T.outl = reinterpret_cast<uint8_t*>(outl);
+    T.stream_bit_off = stream_bit_off;
+    T.stream_end_byte = ~0ull;
+    T.cap = out_cap;
+    T.err = err;
+    T.no_stream = no_stream;
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(nparts, (uint64_t)grid_cap));
+    const uint32_t stage_words = 32u * (uint32_t)std::max(max_len, 1) + 2;
+    const int book = max_len <= 32 ? kBookSplit : kBookPacked;
+    const size_t lds = ((size_t)book_words(book, dict) + 4 * (size_t)stage_words) * 4;
+    uint64_t* sb = status;
+    uint64_t* sz = status + nparts;
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(status + 2 * nparts);
+#define ENC_ARGS codes, n, nchunks, nparts, cw_hi, cw_lo, len_tab, dict, sparse, archive, T, sb, sz, ctr, stage_words, index, L
+    if (book == kBookSplit)
+        hipLaunchKernelGGL((k_encode<kBookSplit, kEncWaveChunks>), dim3(grid), dim3(256), lds, st, ENC_ARGS);
+    else
+        hipLaunchKernelGGL((k_encode<kBookPacked, kEncWaveChunks>), dim3(grid), dim3(256), lds, st, ENC_ARGS);
+#undef ENC_ARGS
+}
+
+void launch_enc_finish(const uint64_t* status, uint64_t n, const uint32_t* outl, uint64_t n_outlier_max,
+                       EncSummary* sum, uint8_t* archive, uint64_t meta_bytes, uint64_t pad_bytes, uint64_t tree_bytes,
+                       uint64_t out_cap, uint32_t* err, uint64_t* index, hipStream_t st) {
+    const uint64_t nparts = encode_parts((n + kChunk - 1) / kChunk);
+    const unsigned grid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n_outlier_max + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_enc_finish, dim3(grid), dim3(256), 0, st, status, status + nparts, nparts, outl, sum, archive,
+                       meta_bytes, pad_bytes, tree_bytes, out_cap, err, index);
 }
 
 // Batched launches (one per stage for all fields).

@@ -298,6 +298,11 @@ class Context:
         autotune() picks); archives do not depend on them."""
         self._check(self._lib.vecsz_ctx_set_launch_shape(self._ctx, int(dq_grid), int(enc_grid)))
 
+    def set_fused_encode(self, on: bool):
+        """One-pass encode (True) or the three-kernel encode (False, the
+        default: faster on an MI355X); identical archives."""
+        self._check(self._lib.vecsz_ctx_set_fused_encode(self._ctx, 1 if on else 0))
+
     def set_batch_fused(self, on: bool):
         """Fused compress batches (one launch per stage for all fields) when
         they qualify; False forces the worker-context path."""
