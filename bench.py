@@ -1,12 +1,13 @@
 #!/usr/bin/env python3
 """Throughput of the dual-quantization + Huffman path on one MI355X.
 
-Workload (default): 3D float32 field 512 x 512 x 512 (512 MiB), Gaussian
-random field with E(k) ~ k^-5/3 synthesised on the GPU from a seed, scaled to
-[-50, 50], compressed at r2r eb = 1e-4 with 8^3 Lorenzo blocks and a 4096-code
-dictionary.  A step is one compress of the field (input resident in HBM,
-archive written to HBM); decompression of the resulting archive is timed the
-same way and reported beside it.
+Workload (default): 3D float32 field 512 x 512 x 1024 (x first; 1 GiB),
+Gaussian random field with E(k) ~ k^-5/3 synthesised on the GPU from a seed,
+scaled to [-50, 50], compressed at r2r eb = 1e-4 with 8^3 Lorenzo blocks and a
+4096-code dictionary.  A step is one compress of the field (input resident in
+HBM, archive written to HBM); decompression of the resulting archive is timed
+the same way and reported beside it.  The 512^3 field at the same settings is
+measured after it and reported under "grf512_r2r1e-4".
 
 Prints ONE JSON line on stdout; progress goes to stderr.
     python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -79,17 +80,27 @@ def measured_traffic(kernel, dims, eb):
     return int(round((f * fetch_scale + w * write_scale) * 1024))
 
 
-def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
-    """The reference's own Compress/Decompress (oracle/_ref) when it was built,
-    else the C restatement, timed on this host's cores."""
-    import numpy as np
+def host_threads():
+    """Host threads this process may use: OMP_NUM_THREADS when set (the GPU
+    box sets it to its CPU share), else the CPUs in the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
 
-    from oracle import pyoracle as po
 
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    # abs mode with eb_final: the same archive bits as r2r without the
-    # reference's re-read of the input file
-    if po.ref_available():
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _time_cpu(po, x_host, eb_abs, dict_size, block, threads, ref):
+    if ref:
         po.ref().ref_set_threads(threads)
         t0 = time.perf_counter()
         a, _ = po.ref_compress(x_host, eb_abs, False, dict_size, block, threads=threads)
@@ -97,7 +108,6 @@ def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
         t0 = time.perf_counter()
         po.ref_decompress(a, None, False, dict_size, block)
         td = time.perf_counter() - t0
-        kind, used = "reference", threads
     else:
         t0 = time.perf_counter()
         a, _, _ = po.compress(x_host, eb_abs, False, dict_size, block)
@@ -105,18 +115,41 @@ def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
         t0 = time.perf_counter()
         po.decompress(a, x_host.shape, False, 0.0, dict_size, block)
         td = time.perf_counter() - t0
-        kind, used = "port", 1
-    n = x_host.size
+    return tc, td
+
+
+def cpu_baseline(x_host, eb_abs, dims, dict_size, block):
+    """The reference's own Compress/Decompress (oracle/_ref, built from its
+    sources in the CPU container and shipped with the tree) when present,
+    else the C restatement (one thread), timed on this host: the whole field
+    on all of this process's threads, and a slab of it (the first quarter of
+    its slowest axis) on one thread."""
+    import numpy as np
+
+    from oracle import pyoracle as po
+
+    ref = po.ref_available()
+    threads = host_threads() if ref else 1
+    # abs mode with eb_final: the same archive bits as r2r without the
+    # reference's re-read of the input file
+    tc, td = _time_cpu(po, x_host, eb_abs, dict_size, block, threads, ref)
+    slab = np.ascontiguousarray(x_host[: max(1, x_host.shape[0] // 4)])
+    tc1, td1 = _time_cpu(po, slab, eb_abs, dict_size, block, 1, ref)
+    n, n1 = x_host.size, slab.size
     return {
         "value": round(4 * n / tc / 1e9, 4),
         "unit": "GB/s",
-        "cores": used,
-        "kind": kind,
-        "sample": f"one compress of the full {'x'.join(map(str, dims))} field in abs mode at the same eb_final "
-                  f"(no file IO) on {used} OpenMP threads; decompress {4 * n / td / 1e9:.4f} GB/s",
+        "cores": threads,
+        "kind": "reference" if ref else "port",
+        "sample": f"one compress and one decompress of the full {'x'.join(map(str, dims))} field in abs mode at the "
+                  f"same eb_final (no file IO) on {threads} OpenMP threads; the 1-thread figures are one compress and "
+                  f"one decompress of its first {slab.shape[0]} slowest-axis planes ({4 * n1 / 2**20:.0f} MiB)",
+        "cpu_model": cpu_model(),
         "compress_s": round(tc, 3),
         "decompress_s": round(td, 3),
         "decompress_gbps": round(4 * n / td / 1e9, 4),
+        "one_thread": {"compress_gbps": round(4 * n1 / tc1 / 1e9, 4), "decompress_gbps": round(4 * n1 / td1 / 1e9, 4),
+                       "compress_s": round(tc1, 3), "decompress_s": round(td1, 3)},
     }
 
 
@@ -140,15 +173,15 @@ def apply_launch_shape(ctx, args, x, eb, mode):
 
 
 def bench_batch(args):
-    """96 fields of uneven shape (2D and 3D GRFs, per-field r2r eb): one
-    batched call against the same fields compressed by one call each, on one
-    context; archives must be identical."""
+    """The ~8 GiB mix of 96 fields of uneven shape (2D and 3D GRFs, per-field
+    r2r eb; datagen.batch_mix): one batched call against the same fields
+    compressed by one call each, on one context; archives must be identical."""
     import torch
 
     from vecsz_amd import Context, datagen, make_params
 
     torch.cuda.set_device(0)
-    mix = datagen.batch_mix_shapes(96, seed=0)
+    mix, mix_seed = datagen.batch_mix(96, seed=0)  # ~8 GiB
     fields, params, outs = [], [], []
     for k, (shape, eb) in enumerate(mix):
         x = datagen.grf(shape, seed=100 + k, device="cuda", pow2=True)
@@ -256,8 +289,8 @@ def bench_batch(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-            "data": "synthetic (96 GRFs E(k)~k^-5/3 cropped from power-of-two grids, seeds 100..195, shapes from "
-                "datagen.batch_mix_shapes(96, 0))",
+        "data": "synthetic (96 GRFs E(k)~k^-5/3 cropped from power-of-two grids, seeds 100..195, shapes and error "
+                f"bounds from datagen.batch_mix(96, 0), draw seed {mix_seed})",
         "config": {"workload": f"batched mix: {len(fields)} fields, {total / 2**30:.3f} GiB, r2r eb per field in "
                                "{1e-2,1e-3,1e-4}, B 8, dict 4096", "parallelism": "single-gpu"},
         "per_field_calls_ms": round(tp * 1e3, 3),
@@ -390,21 +423,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--shape", default="512,512,512", help="C order (z,y,x)")
+    ap.add_argument("--shape", default="512,512,512", help="field workload: C order (z,y,x)")
     ap.add_argument("--eb", type=float, default=1e-4)
     ap.add_argument("--block", type=int, default=8)
     ap.add_argument("--dict", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="grf512", choices=["grf512", "c1", "c3", "target", "c4", "c5"],
-                    help="grf512: 3D GRF of --shape at r2r --eb (default line); c1: 2D 1800x3600 sinusoids, abs "
-                         "1e-4; c3: 1D 2^28 walk + spikes, r2r 1e-4; target: 1024x512x512 GRF, r2r 1e-4; "
-                         "c4: 96-field batched mix vs per-field calls; c5: served Poisson request stream")
+    ap.add_argument("--workload", default="target", choices=["target", "grf512", "field", "c1", "c3", "c4", "c5"],
+                    help="target (default line): 3D GRF 512x512x1024 (1 GiB), r2r 1e-4, with the 512^3 "
+                         "configuration reported beside it; grf512: 3D GRF 512^3, r2r 1e-4; field: 3D GRF of "
+                         "--shape at r2r --eb; c1: 2D 1800x3600 sinusoids, abs 1e-4; c3: 1D 2^28 walk + spikes, "
+                         "r2r 1e-4; c4: the ~8 GiB 96-field batched mix vs per-field calls; c5: served Poisson "
+                         "request stream")
     ap.add_argument("--requests", type=int, default=96, help="c5: requests in the stream")
     ap.add_argument("--load", type=float, default=0.7, help="c5: offered load as a fraction of measured peak")
     ap.add_argument("--workers", default="8", help="c4: comma-separated worker counts to time (first is reported)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the launch-shape autotuner (shapes from --launch-shape, else the defaults)")
     ap.add_argument("--launch-shape", default="", help="DQ_GRID,ENC_GRID: set launch shapes instead of tuning")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="target: skip the 512^3 configuration measured after it (profiling runs)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain device copy of the field (known bytes) to scale HBM counters")
     args = ap.parse_args()
@@ -423,32 +460,63 @@ def main():
         return bench_serve(args)
 
     torch.cuda.set_device(0)
-    mode = "r2r"
-    if args.workload == "c1":
-        shape, mode, args.eb = (1800, 3600), "abs", 1e-4
+    if args.workload == "target":
+        # the 1 GiB target field is the line; the 512^3 configuration rides along
+        out = measure_single(args, "target", full=True)
+        if not args.no_extra:
+            log("measuring the 512^3 configuration")
+            torch.cuda.empty_cache()
+            extra = measure_single(args, "grf512", full=False)
+            out["grf512_r2r1e-4"] = {k: extra[k] for k in ("value", "ms_per_step", "decompress_gbps", "decompress_ms",
+                                                              "archive_bytes", "pipeline_hbm_frac", "roofline",
+                                                              "stages_ms", "latency_ms", "decode_index")}
+    else:
+        out = measure_single(args, args.workload, full=True)
+    print(json.dumps(out), flush=True)
+
+
+def make_field(args, workload):
+    """(tensor, C-order shape, mode, eb, description) of a single-field workload."""
+    from vecsz_amd import datagen
+    import torch
+
+    if workload == "c1":
         log("generating the 2D sinusoid field on the host")
         x = torch.from_numpy(datagen.sinusoids_2d(1800, 3600, seed=0)).cuda()
-        what = "2D f32 3600x1800 sinusoids + noise, abs eb=1e-4"
-    elif args.workload == "c3":
-        shape, args.eb = (1 << 28,), 1e-4
+        return x, (1800, 3600), "abs", 1e-4, "2D f32 3600x1800 sinusoids + noise, abs eb=1e-4"
+    if workload == "c3":
         log("generating the 1D walk + spikes on the GPU")
         x = datagen.walk_spikes_1d(1 << 28, seed=0, device="cuda")
-        what = "1D f32 2^28 random walk + 0.1% spikes, r2r eb=1e-4"
-    else:
-        if args.workload == "target":
-            args.shape, args.eb = "1024,512,512", 1e-4
-        shape = tuple(int(s) for s in args.shape.split(","))
-        log(f"generating GRF {shape} on the GPU")
-        x = datagen.grf(shape, seed=0, device="cuda")
-        what = f"{len(shape)}D f32 {'x'.join(map(str, reversed(shape)))} GRF r2r eb={args.eb}"
+        return x, (1 << 28,), "r2r", 1e-4, "1D f32 2^28 random walk + 0.1% spikes, r2r eb=1e-4"
+    if workload == "target":
+        shape, eb = (1024, 512, 512), 1e-4
+    elif workload == "grf512":
+        shape, eb = (512, 512, 512), 1e-4
+    else:  # "field": --shape / --eb
+        shape, eb = tuple(int(v) for v in args.shape.split(",")), args.eb
+    log(f"generating GRF {shape} on the GPU")
+    x = datagen.grf(shape, seed=0, device="cuda")
+    what = f"{len(shape)}D f32 {'x'.join(map(str, reversed(shape)))} GRF r2r eb={eb}"
+    return x, shape, "r2r", eb, what
+
+
+def measure_single(args, workload, full=True):
+    """One single-field workload: compress and decompress steps, per-stage
+    times, the dominant kernel's roofline, and (full) the decode-index path,
+    the one-pass encode for comparison and the CPU baseline."""
+    import torch
+
+    from vecsz_amd import Context, make_params
+
+    x, shape, mode, eb, what = make_field(args, workload)
     dims = tuple(reversed(shape))
     torch.cuda.synchronize()
     n = x.numel()
     value_range = float((x.max() - x.min()).item()) if mode == "r2r" else 0.0
 
     ctx = Context(0)
-    tune = apply_launch_shape(ctx, args, x, args.eb, mode)
-    p = make_params(dims, args.eb, mode, args.block, args.dict)
+    tune = apply_launch_shape(ctx, args, x, eb, mode)
+    p = make_params(dims, eb, mode, args.block, args.dict)
     cap = ctx.compress_bound(p)
     arch = torch.empty(cap, dtype=torch.uint8, device="cuda")
     y = torch.empty_like(x)
@@ -473,6 +541,7 @@ def main():
     torch.cuda.synchronize()
 
     # timed compress steps
+    log(f"timing {args.steps} compress steps: {what}")
     ctx.enable_timing(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -489,6 +558,7 @@ def main():
     lat.sort()
 
     # timed decompress steps
+    log(f"timing {args.steps} decompress steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -528,16 +598,33 @@ def main():
     # reference's fp32 slack (bit-exact parity is tested in tests/).
     err = (x.double() - y.double()).abs()
     slack = x.double().abs() * 2.0 ** -22
-    eb_final = args.eb * value_range if mode == "r2r" else args.eb
+    eb_final = eb * value_range if mode == "r2r" else eb
     bound_ok = bool((err <= eb_final + slack).all().item())
+    del err, slack
 
     # per-stage device time: HIP events on the library's stream, averaged over
     # a second run of the same K compress steps
     ctx.enable_timing(True)
     cst = stage_avg(ctx, comp, args.steps)
-    ctx.set_fused_encode(True)  # the one-pass encode, for comparison (same archive)
-    cst1 = stage_avg(ctx, comp, args.steps)
-    ctx.set_fused_encode(False)
+    cst1 = cstd = None
+    tdev = 0.0
+    if full:
+        ctx.set_fused_encode(True)  # the one-pass encode, for comparison (same archive)
+        cst1 = stage_avg(ctx, comp, args.steps)
+        ctx.set_fused_encode(False)
+        # the codebook built on the device (no host stop), for comparison (same archive)
+        ctx.set_device_codebook(True)
+        cstd = stage_avg(ctx, comp, min(args.steps, 5))
+        ctx.enable_timing(False)
+        comp()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(min(args.steps, 5)):
+            comp()
+        torch.cuda.synchronize()
+        tdev = (time.perf_counter() - t0) / min(args.steps, 5)
+        ctx.enable_timing(True)
+        ctx.set_device_codebook(False)
     dst = stage_avg(ctx, decomp, args.steps)
     dsti = stage_avg(ctx, decomp_idx, args.steps)
     ctx.enable_timing(False)
@@ -554,7 +641,7 @@ def main():
     dom = max((k for k in cst if k in kern), key=lambda k: cst[k])
     kname, abytes = kern[dom]
     achieved = abytes / (cst[dom] * 1e-3) / 1e9
-    traffic = measured_traffic(kname, tuple(dims), args.eb)
+    traffic = measured_traffic(kname, tuple(dims), eb)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 2),
@@ -569,6 +656,12 @@ def main():
 
     comp_gbps = 4 * n / tc / 1e9
     dec_gbps = 4 * n / td / 1e9
+    stages = {"compress": {k: round(v, 4) for k, v in cst.items()},
+              "decompress": {k: round(v, 4) for k, v in dst.items()}}
+    if cst1 is not None:
+        stages["compress_one_pass_encode"] = {k: round(v, 4) for k, v in cst1.items()}
+    if cstd is not None:
+        stages["compress_device_codebook"] = {k: round(v, 4) for k, v in cstd.items()}
     out = {
         "metric": METRIC,
         "value": round(comp_gbps, 2),
@@ -583,7 +676,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic, seed 0 (see vecsz_amd/datagen.py)",
         "config": {"workload": what + " compress",
-                   "dims_xyz": list(dims), "eb": args.eb, "mode": mode, "block": args.block,
+                   "dims_xyz": list(dims), "eb": eb, "mode": mode, "block": args.block,
                    "dict_size": args.dict, "parallelism": "single-gpu"},
         "decompress_gbps": round(dec_gbps, 2),
         "decompress_ms": round(td * 1e3, 4),
@@ -596,22 +689,26 @@ def main():
         "n_outlier": nout,
         "error_bound_ok": bound_ok,
         "latency_ms": {"p50": round(lat[len(lat) // 2] * 1e3, 4), "p99": round(lat[-1] * 1e3, 4)},
-        "stages_ms": {"compress": {k: round(v, 4) for k, v in cst.items()},
-                      "compress_one_pass_encode": {k: round(v, 4) for k, v in cst1.items()},
-                      "decompress": {k: round(v, 4) for k, v in dst.items()}},
+        "stages_ms": stages,
         "roofline": roofline,
         "launch_shape": tune,
         "pipeline_hbm_frac": {"compress": round((4 * n + size) / tc / 1e9 / HBM_PEAK_GBS, 4),
                               "decompress": round((4 * n + size) / td / 1e9 / HBM_PEAK_GBS, 4)},
     }
-    if not args.no_cpu:
+    if full:
+        out["codebook_place"] = {"host_compress_ms": round(tc * 1e3, 4), "device_compress_ms": round(tdev * 1e3, 4),
+                                 "host_codebook_ms": round(cst.get("host_codebook", 0.0), 4),
+                                 "device_codebook_ms": round(cstd.get("codebook", 0.0), 4)}
+    if full and not args.no_cpu:
         log("timing the CPU baseline")
         x_host = x.cpu().numpy()
         out["cpu_baseline"] = cpu_baseline(x_host, eb_final, dims, args.dict, args.block)
         out["speedup_vs_cpu"] = round(comp_gbps / out["cpu_baseline"]["value"], 2)
-    else:
+        del x_host
+    elif full:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    del x, y, yi, arch, idx, ctx
+    return out
 
 
 if __name__ == "__main__":

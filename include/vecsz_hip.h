@@ -178,6 +178,14 @@ int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
  * identical archives. */
 int vecsz_ctx_set_fused_encode(vecsz_ctx* ctx, int on);
 
+/* 1: a single-field compress with dict_size at most 4096 builds its Huffman
+ * codebook on the device (one workgroup replays the reference's queue, then
+ * codes and the tree bytes in parallel), so it runs to its end without a stop
+ * on the host; 0 (default): the histogram goes to the host, which builds the
+ * codebook (faster: the queue is a sequential chain of dependent steps that a
+ * host core runs many times faster than one GPU lane).  Identical archives. */
+int vecsz_ctx_set_device_codebook(vecsz_ctx* ctx, int on);
+
 /* GPU launch-shape autotuner; replaces autotune_vector_len /
  * autotune_block_sizes (src/autotune.hh:80-127, called from
  * src/vecsz_interface.hh:60-68).  Compresses the field repeatedly (d_out is

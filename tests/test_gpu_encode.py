@@ -90,3 +90,43 @@ def test_one_pass_encode_output_buffer_sizes(gpu_ctx):
                     gpu_ctx.compress_into(p, x.data_ptr(), out.data_ptr(), cap)
     finally:
         gpu_ctx.set_fused_encode(False)
+
+
+@pytest.mark.parametrize("name,x,eb,mode,kw", list(_cases()), ids=[c[0] for c in _cases()])
+def test_device_codebook_matches_host_codebook(gpu_ctx, name, x, eb, mode, kw):
+    """The codebook built on the device (the reference's queue replayed by one
+    lane, codes and tree bytes in parallel; the compress then has no stop on
+    the host) gives the host build's archive, decode index and sidecar."""
+    print(f"device codebook {name} {x.shape}", flush=True)
+    xd = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    a0, n0, i0 = gpu_ctx.compress_indexed(xd, eb, mode, **kw)
+    s0 = gpu_ctx.last_sidecar()
+    try:
+        gpu_ctx.set_device_codebook(True)
+        a1, n1, i1 = gpu_ctx.compress_indexed(xd, eb, mode, **kw)
+        s1 = gpu_ctx.last_sidecar()
+    finally:
+        gpu_ctx.set_device_codebook(False)
+    assert n1 == n0 and torch.equal(a1, a0), name
+    assert torch.equal(i1, i0) and s1 == s0, name
+
+
+def test_device_codebook_errors(gpu_ctx):
+    from vecsz_amd import VecszError, datagen as dg, make_params
+
+    x = torch.from_numpy(dg.grf((20, 30, 40), seed=9)).cuda()
+    a, _ = gpu_ctx.compress(x, 1e-4, "r2r")
+    p = make_params((40, 30, 20), 1e-4, "r2r")
+    try:
+        gpu_ctx.set_device_codebook(True)
+        for cap in (a.numel() - 1, 200, 40):  # short of the stream, of the tree, of the header
+            out = torch.zeros(cap, dtype=torch.uint8, device="cuda")
+            with pytest.raises(VecszError):
+                gpu_ctx.compress_into(p, x.data_ptr(), out.data_ptr(), cap)
+        with pytest.raises(VecszError):  # zero r2r range, found at the end of the call
+            gpu_ctx.compress(torch.zeros(64, 64, device="cuda"), 1e-3, "r2r")
+        out = torch.zeros(a.numel(), dtype=torch.uint8, device="cuda")
+        size, _ = gpu_ctx.compress_into(p, x.data_ptr(), out.data_ptr(), a.numel())
+        assert size == a.numel() and torch.equal(out, a)
+    finally:
+        gpu_ctx.set_device_codebook(False)

@@ -4,7 +4,7 @@ reference by the golden vectors and the live comparisons), at
 
   * 3D 512^3 GRF, r2r eb 1e-2, 1e-3 and 1e-4 (the 512 MiB configuration),
   * 1D 2^28 random walk with 0.1% spikes, r2r 1e-4 (1 GiB),
-  * 3D 1024x512x512 GRF, r2r 1e-4 (the 1 GiB target field),
+  * 3D 512x512x1024 GRF (x first), r2r 1e-4 (the 1 GiB target field),
 
 for the archive-only decode and the decode-index path, and for the ~8 GiB
 96-field batched mix through one fused batch call (src/vecsz_interface.hh:31-229
@@ -86,8 +86,8 @@ def test_walk_2p28(gpu_ctx, pool):
 def test_grf_1gib_target(gpu_ctx, pool):
     from vecsz_amd import datagen as dg
 
-    x = dg.grf((512, 512, 1024), seed=0, device="cuda")  # C order: z 512, y 512, x 1024
-    _check_field(gpu_ctx, pool, x, 1e-4, "1024x512x512 GRF")
+    x = dg.grf((1024, 512, 512), seed=0, device="cuda")  # the bench's target field (x 512, y 512, z 1024)
+    _check_field(gpu_ctx, pool, x, 1e-4, "512x512x1024 GRF")
 
 
 def test_batch_mix_8gib(gpu_ctx, pool):

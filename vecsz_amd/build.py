@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libvecsz_hip.so")
 ARCH = os.environ.get("VECSZ_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "kernels_compress.hip", "kernels_decompress.hip", "codebook.cpp", "lossless.cpp", "kernels_verify.hip",
-           "kernels_pad.hip"]
+           "kernels_pad.hip", "kernels_codebook.hip"]
 # zlib (the same 1.2.11 release the reference vendors) for the gzip pass;
 # libdl to load the system zstd runtime on first use
 LIBS = ["-lz", "-ldl"]

@@ -108,12 +108,14 @@ struct vecsz_ctx {
     int nworkers = 8;
     bool fused_batch = true;  // compress batches as one launch per stage when they qualify
     bool fused_encode = false; // single-field encode as one pass (k_encode): measured slower, off by default
+    bool device_codebook = false; // build the codebook on the device (no stop; measured slower, off by default)
     // launch shapes (workgroup caps of the grid-stride compress kernels), set
     // by vecsz_autotune; the output does not depend on them
     int dq_grid = 768, enc_grid = 2048;
     // compress scratch
     DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
     DevBuf enc_status, outl;  // fused encode: partition status words + counter, compact outliers
+    DevBuf cbinfo;            // device-built codebook: node count, tree bytes, longest codeword, status
     DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
     // fused batch compress: field descriptors, unit starts, per-field histograms,
     // range words, eb families, summaries, status words, codebooks, headers
@@ -585,47 +587,17 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
         launch_dualquant(ndim, B, d_in, S.codes, S.sparse, S.hist, S.partials, a, grid, c->stream);
     }
     CHIP(hipGetLastError());
-    uint8_t* hp = c->pin.as<uint8_t>();
-    CHIP(hipMemcpyAsync(hp, S.hist, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
-    if (device_range)
-        CHIP(hipMemcpyAsync(hp + (size_t)dict * 8, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
-    CHIP(hipStreamSynchronize(c->stream));
-    const uint64_t hist0 = reinterpret_cast<const uint64_t*>(hp)[0];  // code 0: the outliers
-    if (device_range) {
-        EbDev e;
-        memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
-        if (e.zero_range) CFAIL(VECSZ_ERANGE0, "r2r value range is zero");
-        c->sc_range = e.range;
-        c->sc_eb_final = e.eb_final;
-    } else {
-        c->sc_range = r2r ? p->value_range : 0.0;
-        c->sc_eb_final = eb.eb_final;
-    }
-
-    Codebook& cb = NA ? g_na_host.cb : c->cb;
-    {
-        HostTimer ht(c, "host_codebook");
-        rc = build_codebook(reinterpret_cast<const uint64_t*>(hp), dict, cb, NA ? &g_na_host.work : nullptr);
-    }
-    if (rc == VECSZ_ECODELEN) CFAIL(rc, "Huffman code longer than 64 bits");
-    if (rc) CFAIL(rc, "codebook construction failed");
-
-    // Header: datapack metadata (huff_size / n_outlier filled by k_enc_summary),
-    // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and
-    // tree; then the codebook tables.  Both staged in pinned memory.
+    // Codebook: on the host from the histogram (one stop: histogram down,
+    // header, tree and tables up), or, with device_codebook, by k_codebook on
+    // the device, so the compress runs to its end without a stop.
     const uint64_t mb = meta_bytes(ndim);
-    const uint64_t tb = cb.tree.size();
-    const uint64_t head = mb + 8 + tb;
     const uint64_t pad_bytes = 4 * pad_len;
-    if (head + pad_bytes > out_cap) {
-        *out_bytes = head + pad_bytes;
-        CFAIL(VECSZ_ENOMEM, "output buffer too small");
-    }
+    const bool dev_cb = !NA && c->device_codebook && dict <= kMaxLdsBook;
     const size_t book_bytes = (size_t)dict * 9;
-    if (NA ? c->pin.cap < head + book_bytes + 64 : !c->pin.ensure(head + book_bytes + 64))
-        CFAIL(VECSZ_ENOMEM, "pinned buffer");
-    uint8_t* hdr = c->pin.as<uint8_t>();
-    {
+    Codebook& cb = NA ? g_na_host.cb : c->cb;
+    uint64_t hist0 = 0, tb = 0, head = 0;
+    int max_len = 64;  // the longest a device-built codebook may hold
+    auto put_meta = [&](uint8_t* hdr) {
         memset(hdr, 0, mb);
         int32_t nd = ndim;
         memcpy(&hdr[0], &nd, 4);
@@ -635,28 +607,83 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
         }
         memcpy(&hdr[4 * (ndim + 1)], &p->eb, 8);
         memcpy(&hdr[4 * (ndim + 1) + 8], &pad_len, 8);
-        put_u32be(&hdr[mb], (uint32_t)cb.node_count);
-        put_u32be(&hdr[mb + 4], (uint32_t)dict);  // stateNum / 2
-        memcpy(&hdr[mb + 8], cb.tree.data(), tb);
-        uint8_t* bk = hdr + head;
-        memcpy(bk, cb.cw_hi.data(), (size_t)dict * 4);
-        memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
-        memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
-    }
-    if (pad_bytes == 0) {
-        CHIP(hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+    };
+    if (!dev_cb) {
+        uint8_t* hp = c->pin.as<uint8_t>();
+        CHIP(hipMemcpyAsync(hp, S.hist, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
+        if (device_range)
+            CHIP(hipMemcpyAsync(hp + (size_t)dict * 8, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
+        CHIP(hipStreamSynchronize(c->stream));
+        hist0 = reinterpret_cast<const uint64_t*>(hp)[0];  // code 0: the outliers
+        if (device_range) {
+            EbDev e;
+            memcpy(&e, hp + (size_t)dict * 8, sizeof(e));
+            if (e.zero_range) CFAIL(VECSZ_ERANGE0, "r2r value range is zero");
+            c->sc_range = e.range;
+            c->sc_eb_final = e.eb_final;
+        } else {
+            c->sc_range = r2r ? p->value_range : 0.0;
+            c->sc_eb_final = eb.eb_final;
+        }
+        {
+            HostTimer ht(c, "host_codebook");
+            rc = build_codebook(reinterpret_cast<const uint64_t*>(hp), dict, cb, NA ? &g_na_host.work : nullptr);
+        }
+        if (rc == VECSZ_ECODELEN) CFAIL(rc, "Huffman code longer than 64 bits");
+        if (rc) CFAIL(rc, "codebook construction failed");
+        max_len = cb.max_len;
+
+        // Header: datapack metadata (huff_size / n_outlier filled by k_enc_summary),
+        // Huffman header [u32be nodeCount][u32be dict] (src/huffman.hh:649-653) and
+        // tree; then the codebook tables.  Both staged in pinned memory.
+        tb = cb.tree.size();
+        head = mb + 8 + tb;
+        if (head + pad_bytes > out_cap) {
+            *out_bytes = head + pad_bytes;
+            CFAIL(VECSZ_ENOMEM, "output buffer too small");
+        }
+        if (NA ? c->pin.cap < head + book_bytes + 64 : !c->pin.ensure(head + book_bytes + 64))
+            CFAIL(VECSZ_ENOMEM, "pinned buffer");
+        uint8_t* hdr = c->pin.as<uint8_t>();
+        {
+            put_meta(hdr);
+            put_u32be(&hdr[mb], (uint32_t)cb.node_count);
+            put_u32be(&hdr[mb + 4], (uint32_t)dict);  // stateNum / 2
+            memcpy(&hdr[mb + 8], cb.tree.data(), tb);
+            uint8_t* bk = hdr + head;
+            memcpy(bk, cb.cw_hi.data(), (size_t)dict * 4);
+            memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
+            memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
+        }
+        if (pad_bytes == 0) {
+            CHIP(hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
+        } else {
+            // [metadata][pad values][Huffman header + tree] (src/utils/datapack.hh:17-29)
+            CHIP(hipMemcpyAsync(d_out, hdr, mb, hipMemcpyHostToDevice, c->stream));
+            CHIP(hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
+            CHIP(hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
+        }
+        CHIP(hipMemcpyAsync(S.book, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
     } else {
-        // [metadata][pad values][Huffman header + tree] (src/utils/datapack.hh:17-29)
+        if (mb + pad_bytes + 8 > out_cap) {
+            *out_bytes = mb + pad_bytes + 8;
+            return fail(c, VECSZ_ENOMEM, "output buffer too small");
+        }
+        if (!c->cbinfo.ensure(sizeof(CodebookInfo))) return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+        // the metadata goes up from pinned memory (no stop); pad values follow it
+        uint8_t* hdr = c->pin.as<uint8_t>();
+        put_meta(hdr);
         CHIP(hipMemcpyAsync(d_out, hdr, mb, hipMemcpyHostToDevice, c->stream));
-        CHIP(hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
-        CHIP(hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
+        if (pad_bytes) CHIP(hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
+        StageScope s(c, "codebook");
+        launch_codebook(S.hist, dict, S.book, d_out + mb + pad_bytes, out_cap - mb - pad_bytes,
+                        c->cbinfo.as<CodebookInfo>(), c->stream);
     }
-    CHIP(hipMemcpyAsync(S.book, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
     const uint32_t* cw_hi = S.book;
     const uint32_t* cw_lo = cw_hi + dict;
     const uint8_t* cw_len = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
     launch_zero_u32(S.err, 4, c->stream);
-    if (NA ? c->pin2.cap < 256 : !c->pin2.ensure(256)) CFAIL(VECSZ_ENOMEM, "pinned buffer");
+    if (NA ? c->pin2.cap < 512 : !c->pin2.ensure(512)) CFAIL(VECSZ_ENOMEM, "pinned buffer");
     if (d_index) {
         // index header words 0-2 and 5; k_enc_summary writes words 3 and 4
         uint64_t* ih = reinterpret_cast<uint64_t*>(c->pin2.as<uint8_t>() + 192);
@@ -670,7 +697,7 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     // Encode: one pass (k_encode + k_enc_finish) when the codebook fits in LDS,
     // else chunk bits, a scan of them, the sizes and the packing.
     const uint64_t nparts = encode_parts(nchunks);
-    bool fused = c->fused_encode && encode_fused_ok(dict);
+    bool fused = c->fused_encode && encode_fused_ok(dict) && !dev_cb;  // (needs the outlier count on the host)
     if (fused && !NA) {
         if (!c->enc_status.ensure((2 * nparts + 1) * 8) || !c->outl.ensure(hist0 * 4 + 64))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
@@ -682,8 +709,8 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     if (fused) {
         StageScope s(c, "encode");
         launch_zero_u32(reinterpret_cast<uint32_t*>(S.enc_status), (2 * nparts + 1) * 2, c->stream);
-        launch_encode(S.codes, n, cw_hi, cw_lo, cw_len, dict, cb.max_len, S.sparse, d_out, (head + pad_bytes) * 8,
-                      out_cap, cb.max_len == 0, S.outl, S.enc_status, S.err, d_index, c->enc_grid, c->stream);
+        launch_encode(S.codes, n, cw_hi, cw_lo, cw_len, dict, max_len, S.sparse, d_out, (head + pad_bytes) * 8,
+                      out_cap, max_len == 0, S.outl, S.enc_status, S.err, d_index, c->enc_grid, c->stream);
         launch_enc_finish(S.enc_status, n, S.outl, hist0, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index,
                           c->stream);
     } else {
@@ -707,24 +734,54 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
             StageScope s(c, "scan");
             launch_scan_u32x2(S.chunk_bits, S.chunk_zeros, nchunks, S.bit_off, S.out_off, S.scan_tmp, S.scan_tot,
                               c->stream);
-            launch_enc_summary(S.scan_tot, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index, c->stream);
+            if (dev_cb)
+                launch_enc_summary_dev(S.scan_tot, S.summary, d_out, mb, pad_bytes, c->cbinfo.as<CodebookInfo>(),
+                                       out_cap, S.err, d_index, c->stream);
+            else
+                launch_enc_summary(S.scan_tot, S.summary, d_out, mb, pad_bytes, tb, out_cap, S.err, d_index,
+                                   c->stream);
         }
         {
             StageScope s(c, "pack");
             launch_pack(S.codes, n, cw_hi, cw_lo, cw_len, dict, S.bit_off, S.out_off, nchunks, S.summary, S.sparse,
-                        d_out, S.err, std::max(cb.max_len, 1), c->enc_grid, d_index, c->stream);
+                        d_out, S.err, std::max(max_len, 1), c->enc_grid, d_index, c->stream);
         }
     }
     CHIP(hipGetLastError());
     uint8_t* h = c->pin2.as<uint8_t>();
     CHIP(hipMemcpyAsync(h, S.summary, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
     CHIP(hipMemcpyAsync(h + 128, S.err, 4, hipMemcpyDeviceToHost, c->stream));
+    if (dev_cb) {
+        CHIP(hipMemcpyAsync(h + 256, c->cbinfo.p, sizeof(CodebookInfo), hipMemcpyDeviceToHost, c->stream));
+        if (device_range) CHIP(hipMemcpyAsync(h + 320, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
+    }
     CHIP(hipStreamSynchronize(c->stream));
     EncSummary ES;
     memcpy(&ES, h, sizeof(ES));
     uint32_t e;
     memcpy(&e, h + 128, 4);
     timing_end(c);
+    if (dev_cb) {
+        // what the host path checks at its stop: the range, then the codebook
+        if (device_range) {
+            EbDev ed;
+            memcpy(&ed, h + 320, sizeof(ed));
+            if (ed.zero_range) return fail(c, VECSZ_ERANGE0, "r2r value range is zero");
+            c->sc_range = ed.range;
+            c->sc_eb_final = ed.eb_final;
+        } else {
+            c->sc_range = r2r ? p->value_range : 0.0;
+            c->sc_eb_final = eb.eb_final;
+        }
+        CodebookInfo ci;
+        memcpy(&ci, h + 256, sizeof(ci));
+        if (ci.status == VECSZ_ECODELEN) return fail(c, VECSZ_ECODELEN, "Huffman code longer than 64 bits");
+        if (ci.status == VECSZ_ENOMEM) {
+            *out_bytes = mb + pad_bytes + 8 + ci.tree_bytes;
+            return fail(c, VECSZ_ENOMEM, "output buffer too small");
+        }
+        if (ci.status) return fail(c, ci.status, "codebook construction failed");
+    }
     *out_bytes = ES.archive_size;
     if (n_outlier) *n_outlier = ES.n_outlier;
     if (e) CFAIL(VECSZ_ENOMEM, "output buffer too small");
@@ -1692,6 +1749,12 @@ extern "C" {
 int vecsz_ctx_set_workers(vecsz_ctx* c, int nworkers) {
     if (!c || nworkers < 1 || nworkers > 16) return VECSZ_EINVAL;
     c->nworkers = nworkers;
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_set_device_codebook(vecsz_ctx* c, int on) {
+    if (!c) return VECSZ_EINVAL;
+    c->device_codebook = on != 0;
     return VECSZ_OK;
 }
 

@@ -38,6 +38,23 @@ void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const
                  const float* sparse, uint8_t* archive, const uint32_t* err, int max_len, int grid_cap,
                  uint64_t* index, hipStream_t st);
 
+// Huffman codebook built on the device (kernels_codebook.hip), one
+// workgroup, dictionaries up to kMaxLdsBook: book = cw_hi[dict], cw_lo[dict],
+// u8 len[dict]; huff = the archive's Huffman section ([u32be node count]
+// [u32be dict][tree]), huff_cap bytes available there.
+struct CodebookInfo {
+    uint32_t node_count, tree_bytes, max_len;
+    int32_t status;  // VECSZ_OK, VECSZ_ECODELEN, VECSZ_EINVAL (empty histogram), VECSZ_ENOMEM (no room)
+};
+size_t codebook_lds_bytes(int dict);
+void launch_codebook(const unsigned long long* hist, int dict, uint32_t* book, uint8_t* huff, uint64_t huff_cap,
+                     CodebookInfo* info, hipStream_t st);
+// k_enc_summary with the tree size from a device-built codebook (a failed
+// build sets err to 2 and writes nothing).
+void launch_enc_summary_dev(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
+                            uint64_t pad_bytes, const CodebookInfo* info, uint64_t out_cap, uint32_t* err,
+                            uint64_t* index, hipStream_t st);
+
 // Fused single-pass encode (bits, prefix by decoupled look-back, packing) for
 // dictionaries whose codebook fits in LDS (encode_fused_ok).  status: 2 *
 // encode_parts(nchunks) + 1 words, zeroed before the launch; outl: 4 bytes per

@@ -939,6 +939,18 @@ __global__ void k_enc_summary(const uint64_t* __restrict__ totals, EncSummary* _
         enc_summary(totals[0], totals[1], sum, archive, meta_bytes, pad_bytes, tree_bytes, out_cap, err, index);
 }
 
+__global__ void k_enc_summary_dev(const uint64_t* __restrict__ totals, EncSummary* __restrict__ sum,
+                                  uint8_t* __restrict__ archive, uint64_t meta_bytes, uint64_t pad_bytes,
+                                  const CodebookInfo* __restrict__ info, uint64_t out_cap, uint32_t* __restrict__ err,
+                                  uint64_t* __restrict__ index) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (info->status != 0) {
+        *err = 2;
+        return;
+    }
+    enc_summary(totals[0], totals[1], sum, archive, meta_bytes, pad_bytes, info->tree_bytes, out_cap, err, index);
+}
+
 // Bit packing, one wavefront per 1024-symbol chunk, no workgroup barriers or
 // global atomics.  Lane L holds 16 consecutive symbols; a wavefront scan gives
 // its first bit inside the chunk.  The lanes OR their codewords into a
@@ -2057,6 +2069,13 @@ void launch_enc_summary(const uint64_t* totals, EncSummary* sum, uint8_t* archiv
                         hipStream_t st) {
     hipLaunchKernelGGL(k_enc_summary, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, pad_bytes,
                        tree_bytes, out_cap, err, index);
+}
+
+void launch_enc_summary_dev(const uint64_t* totals, EncSummary* sum, uint8_t* archive, uint64_t meta_bytes,
+                            uint64_t pad_bytes, const CodebookInfo* info, uint64_t out_cap, uint32_t* err,
+                            uint64_t* index, hipStream_t st) {
+    hipLaunchKernelGGL(k_enc_summary_dev, dim3(1), dim3(64), 0, st, totals, sum, archive, meta_bytes, pad_bytes, info,
+                       out_cap, err, index);
 }
 
 void launch_pack(const uint16_t* codes, uint64_t n, const uint32_t* cw_hi, const uint32_t* cw_lo, const uint8_t* len_tab,

@@ -298,6 +298,11 @@ class Context:
         autotune() picks); archives do not depend on them."""
         self._check(self._lib.vecsz_ctx_set_launch_shape(self._ctx, int(dq_grid), int(enc_grid)))
 
+    def set_device_codebook(self, on: bool):
+        """Codebook built on the device (True: no host stop) or on the host
+        (False, the default: faster); identical archives."""
+        self._check(self._lib.vecsz_ctx_set_device_codebook(self._ctx, 1 if on else 0))
+
     def set_fused_encode(self, on: bool):
         """One-pass encode (True) or the three-kernel encode (False, the
         default: faster on an MI355X); identical archives."""
