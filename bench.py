@@ -276,6 +276,27 @@ def bench_batch(args):
     torch.cuda.synchronize()
     tdb = (time.perf_counter() - t0) / args.steps
     dsame = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
+    # the worker-context path on the same archives (the fused path is the default)
+    log("timing the worker-context decompress batch")
+    ctx.set_batch_fused(False)
+    ys.clear()
+    ys.extend(torch.empty_like(f) for f in fields)
+    dbatch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dbatch()
+    torch.cuda.synchronize()
+    tdw = (time.perf_counter() - t0) / args.steps
+    ctx.set_batch_fused(True)
+    dsame = dsame and all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
+    # stage times of the fused batch calls (HIP events on the context's stream)
+    ctx.enable_timing(True)
+    bst = stage_avg(ctx, batch, min(args.steps, 3))
+    dbst = stage_avg(ctx, dbatch, min(args.steps, 3))
+    ctx.enable_timing(False)
+    log("fused batch compress stages (ms):", {k: round(v, 4) for k, v in bst.items()})
+    log("fused batch decompress stages (ms):", {k: round(v, 4) for k, v in dbst.items()})
     arch = sum(s for s, _ in rb)
     out = {
         "metric": METRIC + " [batched form, 96-field mix]",
@@ -303,6 +324,9 @@ def bench_batch(args):
         "decompress_per_field_calls_ms": round(tdp * 1e3, 3),
         "decompress_batch_ms": round(tdb * 1e3, 3),
         "decompress_batch_speedup": round(tdp / tdb, 3),
+        "decompress_batch_worker_contexts_ms": round(tdw * 1e3, 3),
+        "batch_stages_ms": {"compress": {k: round(v, 4) for k, v in bst.items()},
+                            "decompress": {k: round(v, 4) for k, v in dbst.items()}},
         "identical_outputs": bool(dsame),
         "archive_bytes": arch,
     }

@@ -102,7 +102,7 @@ int vecsz_decompress(vecsz_ctx* ctx, const vecsz_params* p, const uint8_t* d_in,
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc): the same archive as
  * vecsz_compress, byte for byte, with every device scratch buffer inside a
- * static array of the library (about 28 MiB per GPU, held until the process
+ * static array of the library (about 32 MiB per GPU, held until the process
  * ends) and no host allocation during the call (the context reserves its
  * staging when it is created).  Limits: at most 2^22 elements (a 16 MiB
  * field), dict_size at most 4096, no padding; larger requests return
@@ -153,10 +153,15 @@ int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint
  * most 4096) and use no padding runs fused: one launch per stage covers every
  * field (a work queue of Lorenzo-block tiles and encode chunks spanning all
  * fields, per-field histograms and codebooks, the codebooks built on
- * set_workers host threads).  Other batches, and every decompress batch, are
- * spread, largest first, over worker contexts (own HIP stream and scratch
- * each, set_workers of them), so small fields overlap on the GPU and host
- * codebook work overlaps kernels.  status[k] receives each field's VECSZ_*
+ * set_workers host threads).  Decompress of two or more archives with one
+ * block size and dict_size (at most 4096) and no pad section runs fused the
+ * same way: one copy of every archive's header and tree to the host, the
+ * decode tables built on host threads, then one launch per stage (sync pass,
+ * re-decode passes over one batch-wide list, scan, write pass, one
+ * reconstruction launch per tile form).  Other batches are spread, largest
+ * first, over worker contexts (own HIP stream and scratch each, set_workers
+ * of them), so small fields overlap on the GPU and host codebook work
+ * overlaps kernels.  status[k] receives each field's VECSZ_*
  * status; the call returns the first failure (or VECSZ_OK) after all fields
  * are done.  Inputs must be complete on the context's stream. */
 int vecsz_compress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* params, const float* const* d_in,
@@ -166,8 +171,9 @@ int vecsz_decompress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* para
                            const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status);
 /* Worker contexts used by the batched calls (1..16, default 8). */
 int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
-/* 0: compress batches always take the worker-context path; 1 (default): the
- * fused path when the batch qualifies.  Both write identical archives. */
+/* 0: batches always take the worker-context path; 1 (default): the fused
+ * path when the batch qualifies.  Both write identical archives and
+ * identical decompressed fields. */
 int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
 
 /* 1: a single-field compress encodes in one pass (bit counts, prefix by a
@@ -220,10 +226,11 @@ int vecsz_ctx_set_launch_shape(vecsz_ctx* ctx, int dq_grid, int enc_grid);
  * queue; a request is admitted when a slot is free and the device bytes it
  * holds while running (its buffers plus its scratch) fit in what is left of
  * hbm_budget, so requests that arrive while others run start as soon as room
- * frees up and overlap on the GPU.  A slot that starts a small compress
- * request also takes the compatible compress requests queued right behind it
- * (same block size and dictionary, no padding) and runs them as one fused
- * batch launch.  Results are those of vecsz_compress / vecsz_decompress on the
+ * frees up and overlap on the GPU.  A slot that starts a small request (at
+ * most 4M elements) also takes the compatible requests of the same kind
+ * queued right behind it (same block size and dictionary, at most 4096; no
+ * padding for compress) and runs them as one fused batch launch, compress or
+ * decompress.  Results are those of vecsz_compress / vecsz_decompress on the
  * same request.  Buffers are the caller's device memory and must stay valid
  * until the request is waited for. */
 typedef struct vecsz_server vecsz_server;

@@ -105,3 +105,47 @@ def test_server_folds_queued_compress_requests(gpu_ctx):
         assert torch.equal(r["out"][: q["out_bytes"]], r["a"])
     # requests that shared a launch started and ended together
     assert len(set(q["run_ms"] for q in res)) < len(res), [q["run_ms"] for q in res]
+
+
+def test_server_folds_queued_decompress_requests(gpu_ctx):
+    """Small decompress requests waiting together are folded into one fused
+    batch decompress; each output is the one the request gives alone, and a
+    member whose archive is corrupt fails on its own while the others
+    succeed."""
+    from vecsz_amd import Server, VecszError, datagen as dg, make_params
+
+    rng = np.random.default_rng(21)
+    reqs = []
+    for i in range(40):
+        nd = int(rng.integers(1, 4))
+        s = (int(rng.integers(100, 50000)),) if nd == 1 else tuple(int(v) for v in rng.integers(8, 300 if nd == 2 else 48, nd))
+        x = torch.from_numpy(dg.grf(s, seed=90 + i) if nd > 1 else
+                             np.cumsum(rng.normal(0, 0.05, s)).astype(np.float32)).cuda()
+        eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        a, _ = gpu_ctx.compress(x, eb, "abs")
+        a = a.clone()
+        y = gpu_ctx.decompress(a, "abs")
+        reqs.append({"a": a, "y": y.clone(), "out": torch.empty_like(x)})
+    bad = 17
+    reqs[bad]["a"][0] = 9  # ndim 9: a corrupt header
+    torch.cuda.synchronize()
+    srv = Server(0, slots=1)
+    print("folded decompress requests", flush=True)
+    pd = make_params((1,), 1e-4, "abs")
+    tickets = [srv.submit_decompress(pd, r["a"].data_ptr(), r["a"].numel(), r["out"].data_ptr(), r["out"].numel())
+               for r in reqs]
+    res = []
+    for k, t in enumerate(tickets):
+        if k == bad:
+            with pytest.raises(VecszError):
+                srv.wait(t)
+            res.append(None)
+        else:
+            res.append(srv.wait(t))
+    srv.close()
+    for k, (r, q) in enumerate(zip(reqs, res)):
+        if k != bad:
+            assert torch.equal(r["out"].view(torch.int32), r["y"].view(torch.int32)), k
+    # requests that shared a launch started and ended together
+    runs = [q["run_ms"] for q in res if q is not None]
+    assert len(set(runs)) < len(runs), runs

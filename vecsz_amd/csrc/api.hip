@@ -106,7 +106,7 @@ struct vecsz_ctx {
     // batch workers: each a context of its own (stream + scratch), created on first use
     std::vector<vecsz_ctx*> workers;
     int nworkers = 8;
-    bool fused_batch = true;  // compress batches as one launch per stage when they qualify
+    bool fused_batch = true;  // batches as one launch per stage when they qualify
     bool fused_encode = false; // single-field encode as one pass (k_encode): measured slower, off by default
     bool device_codebook = false; // build the codebook on the device (no stop; measured slower, off by default)
     // launch shapes (workgroup caps of the grid-stride compress kernels), set
@@ -122,6 +122,12 @@ struct vecsz_ctx {
     DevBuf b_desc, b_start, b_hist, b_range, b_ebd, b_sums, b_errs, b_book, b_head, b_rows;
     HostBuf b_pin, b_pin_hist;
     std::vector<Codebook> b_cb;
+    // fused batch decompress: field descriptors, unit starts, head copies,
+    // gathered heads, summaries, symbol totals, fallback counts; pinned
+    // staging of the same; decode tables per field
+    DevBuf db_desc, db_start, db_hc, db_heads, db_sums, db_tot, db_nfl;
+    HostBuf db_pin, db_hpin, db_back;
+    std::vector<DecodeTables> db_dt;
     Codebook cb;
     DecodeTables dt;  // kept between calls: its vectors keep their capacity
     // metadata of the last successful compress
@@ -1011,7 +1017,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         da.nsub = nsub;
         da.outliers = d_outl;
         da.n_outlier = nout;
-        SubState ss;
+        SubState ss{};  // jbase 0: list entries are this field's subsequence numbers
         ss.b = c->sub_b.as<uint64_t>();
         ss.e = c->sub_e.as<uint64_t>();
         ss.cnt = c->sub_cnt.as<uint32_t>();
@@ -1744,6 +1750,375 @@ static int compress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, con
     return rc;
 }
 
+// Fused batch decompress: every field shares the block size and a dictionary
+// of at most kMaxLdsBook, and has no pad section.  One copy brings every
+// archive's metadata, Huffman header and tree to the host (the tree of such
+// a dictionary is at most tree_byte_size(2 dict - 1) bytes), the decode
+// tables are built on host threads and go back in one copy, and then each
+// stage is one launch over all fields: the sync pass, the re-decode passes
+// over one batch-wide list, one scan over every subsequence of the batch,
+// the write pass, and the reconstruction (one launch per tile form) with its
+// fp32 fallback.  Outputs are the single-field path's.
+static bool dbatch_fusable(const vecsz_ctx* c, int nf, const vecsz_params* p, const uint8_t* const* d_in,
+                           float* const* d_out) {
+    if (nf < 2 || nf > 4096) return false;
+    for (int k = 0; k < nf; k++) {
+        if (!d_in[k] || !d_out[k]) return false;
+        if (p[k].block != p[0].block || p[k].dict_size != p[0].dict_size) return false;
+        if (check_params(const_cast<vecsz_ctx*>(c), &p[k], false)) return false;
+    }
+    return p[0].dict_size <= kMaxLdsBook;
+}
+
+// *fallback: an archive has a pad section; nothing was written and the
+// caller takes the per-field path instead.
+static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* p, const uint8_t* const* d_in,
+                                      const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap,
+                                      int* status, bool* reported, bool* fallback) {
+    HIPCHK(c, hipSetDevice(c->device));
+    timing_begin(c);
+    const int B = p[0].block, dict = p[0].dict_size, R = dict / 2;
+
+    // 1. the head of every archive (metadata, Huffman header, tree) in one copy
+    const uint64_t head_max = meta_bytes(3) + 8 + tree_byte_size(2 * (uint64_t)dict - 1);
+    std::vector<HeadCopy> hc(nf);
+    uint64_t htot = 0;
+    for (int k = 0; k < nf; k++) {
+        hc[k].src = d_in[k];
+        hc[k].len = std::min<uint64_t>(in_bytes[k], head_max);
+        hc[k].off = htot;
+        htot += (hc[k].len + 15) / 16 * 16;
+    }
+    const size_t hc_bytes = (size_t)nf * sizeof(HeadCopy);
+    if (!c->db_hc.ensure(hc_bytes) || !c->db_heads.ensure(htot + 64))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    if (!c->db_pin.ensure(hc_bytes + 64) || !c->db_hpin.ensure(htot + 64)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    memcpy(c->db_pin.p, hc.data(), hc_bytes);
+    HIPCHK(c, hipMemcpyAsync(c->db_hc.p, c->db_pin.p, hc_bytes, hipMemcpyHostToDevice, c->stream));
+    launch_gather_heads(c->db_hc.as<HeadCopy>(), nf, c->db_heads.as<uint8_t>(), c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->db_hpin.p, c->db_heads.p, htot, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+
+    // 2. headers, then the decode tables on host threads
+    struct Hdr {
+        int ndim;
+        uint64_t dims[3], n, pad_len, huff_size, nout, tb;
+        double ebu;
+    };
+    std::vector<Hdr> hd(nf);
+    const uint8_t* heads = c->db_hpin.as<uint8_t>();
+    for (int k = 0; k < nf; k++) {
+        Hdr& h = hd[k];
+        const uint8_t* hp = heads + hc[k].off;
+        if (parse_meta(hp, hc[k].len, in_bytes[k], &h.ndim, h.dims, &h.ebu, &h.pad_len, &h.huff_size, &h.nout)) {
+            status[k] = VECSZ_ECORRUPT;
+            continue;
+        }
+        if (h.pad_len) {
+            *fallback = true;
+            return VECSZ_OK;
+        }
+        h.n = h.dims[0] * h.dims[1] * h.dims[2];
+        const uint64_t mb = meta_bytes(h.ndim);
+        if (h.n > out_cap[k]) {
+            status[k] = VECSZ_ENOMEM;
+        } else if (p[k].mode == VECSZ_MODE_R2R && !(p[k].value_range != 0)) {
+            status[k] = VECSZ_EINVAL;
+        } else if (h.huff_size < 8 || mb + 8 > hc[k].len) {
+            status[k] = VECSZ_ECORRUPT;
+        } else {
+            const uint8_t* q = hp + mb;
+            const int32_t ncs = (int32_t)(((uint32_t)q[0] << 24) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 8) | q[3]);
+            h.tb = ncs > 0 ? tree_byte_size((uint64_t)ncs) : 0;
+            // a tree beyond what the dictionary allows cannot be in the head read
+            if (ncs <= 0 || 8 + h.tb > h.huff_size || mb + 8 + h.tb > hc[k].len) status[k] = VECSZ_ECORRUPT;
+        }
+    }
+    if ((int)c->db_dt.size() < nf) c->db_dt.resize(nf);
+    {
+        HostTimer ht(c, "host_tables");
+        std::atomic<int> next{0};
+        auto body = [&]() {
+            for (int k; (k = next.fetch_add(1)) < nf;) {
+                if (status[k]) continue;
+                DecodeTables& dt = c->db_dt[k];
+                if (build_decode_tables(heads + hc[k].off + meta_bytes(hd[k].ndim), hd[k].huff_size, dt) ||
+                    (!dt.root_leaf && !tree_is_full(dt)))
+                    status[k] = VECSZ_ECORRUPT;
+            }
+        };
+        const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+        const int nt = std::max(1, std::min(nf, std::max(c->nworkers, std::min(16, hw))));
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(body);
+        body();
+        for (auto& t : th) t.join();
+    }
+
+    // 3. layout: code / outlier regions (64-element aligned), tables,
+    // subsequences, fallback block lists, unit and tile starts
+    bool t16 = true;
+    uint32_t max_sub16 = 0;
+    for (int k = 0; k < nf; k++) {
+        if (status[k] || c->db_dt[k].root_leaf) continue;
+        const DecodeTables& dt = c->db_dt[k];
+        if (dt.lut16.empty() || dt.sub16.size() > kMaxSub16) t16 = false;
+        max_sub16 = std::max<uint32_t>(max_sub16, (uint32_t)dt.sub16.size());
+    }
+    std::vector<uint64_t> off(nf, 0), toff(nf, 0), fl_off(nf, 0);
+    uint64_t ntot = 0, tab_tot = 0, nsub_tot = 0, fl_tot = 0;
+    const size_t ns = (size_t)nf + 1;
+    // starts: sync units, write units, subsequences, then one array per tile form
+    std::vector<uint64_t> st((3 + kRcKinds) * ns, 0);
+    uint64_t* su = st.data();
+    uint64_t* wu = su + ns;
+    uint64_t* sb = wu + ns;
+    uint64_t* rk = sb + ns;  // rk + kind * ns
+    std::vector<DecField> fd(nf);
+    for (int k = 0; k < nf; k++) {
+        DecField& F = fd[k];
+        memset(&F, 0, sizeof(F));
+        F.kind = -1;
+        su[k + 1] = su[k];
+        wu[k + 1] = wu[k];
+        sb[k + 1] = sb[k];
+        for (int kind = 0; kind < kRcKinds; kind++) rk[kind * ns + k + 1] = rk[kind * ns + k];
+        if (status[k]) continue;
+        const Hdr& h = hd[k];
+        const DecodeTables& dt = c->db_dt[k];
+        off[k] = ntot;
+        ntot += (h.n + 63) / 64 * 64;
+        F.ndim = h.ndim;
+        F.a.n = h.n;
+        F.a.n_outlier = h.nout;
+        const uint8_t* d_huff = d_in[k] + meta_bytes(h.ndim);
+        F.a.outliers = d_huff + h.huff_size;
+        if (!dt.root_leaf) {
+            const uint64_t stream_bytes = h.huff_size - 8 - h.tb;
+            F.a.stream = d_huff + 8 + h.tb;
+            F.a.stream_bits = stream_bytes * 8;
+            F.a.nsub = std::max<uint64_t>(1, (F.a.stream_bits + kSubBits - 1) / kSubBits);
+            toff[k] = tab_tot;
+            tab_tot += t16 ? ((dt.lut16.size() + dt.sub16.size()) * 2 + 15) / 16 * 16
+                           : ((dt.lut.size() + dt.sub.size()) * 4 + 15) / 16 * 16;
+            F.s.jbase = (uint32_t)sb[k];
+            su[k + 1] = su[k] + decode_sync_units(F.a.nsub);
+            wu[k + 1] = wu[k] + decode_write_units(F.a.nsub);
+            sb[k + 1] = sb[k] + F.a.nsub;
+        } else if (dt.root_sym == 0 && h.nout < h.n) {
+            status[k] = VECSZ_ECORRUPT;  // every element an outlier, but fewer outliers than elements
+            continue;
+        }
+        ReconArgs& ra = F.ra;
+        ra.d0 = h.dims[0];
+        ra.d1 = h.dims[1];
+        ra.d2 = h.dims[2];
+        ra.n = h.n;
+        ra.radius = R;
+        ra.twoeb = eb_family(h.ebu, p[k].mode == VECSZ_MODE_R2R, p[k].value_range).twoeb;
+        ra.ntiles_x = (ra.d0 + 63) / 64;
+        ra.ntiles_y = h.ndim >= 2 ? (ra.d1 + B - 1) / B : 1;
+        ra.ntiles_z = h.ndim >= 3 ? (ra.d2 + B - 1) / B : 1;
+        ra.halo = nullptr;
+        ra.halo_stride = 0;
+        ra.hb0 = ref_nblk(ra.d0, B);
+        ra.hb1 = h.ndim >= 2 ? ref_nblk(ra.d1, B) : 1;
+        const bool x4 = ra.d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(d_out[k]) & 15) == 0 && (h.ndim < 3 || B <= 16);
+        F.kind = h.ndim == 1 ? (x4 ? kRc1x4 : kRc1) : h.ndim == 2 ? (x4 ? kRc2x4 : kRc2) : (x4 ? kRc3x4 : kRc3);
+        rk[F.kind * ns + k + 1] = rk[F.kind * ns + k] + recon_tiles(F.kind, ra);
+        fl_off[k] = fl_tot;
+        fl_tot += ((ra.d0 + B - 1) / B) * (h.ndim >= 2 ? (ra.d1 + B - 1) / B : 1) * (h.ndim >= 3 ? (ra.d2 + B - 1) / B : 1);
+        F.out = d_out[k];
+    }
+    nsub_tot = sb[nf];
+    if (nsub_tot > 0xFFFFFFF0ull) return fail(c, VECSZ_EINVAL, "batch too large");  // u32 list entries
+    const size_t st_bytes = st.size() * 8, fd_bytes = (size_t)nf * sizeof(DecField);
+    const uint64_t nsub_alloc = std::max<uint64_t>(nsub_tot, 1);
+    if (!c->codes.ensure(ntot * 2 + 64) || !c->sparse.ensure(ntot * 4 + 64) || !c->tabs.ensure(tab_tot + 64) ||
+        !c->sub_b.ensure(nsub_alloc * 8) || !c->sub_e.ensure(nsub_alloc * 8) || !c->sub_cnt.ensure(nsub_alloc * 4) ||
+        !c->sub_zeros.ensure(nsub_alloc * 4) || !c->sub_lj.ensure(nsub_alloc * 8) ||
+        !c->sub_ls.ensure(nsub_alloc * 16) || !c->sub_ln.ensure(16) || !c->sym_off.ensure(nsub_alloc * 8) ||
+        !c->zero_off.ensure(nsub_alloc * 8) || !c->scan_tmp.ensure(((nsub_alloc + 4095) / 4096 + 1) * 16) ||
+        !c->scan_tot.ensure(16) || !c->flagged.ensure(fl_tot * 4 + 64) || !c->db_nfl.ensure((size_t)nf * 4) ||
+        !c->db_sums.ensure((size_t)nf * sizeof(DecSummary)) || !c->db_tot.ensure((size_t)nf * 8) ||
+        !c->db_desc.ensure(fd_bytes) || !c->db_start.ensure(st_bytes))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    if (!c->db_pin.ensure(fd_bytes + st_bytes + tab_tot + 64) ||
+        !c->db_back.ensure((size_t)nf * (sizeof(DecSummary) + 8) + 64))
+        return fail(c, VECSZ_ENOMEM, "pinned buffer");
+    SubState S{};
+    S.b = c->sub_b.as<uint64_t>();
+    S.e = c->sub_e.as<uint64_t>();
+    S.cnt = c->sub_cnt.as<uint32_t>();
+    S.zer = c->sub_zeros.as<uint32_t>();
+    S.list_j[0] = c->sub_lj.as<uint32_t>();
+    S.list_j[1] = c->sub_lj.as<uint32_t>() + nsub_alloc;
+    S.list_start[0] = c->sub_ls.as<uint64_t>();
+    S.list_start[1] = c->sub_ls.as<uint64_t>() + nsub_alloc;
+    S.list_n = c->sub_ln.as<uint32_t>();
+    S.jbase = 0;
+    uint8_t* tab_h = c->db_pin.as<uint8_t>() + fd_bytes + st_bytes;
+    for (int k = 0; k < nf; k++) {
+        DecField& F = fd[k];
+        if (status[k]) continue;
+        const DecodeTables& dt = c->db_dt[k];
+        F.codes = c->codes.as<uint16_t>() + off[k];
+        F.sparse = c->sparse.as<float>() + off[k];
+        F.flagged = c->flagged.as<uint32_t>() + fl_off[k];
+        F.nflagged = c->db_nfl.as<uint32_t>() + k;
+        F.sum = c->db_sums.as<DecSummary>() + k;
+        const uint64_t s0 = sb[k];
+        F.s.b = S.b + s0;
+        F.s.e = S.e + s0;
+        F.s.cnt = S.cnt + s0;
+        F.s.zer = S.zer + s0;
+        for (int q = 0; q < 2; q++) {
+            F.s.list_j[q] = S.list_j[q];
+            F.s.list_start[q] = S.list_start[q];
+        }
+        F.s.list_n = S.list_n;
+        F.sym_off = c->sym_off.as<uint64_t>() + s0;
+        F.zero_off = c->zero_off.as<uint64_t>() + s0;
+        if (dt.root_leaf) continue;
+        uint8_t* tp = tab_h + toff[k];
+        uint8_t* tdev = c->tabs.as<uint8_t>() + toff[k];
+        if (t16) {
+            memcpy(tp, dt.lut16.data(), dt.lut16.size() * 2);
+            memcpy(tp + dt.lut16.size() * 2, dt.sub16.data(), dt.sub16.size() * 2);
+            F.a.lut16 = reinterpret_cast<const uint16_t*>(tdev);
+            F.a.sub16 = reinterpret_cast<const uint16_t*>(tdev) + dt.lut16.size();
+            F.a.n_sub16 = (uint32_t)dt.sub16.size();
+        } else {
+            memcpy(tp, dt.lut.data(), dt.lut.size() * 4);
+            memcpy(tp + dt.lut.size() * 4, dt.sub.data(), dt.sub.size() * 4);
+            F.a.lut = reinterpret_cast<const uint32_t*>(tdev);
+            F.a.sub = reinterpret_cast<const uint32_t*>(tdev) + dt.lut.size();
+        }
+    }
+    DecField* fdd = c->db_desc.as<DecField>();
+    uint64_t* std_ = c->db_start.as<uint64_t>();
+    const uint64_t* su_d = std_;
+    const uint64_t* wu_d = std_ + ns;
+    const uint64_t* sb_d = std_ + 2 * ns;
+    const uint64_t* rk_d = std_ + 3 * ns;
+    memcpy(c->db_pin.p, fd.data(), fd_bytes);
+    memcpy(c->db_pin.as<uint8_t>() + fd_bytes, st.data(), st_bytes);
+    HIPCHK(c, hipMemcpyAsync(c->db_desc.p, c->db_pin.p, fd_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->db_start.p, c->db_pin.as<uint8_t>() + fd_bytes, st_bytes, hipMemcpyHostToDevice,
+                             c->stream));
+    if (tab_tot) HIPCHK(c, hipMemcpyAsync(c->tabs.p, tab_h, tab_tot, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->db_sums.p, 0, (size_t)nf * sizeof(DecSummary), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->db_nfl.p, 0, (size_t)nf * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->sub_ln.p, 0, 16, c->stream));
+
+    // 4. decode: one-symbol trees by a fill; the others by the batch passes
+    {
+        StageScope s(c, "decode_sync");
+        for (int k = 0; k < nf; k++) {
+            if (status[k] || !c->db_dt[k].root_leaf) continue;
+            launch_fill_u16(fd[k].codes, hd[k].n, (uint16_t)c->db_dt[k].root_sym, c->stream);
+            if (c->db_dt[k].root_sym == 0) launch_scatter_all_outliers(fd[k].a.outliers, hd[k].n, fd[k].sparse, c->stream);
+        }
+        launch_decode_sync_multi(t16, max_sub16, fdd, su_d, nf, su[nf], c->stream);
+    }
+    DecSummary* sums_h = c->db_back.as<DecSummary>();
+    uint64_t* tot_h = reinterpret_cast<uint64_t*>(c->db_back.as<uint8_t>() + (size_t)nf * sizeof(DecSummary));
+    uint32_t* left_h = reinterpret_cast<uint32_t*>(tot_h + nf);
+    uint64_t max_nsub = 0;
+    for (int k = 0; k < nf; k++) max_nsub = std::max<uint64_t>(max_nsub, fd[k].a.nsub);
+    if (nsub_tot) {
+        constexpr uint32_t kPassesAhead = 2;
+        uint32_t pass = 0;
+        {
+            StageScope s(c, "decode_redo");
+            for (; pass < kPassesAhead; pass++) launch_decode_pass_multi(t16, fdd, sb_d, nf, S, pass, nsub_tot, c->stream);
+        }
+        HIPCHK(c, hipGetLastError());
+        uint32_t batch = 2;
+        for (;;) {
+            HIPCHK(c, hipMemcpyAsync(left_h, S.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (*left_h == 0) break;
+            if (pass > max_nsub + 2) return fail(c, VECSZ_ECORRUPT, "subsequence boundaries do not converge");
+            StageScope s(c, "decode_redo");
+            for (uint32_t b = 0; b < batch; b++) launch_decode_pass_multi(t16, fdd, sb_d, nf, S, pass++, nsub_tot, c->stream);
+            batch = std::min<uint32_t>(2 * batch, 16);
+        }
+        // fields whose stream does not decode are left out of the write pass
+        // and the reconstruction (their subsequence state is not trusted)
+        HIPCHK(c, hipMemcpyAsync(sums_h, c->db_sums.p, (size_t)nf * sizeof(DecSummary), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        bool changed = false;
+        for (int k = 0; k < nf; k++) {
+            if (status[k] || !sums_h[k].error) continue;
+            status[k] = VECSZ_ECORRUPT;
+            changed = true;
+        }
+        if (changed) {
+            for (int k = 0; k < nf; k++) {
+                wu[k + 1] = wu[k] + (status[k] ? 0 : decode_write_units(fd[k].a.nsub));
+                for (int kind = 0; kind < kRcKinds; kind++)
+                    rk[kind * ns + k + 1] = rk[kind * ns + k] + (!status[k] && fd[k].kind == kind ? recon_tiles(kind, fd[k].ra) : 0);
+                if (status[k]) fd[k].kind = -1;
+            }
+            memcpy(c->db_pin.p, fd.data(), fd_bytes);
+            memcpy(c->db_pin.as<uint8_t>() + fd_bytes, st.data(), st_bytes);
+            HIPCHK(c, hipMemcpyAsync(c->db_desc.p, c->db_pin.p, fd_bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->db_start.p, c->db_pin.as<uint8_t>() + fd_bytes, st_bytes,
+                                     hipMemcpyHostToDevice, c->stream));
+        }
+        {
+            StageScope s(c, "decode_scan");
+            launch_scan_u32x2(S.cnt, S.zer, nsub_tot, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
+                              c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream);
+        }
+        {
+            StageScope s(c, "decode_write");
+            launch_decode_write_multi(t16, max_sub16, fdd, wu_d, nf, wu[nf], c->stream);
+        }
+        launch_dec_field_totals(fdd, nf, c->db_tot.as<uint64_t>(), c->stream);
+        HIPCHK(c, hipGetLastError());
+    }
+    {
+        StageScope s(c, "reconstruct");
+        for (int kind = 0; kind < kRcKinds; kind++)
+            launch_reconstruct_multi(kind, B, fdd, rk_d + (size_t)kind * ns, nf, rk[kind * ns + nf], c->stream);
+        launch_reconstruct_fallback_multi(B, fdd, nf, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(sums_h, c->db_sums.p, (size_t)nf * sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(tot_h, c->db_tot.p, (size_t)nf * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    timing_end(c);
+    *reported = true;
+    for (int k = 0; k < nf; k++) {
+        if (status[k] || c->db_dt[k].root_leaf) continue;
+        if (sums_h[k].error || tot_h[k] < hd[k].n) status[k] = VECSZ_ECORRUPT;
+    }
+    for (int k = 0; k < nf; k++)
+        if (status[k]) {
+            const char* why = status[k] == VECSZ_ENOMEM   ? "output buffer too small"
+                              : status[k] == VECSZ_EINVAL ? "r2r decompress needs params.value_range"
+                                                          : "archive is corrupt";
+            return fail(c, status[k], "batch field " + std::to_string(k) + " failed: " + why);
+        }
+    return VECSZ_OK;
+}
+
+static int decompress_batch_fused(vecsz_ctx* c, int nf, const vecsz_params* p, const uint8_t* const* d_in,
+                                  const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status,
+                                  bool* fallback) {
+    for (int k = 0; k < nf; k++) status[k] = VECSZ_OK;
+    bool reported = false;
+    const int rc = decompress_batch_fused_run(c, nf, p, d_in, in_bytes, d_out, out_cap, status, &reported, fallback);
+    if (rc && !reported)
+        for (int k = 0; k < nf; k++)
+            if (status[k] == VECSZ_OK) status[k] = rc;
+    return rc;
+}
+
 extern "C" {
 
 int vecsz_ctx_set_workers(vecsz_ctx* c, int nworkers) {
@@ -1880,6 +2255,11 @@ int vecsz_decompress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, con
                            const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status) {
     if (!c || nfields < 0 || (nfields && (!p || !d_in || !in_bytes || !d_out || !out_cap || !status)))
         return fail(c, VECSZ_EINVAL, "NULL argument");
+    if (c->fused_batch && dbatch_fusable(c, nfields, p, d_in, d_out)) {
+        bool fallback = false;  // an archive with a pad section: the per-field path below
+        const int rc = decompress_batch_fused(c, nfields, p, d_in, in_bytes, d_out, out_cap, status, &fallback);
+        if (!fallback) return rc;
+    }
     std::vector<uint64_t> weight(nfields);
     for (int k = 0; k < nfields; k++) weight[k] = out_cap[k];
     const uint64_t nmax = nfields ? *std::max_element(weight.begin(), weight.end()) : 0;
@@ -1950,9 +2330,12 @@ struct vecsz_server {
     std::deque<uint64_t> queue;
     std::unordered_map<uint64_t, Request> reqs;
 
-    // Small compress requests at the head of the queue that can share one
-    // fused batch launch: no padding, the same block size and dictionary
-    // (at most 4096), at most kFoldElems elements each.
+    // Small requests at the head of the queue that can share one fused batch
+    // launch: all compress or all decompress, the same block size and
+    // dictionary (at most 4096), at most kFoldElems elements each (a
+    // decompress request counts its output capacity); compress requests
+    // without padding.  A decompress archive with a pad section sends its
+    // folded batch to single calls.
     static constexpr uint64_t kFoldElems = 4ull << 20;   // 16 MiB of f32 per request
     static constexpr uint64_t kFoldTotal = 32ull << 20;  // elements of one folded batch
     static constexpr int kFoldMax = 64;
@@ -1961,8 +2344,10 @@ struct vecsz_server {
         uint64_t n = 0;
         return field_elems(p.ndim, p.dims, &n) ? n : 0;
     }
+    static uint64_t fold_elems(const Request& r) { return r.kind == 0 ? elems(r.p) : r.out_cap; }
     static bool foldable(const Request& r) {
-        return r.kind == 0 && r.p.pad_mode == VECSZ_PAD_NONE && r.p.dict_size <= kMaxLdsBook && elems(r.p) <= kFoldElems;
+        if (r.p.dict_size > kMaxLdsBook || fold_elems(r) > kFoldElems) return false;
+        return r.kind == 1 || r.p.pad_mode == VECSZ_PAD_NONE;
     }
 
     void slot_loop(vecsz_ctx* c) {
@@ -1984,16 +2369,17 @@ struct vecsz_server {
                 in_use += q.need;
                 q.t_start = now_ms();
                 r = q;
-                // continuous batching: compress requests waiting right behind
-                // it join its launch while they fit the budget
+                // continuous batching: requests of the same kind waiting right
+                // behind it join its launch while they fit the budget
                 if (foldable(r)) {
-                    uint64_t total = elems(r.p);
+                    uint64_t total = fold_elems(r);
                     while (!queue.empty() && (int)fold.size() + 1 < kFoldMax) {
                         Request& n = reqs[queue.front()];
-                        if (!foldable(n) || n.p.block != r.p.block || n.p.dict_size != r.p.dict_size ||
-                            total + elems(n.p) > kFoldTotal || n.need > budget - in_use)
+                        if (!foldable(n) || n.kind != r.kind || n.p.block != r.p.block ||
+                            n.p.dict_size != r.p.dict_size || total + fold_elems(n) > kFoldTotal ||
+                            n.need > budget - in_use)
                             break;
-                        total += elems(n.p);
+                        total += fold_elems(n);
                         in_use += n.need;
                         n.t_start = q.t_start;
                         fold.push_back(queue.front());
@@ -2030,13 +2416,13 @@ struct vecsz_server {
     }
 
     // Runs request t (copied in r) and the folded requests as one fused
-    // compress batch; each keeps its own status and sizes.
+    // batch; each keeps its own status and sizes.
     void run_folded(vecsz_ctx* c, uint64_t t, const Request& r, const std::vector<uint64_t>& fold) {
         const int nf = 1 + (int)fold.size();
         std::vector<vecsz_params> ps(nf);
         std::vector<const float*> ins(nf);
         std::vector<uint8_t*> outs(nf);
-        std::vector<uint64_t> caps(nf), sizes(nf, 0), nouts(nf, 0);
+        std::vector<uint64_t> caps(nf), sizes(nf, 0), nouts(nf, 0), in_bytes(nf, 0);
         std::vector<int> st(nf, 0);
         {
             std::lock_guard<std::mutex> lk(m);
@@ -2046,9 +2432,12 @@ struct vecsz_server {
                 ins[k] = static_cast<const float*>(q.in);
                 outs[k] = static_cast<uint8_t*>(q.out);
                 caps[k] = q.out_cap;
+                in_bytes[k] = q.in_bytes;
             }
         }
-        if (batch_fusable(c, nf, ps.data(), ins.data(), outs.data())) {
+        if (r.kind == 1) {
+            run_folded_decompress(c, nf, ps, ins, in_bytes, outs, caps, st);
+        } else if (batch_fusable(c, nf, ps.data(), ins.data(), outs.data())) {
             // every request gets a status: the fused call gives a failure of
             // the whole launch to each field not already failed on its own
             const int rc = compress_batch_fused(c, nf, ps.data(), ins.data(), outs.data(), caps.data(), sizes.data(),
@@ -2059,6 +2448,37 @@ struct vecsz_server {
             for (int k = 0; k < nf; k++)
                 st[k] = compress_device(c, &ps[k], ins[k], outs[k], caps[k], &sizes[k], &nouts[k]);
         }
+        finish_folded(t, fold, st, sizes, nouts);
+    }
+
+    void run_folded_decompress(vecsz_ctx* c, int nf, const std::vector<vecsz_params>& ps,
+                               const std::vector<const float*>& ins_f, const std::vector<uint64_t>& in_bytes,
+                               const std::vector<uint8_t*>& outs_u8, const std::vector<uint64_t>& caps,
+                               std::vector<int>& st) {
+        std::vector<const uint8_t*> ins(nf);
+        std::vector<float*> outs(nf);
+        for (int k = 0; k < nf; k++) {
+            ins[k] = reinterpret_cast<const uint8_t*>(ins_f[k]);
+            outs[k] = reinterpret_cast<float*>(outs_u8[k]);
+        }
+        bool fallback = true;
+        if (dbatch_fusable(c, nf, ps.data(), ins.data(), outs.data())) {
+            fallback = false;
+            const int rc = decompress_batch_fused(c, nf, ps.data(), ins.data(), in_bytes.data(), outs.data(),
+                                                  caps.data(), st.data(), &fallback);
+            if (!fallback && rc && std::all_of(st.begin(), st.end(), [](int v) { return v == VECSZ_OK; }))
+                std::fill(st.begin(), st.end(), rc);
+        }
+        if (fallback)  // invalid parameters or a pad section: each alone, for its own status
+            for (int k = 0; k < nf; k++) {
+                st[k] = decompress_device(c, &ps[k], ins[k], in_bytes[k], outs[k], caps[k]);
+                if (!st[k] && hipStreamSynchronize(c->stream) != hipSuccess) st[k] = VECSZ_EHIP;
+            }
+    }
+
+    void finish_folded(uint64_t t, const std::vector<uint64_t>& fold, const std::vector<int>& st,
+                       const std::vector<uint64_t>& sizes, const std::vector<uint64_t>& nouts) {
+        const int nf = 1 + (int)fold.size();
         {
             std::lock_guard<std::mutex> lk(m);
             const double te = now_ms();

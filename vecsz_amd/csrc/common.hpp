@@ -123,6 +123,21 @@ __host__ __device__ inline IdxLayout idx_layout(uint64_t n) {
     return L;
 }
 
+// Field of work unit u in a batch: the f with start[f] <= u < start[f + 1]
+// (start is nondecreasing, start[0] = 0, start[nf] = total units > u).
+// Where u is uniform across the workgroup the search runs on scalar loads.
+__device__ __forceinline__ int field_of(const uint64_t* __restrict__ start, int nf, uint64_t u) {
+    int lo = 0, hi = nf;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= u)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Decoder summary.
 struct DecSummary {
     uint32_t error;       // nonzero: corrupt stream

@@ -123,6 +123,10 @@ struct SubState {
     uint32_t* list_j[2];
     uint64_t* list_start[2];
     uint32_t* list_n;
+    // List entries hold jbase + j: 0 for one field; in a batch the arrays
+    // above are views of one field's part of batch-wide arrays, and jbase is
+    // the batch-wide number of its subsequence 0 (the lists are batch-wide).
+    uint32_t jbase = 0;
 };
 
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
@@ -163,6 +167,54 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
                         uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st);
 void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
                                  const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st);
+
+// Batched decompress: one field as the batch kernels see it.  The
+// subsequence state, the code-count prefixes and the re-decode lists are
+// batch-wide arrays (a field's subsequences are numbered from sub0); s,
+// sym_off and zero_off are views of its part.
+enum ReconKind { kRc1 = 0, kRc1x4, kRc2, kRc2x4, kRc3, kRc3x4, kRcKinds };
+struct DecField {
+    DecodeArgs a;              // nsub = 0: nothing to decode on the device
+    SubState s;
+    const uint64_t* sym_off;
+    const uint64_t* zero_off;
+    uint16_t* codes;
+    float* sparse;
+    float* out;
+    ReconArgs ra;
+    uint32_t* flagged;         // blocks for the fp32 fallback
+    uint32_t* nflagged;
+    DecSummary* sum;
+    int ndim;
+    int kind;                  // ReconKind; -1: not reconstructed
+};
+// t16: every field has compact tables; max_sub16: the largest n_sub16.
+// start: per-field unit starts (nf + 1), nunits = start[nf].
+void launch_decode_sync_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
+                              uint64_t nunits, hipStream_t st);
+// sub_start: batch-wide number of each field's subsequence 0 (nf + 1 entries);
+// S: the batch-wide lists (list_j, list_start, list_n).
+void launch_decode_pass_multi(bool t16, const DecField* fd, const uint64_t* sub_start, int nf, SubState S,
+                              uint32_t pass, uint64_t nsub_tot, hipStream_t st);
+void launch_decode_write_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
+                               uint64_t nunits, hipStream_t st);
+// Symbols each field's subsequences hold (tot[f]).
+void launch_dec_field_totals(const DecField* fd, int nf, uint64_t* tot, hipStream_t st);
+// One reconstruction form over its fields' tiles (start: per-field tile starts).
+void launch_reconstruct_multi(int kind, int B, const DecField* fd, const uint64_t* start, int nf, uint64_t ntiles,
+                              hipStream_t st);
+void launch_reconstruct_fallback_multi(int B, const DecField* fd, int nf, hipStream_t st);
+// Tiles a field of form `kind` has for the reconstruction kernels.
+uint64_t recon_tiles(int kind, const ReconArgs& a);
+// Workgroups of the sync and write passes for a field of nsub subsequences.
+uint64_t decode_sync_units(uint64_t nsub);
+uint64_t decode_write_units(uint64_t nsub);
+// Copies the first len bytes of each archive to dst + off (one workgroup each).
+struct HeadCopy {
+    const uint8_t* src;
+    uint64_t len, off;
+};
+void launch_gather_heads(const HeadCopy* hc, int n, uint8_t* dst, hipStream_t st);
 
 // Padding (kernels_pad.hip)
 struct PadArgs {

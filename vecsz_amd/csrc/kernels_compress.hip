@@ -1310,7 +1310,7 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
 //
 // k_encode does the work of k_chunk_bits, the chunk scan, k_enc_summary and
 // k_pack in one pass, so the field's codes are read from HBM once (a
-// partition's codes are read again for the packing, from the caches).  The
+// wavefront keeps its chunks' codes in registers until it packs them).  The
 // chunks are cut into partitions of kEncPartChunks (kEncWaveChunks per
 // wavefront, four wavefronts per workgroup).  A workgroup takes the next
 // partition from a counter, sums every chunk's
@@ -1475,14 +1475,17 @@ __global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ cod
         if (part >= nparts) break;
         const uint64_t kw = part * (4 * WC) + (uint64_t)wave * WC;
 
-        // pass 1: bits and outliers of the wavefront's chunks (their codes are
-        // read again by pass 2, from the caches)
+        // pass 1: bits and outliers of the wavefront's chunks, whose codes stay
+        // in registers for pass 2 (a partition's codes do not survive the
+        // look-back in the L2, which other workgroups' streams flush)
         uint32_t ctot[WC];  // chunk bits (low 17) and outliers (high 15), uniform
+        uint32_t keep[WC][8];
         uint32_t wb = 0, wz = 0;
-#pragma unroll 2
+#pragma unroll
+        for (int i = 0; i < WC; i++) enc_load_raw(codes, n, kw + i, lane, fill, keep[i]);
+#pragma unroll
         for (int i = 0; i < WC; i++) {
-            uint32_t raw[8];
-            enc_load_raw(codes, n, kw + i, lane, fill, raw);
+            const uint32_t* raw = keep[i];
             uint32_t tbits = 0, tz = 0;
 #pragma unroll
             for (int j = 0; j < kSymPerThread; j++) {
@@ -1549,7 +1552,8 @@ __global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ cod
             const uint64_t k = kw + i;
             if (k >= nchunks) break;
             ChunkRaw cur;
-            enc_load_raw(codes, n, k, lane, fill, cur.w);
+#pragma unroll
+            for (int j = 0; j < 8; j++) cur.w[j] = keep[0][j];
             const uint64_t base = k * kChunk + lane * kSymPerThread;
             cur.nvalid = base >= n ? 0u : (uint32_t)min<uint64_t>(kSymPerThread, n - base);
             const uint64_t nx = (k + 1) * kChunk + lane;
@@ -1561,9 +1565,14 @@ __global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ cod
                              index, L, gbits, gouts);
             cb += ctot[0] & 0x1FFFF;
             cz += ctot[0] >> 17;
-            // the next chunk's totals move to the front (registers, not an indexed array)
+            // the next chunk's totals and codes move to the front (registers,
+            // not an indexed array)
 #pragma unroll
-            for (int q = 0; q + 1 < WC; q++) ctot[q] = ctot[q + 1];
+            for (int q = 0; q + 1 < WC; q++) {
+                ctot[q] = ctot[q + 1];
+#pragma unroll
+                for (int j = 0; j < 8; j++) keep[q][j] = keep[q + 1][j];
+            }
         }
     }
 }
@@ -1595,21 +1604,6 @@ __global__ void __launch_bounds__(256) k_enc_finish(const uint64_t* __restrict__
 }
 
 // ------------------------------------------------------------------ batched fields
-
-// Field of work unit u: the f with start[f] <= u < start[f + 1] (start is
-// nondecreasing, start[0] = 0, start[nf] = total units > u).  u is uniform
-// across the workgroup, so the search runs on scalar loads.
-__device__ __forceinline__ int field_of(const uint64_t* __restrict__ start, int nf, uint64_t u) {
-    int lo = 0, hi = nf;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (start[mid] <= u)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
 
 // Min/max words of every field whose range is found on the device.
 __global__ void k_range_init_multi(const BatchField* __restrict__ fd, int nf) {

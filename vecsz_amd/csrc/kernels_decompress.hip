@@ -287,8 +287,10 @@ __device__ __forceinline__ uint32_t ckpt(uint32_t off, uint32_t cnt, uint32_t z)
 // Dynamic LDS words of the sync kernel after the tables.
 constexpr uint32_t kSyncLdsWords = kStageLds + kSyncWg * kCkStride + 3 * kSyncWg;
 
+// Workgroup `unit` of the sync pass: subsequences unit * kSyncStride on.
 template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, DecSummary* sum) {
+__device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubState& s, DecSummary* sum,
+                                                 uint64_t unit) {
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     uint32_t* st = lds + dec.load(lds, a);
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
     uint32_t* own_e = own_z + kSyncWg;
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
-    const uint64_t J0 = blockIdx.x * (uint64_t)kSyncStride;
+    const uint64_t J0 = unit * kSyncStride;
     stage_words<kStageWords>(st, sv, J0 * (kSubBits / 32));
     __syncthreads();
     const uint32_t t = threadIdx.x;
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
             // its end with the own end that j + 2's continuation started from
             s.e[j + 1] = base + own_e[u];
             const uint32_t k = atomicAdd(&s.list_n[0], 1u);
-            s.list_j[0][k] = (uint32_t)(j + 1);
+            s.list_j[0][k] = s.jbase + (uint32_t)(j + 1);
             s.list_start[0][k] = base + q0;
         } else if (met) {
             s.cnt[j + 1] = cc + own_cnt[u] - before_c;
@@ -399,12 +401,17 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
             s.e[j + 1] = base + q;
             if (j + 2 < a.nsub) {
                 const uint32_t k = atomicAdd(&s.list_n[0], 1u);
-                s.list_j[0][k] = (uint32_t)(j + 2);
+                s.list_j[0][k] = s.jbase + (uint32_t)(j + 2);
                 s.list_start[0][k] = base + q;
             }
         }
     }
     if (bad) atomicOr(&sum->error, 1u);
+}
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, DecSummary* sum) {
+    decode_sync_unit<T16>(a, s, sum, blockIdx.x);
 }
 
 // Re-decode pass: each listed subsequence j is decoded in full from its
@@ -828,24 +835,28 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
 // a predecessor).  Only the field's last unit may be partial.
 constexpr uint32_t kWriteStageWords = kSyncWg * (kSubBits / 32) + 32;  // + up to 7 codewords past the last range
 constexpr uint32_t kWriteStageLds = kWriteStageWords + kWriteStageWords / 32 + 3;
+// sym_off / zero_off: exclusive prefixes of the subsequences' codeword and
+// zero counts, counted from the field's subsequence 0 (entry 0 is the base
+// subtracted: 0 for one field, the earlier fields' totals in a batch).
 template <bool T16>
-__global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, const uint64_t* __restrict__ sym_off,
-                                                      const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
-                                                      float* __restrict__ sparse, DecSummary* sum) {
+__device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const SubState& s,
+                                                  const uint64_t* __restrict__ sym_off,
+                                                  const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
+                                                  float* __restrict__ sparse, DecSummary* sum, uint64_t unit) {
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     uint32_t* st = lds + dec.load(lds, a);
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
-    const uint64_t J0 = blockIdx.x * (uint64_t)kSyncWg;
+    const uint64_t J0 = unit * kSyncWg;
     stage_words<kWriteStageWords>(st, sv, J0 * (kSubBits / 32));
     __syncthreads();
     const uint64_t j = J0 + threadIdx.x;
     if (j >= a.nsub) return;
     uint32_t lp = (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0;
-    const uint64_t O = sym_off[j];
+    const uint64_t O = sym_off[j] - sym_off[0];
     const uint64_t E = min(O + s.cnt[j], a.n);
-    uint64_t rank = zero_off[j];
+    uint64_t rank = zero_off[j] - zero_off[0];
     bool bad = false;
     // the loops below count symbols, so an invalid codeword (len 0, an
     // error) needs no forced progress
@@ -869,23 +880,44 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
         rank += next() == 0;
     if (W0 < E) {
         const uint64_t W1 = min((E + 7) & ~7ull, a.n);
-        for (; i + 8 <= W1; i += 8) {
-            // outliers are rare: one test per unit, then in symbol order
-            uint32_t w[4], omask = 0;
+        const uint64_t Wf = W0 + ((W1 - W0) & ~7ull);  // end of the whole units
+        // Whole units go out in groups of four (64 bytes) held in registers
+        // and stored back to back: the L2 then sees long runs of each lane's
+        // output, where units stored as they were decoded left partially
+        // written lines that were written back several times.  Groups start
+        // at the lane's first unit, so only its last group is partial.
+        for (uint64_t G = W0; G < Wf; G += 32) {
+            // outliers are rare: one test per group, then in symbol order
+            uint32_t w[16], omask = 0;
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const uint32_t lo = next();
-                const uint32_t hi = next();
-                omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (2 * m);
-                w[m] = lo | (hi << 16);
+            for (int u = 0; u < 4; u++) {
+                const uint64_t iu = G + 8 * u;
+                const bool mine = iu < Wf;  // differs across lanes only in a last group
+#pragma unroll
+                for (int m = 0; m < 4; m++) w[4 * u + m] = 0;
+                if (mine) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const uint32_t lo = next();
+                        const uint32_t hi = next();
+                        omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (8 * u + 2 * m);
+                        w[4 * u + m] = lo | (hi << 16);
+                    }
+                }
             }
-            *reinterpret_cast<uint4*>(codes + i) = make_uint4(w[0], w[1], w[2], w[3]);
-            if (omask) {
 #pragma unroll
-                for (int k = 0; k < 8; k++)
-                    if ((omask >> k) & 1) outlier(i + k);
+            for (int u = 0; u < 4; u++) {
+                const uint64_t iu = G + 8 * u;
+                if (iu < Wf)
+                    *reinterpret_cast<uint4*>(codes + iu) =
+                        make_uint4(w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]);
+            }
+            if (omask) {
+                for (int k = 0; k < 32; k++)
+                    if ((omask >> k) & 1) outlier(G + k);
             }
         }
+        i = Wf;
         for (; i < W1; i++) {  // the field's last, partial unit
             const uint32_t sym = next();
             codes[i] = (uint16_t)sym;
@@ -893,6 +925,13 @@ __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, 
         }
     }
     if (bad) atomicOr(&sum->error, 1u);
+}
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, const uint64_t* __restrict__ sym_off,
+                                                      const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
+                                                      float* __restrict__ sparse, DecSummary* sum) {
+    decode_write_unit<T16>(a, s, sym_off, zero_off, codes, sparse, sum, blockIdx.x);
 }
 
 __global__ void k_fill_u16(uint16_t* p, uint64_t n, uint16_t v) {
@@ -974,17 +1013,14 @@ __device__ __forceinline__ IHalo load_ihalo(const ReconArgs& a, bool xin, uint64
 // the first row / plane of the block and a start of h.x - h.y in the block's
 // first row of its first plane (the only row where X - XY - XZ + XYZ of the
 // halo does not cancel).
-template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
-                                                  float* __restrict__ out, ReconArgs a, uint32_t* flagged,
+template <int B, bool PAD>
+__device__ __forceinline__ void recon_3d_tile(uint64_t t, const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, const ReconArgs& a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
     const uint32_t lane = lane_id2();
-    const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
     const uint64_t nbx = (a.d0 + B - 1) / B, nby = (a.d1 + B - 1) / B;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    {
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = (t / a.ntiles_x) % a.ntiles_y;
         const uint64_t tz = t / (a.ntiles_x * a.ntiles_y);
@@ -1039,18 +1075,25 @@ __global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ c
     }
 }
 
-// PAD: r = s - N runs along x from 0 at the block start (W - NW of the
-// x-face halo cancels), with N = h.y in the block's first row.
 template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+__global__ void __launch_bounds__(256) k_recon_3d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
-    const uint32_t lane = lane_id2();
-    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t t = wave; t < ntiles; t += nwaves) recon_3d_tile<B, PAD>(t, codes, sparse, out, a, flagged, nflagged);
+}
+
+// PAD: r = s - N runs along x from 0 at the block start (W - NW of the
+// x-face halo cancels), with N = h.y in the block's first row.
+template <int B, bool PAD>
+__device__ __forceinline__ void recon_2d_tile(uint64_t t, const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, const ReconArgs& a, uint32_t* flagged,
+                                                  uint32_t* nflagged) {
+    const uint32_t lane = lane_id2();
     const uint64_t nbx = (a.d0 + B - 1) / B, nby = (a.d1 + B - 1) / B;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    {
         const uint64_t tx = t % a.ntiles_x;
         const uint64_t ty = t / a.ntiles_x;
         const uint64_t x = tx * 64 + lane;
@@ -1092,18 +1135,25 @@ __global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ c
     }
 }
 
-// PAD: the running sum starts from the block's pad value (a raw, usually
-// non-integral value for 1D block / edge padding: such blocks are listed).
 template <int B, bool PAD = false>
-__global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+__global__ void __launch_bounds__(256) k_recon_2d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
-    const uint32_t lane = lane_id2();
-    const uint64_t ntiles = a.ntiles_x;
+    const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t t = wave; t < ntiles; t += nwaves) recon_2d_tile<B, PAD>(t, codes, sparse, out, a, flagged, nflagged);
+}
+
+// PAD: the running sum starts from the block's pad value (a raw, usually
+// non-integral value for 1D block / edge padding: such blocks are listed).
+template <int B, bool PAD>
+__device__ __forceinline__ void recon_1d_tile(uint64_t t, const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, const ReconArgs& a, uint32_t* flagged,
+                                                  uint32_t* nflagged) {
+    const uint32_t lane = lane_id2();
     const uint64_t nbx = (a.d0 + B - 1) / B;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    {
         const uint64_t x = t * 64 + lane;
         const bool xin = x < a.d0;
         const bool bstart = (lane % B) == 0;
@@ -1132,18 +1182,29 @@ __global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ c
     }
 }
 
+template <int B, bool PAD = false>
+__global__ void __launch_bounds__(256) k_recon_1d(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, ReconArgs a, uint32_t* flagged,
+                                                  uint32_t* nflagged) {
+    const uint64_t ntiles = a.ntiles_x;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t t = wave; t < ntiles; t += nwaves) recon_1d_tile<B, PAD>(t, codes, sparse, out, a, flagged, nflagged);
+}
+
 // One thread per listed block: the reference's sequential fp32 recurrence in
 // its own association, using the block's region of `out` for the s values
 // before scaling them in place.
-__global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const uint16_t* __restrict__ codes,
-                                                       const float* __restrict__ sparse, float* __restrict__ out,
-                                                       ReconArgs a, const uint32_t* __restrict__ flagged,
-                                                       const uint32_t* __restrict__ nflagged) {
-    const uint32_t nf = *nflagged;
+// Listed blocks i0, i0 + di, ... of one field.
+__device__ __forceinline__ void recon_fallback_blocks(int ndim, int B, const uint16_t* __restrict__ codes,
+                                                      const float* __restrict__ sparse, float* __restrict__ out,
+                                                      const ReconArgs& a, const uint32_t* __restrict__ flagged,
+                                                      const uint32_t* __restrict__ nflagged, uint64_t i0, uint64_t di) {
+    const uint32_t nl = *nflagged;
     const uint64_t nbx = (a.d0 + B - 1) / B, nby = (a.d1 + B - 1) / B;
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
-    for (uint64_t f = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; f < nf; f += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t id = flagged[f];
+    for (uint64_t i = i0; i < nl; i += di) {
+        const uint64_t id = flagged[i];
         const uint64_t bx = id % nbx, by = (id / nbx) % nby, bz = id / (nbx * nby);
         const uint64_t x0 = bx * B, y0 = by * B, z0 = bz * B;
         // halo cells: z face, else y face, else x face in 3D; x face, else y in 2D
@@ -1196,6 +1257,14 @@ __global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const ui
     }
 }
 
+__global__ void __launch_bounds__(64) k_recon_fallback(int ndim, int B, const uint16_t* __restrict__ codes,
+                                                       const float* __restrict__ sparse, float* __restrict__ out,
+                                                       ReconArgs a, const uint32_t* __restrict__ flagged,
+                                                       const uint32_t* __restrict__ nflagged) {
+    recon_fallback_blocks(ndim, B, codes, sparse, out, a, flagged, nflagged,
+                          blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
+}
+
 // Four consecutive x per lane (8-byte code loads, 16-byte stores).  A block
 // row of B elements spans B/4 lanes: each lane runs the Lorenzo prefix over
 // its four elements sequentially, then a segmented scan over the lanes of the
@@ -1232,22 +1301,19 @@ __device__ __forceinline__ void row_scan4(int32_t (&D)[4], const int32_t (&g)[4]
 }
 
 template <int ND, int B>
-__global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
-                                                  float* __restrict__ out, ReconArgs a, uint32_t* flagged,
+__device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, const ReconArgs& a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
     constexpr int L = B / 4;
     constexpr int YB = ND >= 2 ? B : 1;
     constexpr int ZB = ND >= 3 ? B : 1;
     const uint32_t lane = lane_id2();
     const uint64_t ntx = (a.d0 + 255) / 256;
-    const uint64_t nty = ND >= 2 ? a.ntiles_y : 1, ntz = ND >= 3 ? a.ntiles_z : 1;
-    const uint64_t ntiles = ntx * nty * ntz;
-    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
-    const uint64_t nwaves = gridDim.x * 4ull;
+    const uint64_t nty = ND >= 2 ? a.ntiles_y : 1;
     const uint64_t sy = a.d0, sz = a.d0 * a.d1;
     const uint64_t nbx = (a.d0 + B - 1) / B, nby = ND >= 2 ? (a.d1 + B - 1) / B : 1;
     const uint32_t R = (uint32_t)a.radius;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    {
         const uint64_t tx = t % ntx;
         const uint64_t ty = (t / ntx) % nty;
         const uint64_t tz = t / (ntx * nty);
@@ -1311,6 +1377,188 @@ __global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ c
         }
         flag_blocks(bad, L, tx * (256 / B), ty, tz, nbx, nby, flagged, nflagged);
     }
+}
+
+template <int ND, int B>
+__global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
+                                                  float* __restrict__ out, ReconArgs a, uint32_t* flagged,
+                                                  uint32_t* nflagged) {
+    const uint64_t ntx = (a.d0 + 255) / 256;
+    const uint64_t nty = ND >= 2 ? a.ntiles_y : 1, ntz = ND >= 3 ? a.ntiles_z : 1;
+    const uint64_t ntiles = ntx * nty * ntz;
+    const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t t = wave; t < ntiles; t += nwaves) recon_x4_tile<ND, B>(t, codes, sparse, out, a, flagged, nflagged);
+}
+
+// ------------------------------------------------------------------ batched fields
+//
+// A batch decompress runs every stage once for all of its fields: each
+// launch walks a queue of units that spans the fields (unit u belongs to
+// field f when start[f] <= u < start[f + 1]) and runs the single-field body
+// on the field's descriptor.  A unit is uniform across its workgroup (sync,
+// write) or its wavefront (reconstruction), so the field lookup and the
+// descriptor reads are scalar.
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_sync_multi(const DecField* __restrict__ fd,
+                                                           const uint64_t* __restrict__ start, int nf) {
+    const uint64_t u = blockIdx.x;
+    const int f = field_of(start, nf, u);
+    const DecodeArgs a = fd[f].a;
+    const SubState s = fd[f].s;
+    decode_sync_unit<T16>(a, s, fd[f].sum, u - start[f]);
+}
+
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_write_multi(const DecField* __restrict__ fd,
+                                                            const uint64_t* __restrict__ start, int nf) {
+    const uint64_t u = blockIdx.x;
+    const int f = field_of(start, nf, u);
+    const DecodeArgs a = fd[f].a;
+    const SubState s = fd[f].s;
+    decode_write_unit<T16>(a, s, fd[f].sym_off, fd[f].zero_off, fd[f].codes, fd[f].sparse, fd[f].sum, u - start[f]);
+}
+
+// The decode tables of a field read from global memory (entries of one
+// re-decode batch may belong to different fields).
+template <bool T16>
+__device__ __forceinline__ Decoder<T16> global_decoder(const DecodeArgs& a);
+template <>
+__device__ __forceinline__ Decoder<true> global_decoder<true>(const DecodeArgs& a) {
+    Decoder<true> d;
+    d.lut = a.lut16;
+    d.sub = a.sub16;
+    return d;
+}
+template <>
+__device__ __forceinline__ Decoder<false> global_decoder<false>(const DecodeArgs& a) {
+    Decoder<false> d;
+    d.lut = a.lut;
+    d.sub = a.sub;
+    return d;
+}
+
+// Re-decode pass over the batch-wide list: as k_decode_pass, but entry J
+// (a batch-wide subsequence number) belongs to field field_of(sub_start, J),
+// whose stream its row is staged from and whose tables decode it.
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __restrict__ fd,
+                                                           const uint64_t* __restrict__ sub_start, int nf, SubState S,
+                                                           uint32_t pass) {
+    const uint32_t m = S.list_n[pass % 3];
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.list_n[(pass + 2) % 3] = 0;  // the list after next
+    if ((uint64_t)blockIdx.x * 256 >= m) return;                            // whole workgroup
+    __shared__ uint32_t rows[256 * kRowStride];
+    __shared__ int sfield[256];
+    const uint32_t* lj = S.list_j[pass & 1];
+    const uint64_t* ls = S.list_start[pass & 1];
+    uint32_t* oj = S.list_j[(pass + 1) & 1];
+    uint64_t* os = S.list_start[(pass + 1) & 1];
+    uint32_t* on = &S.list_n[(pass + 1) % 3];
+    for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256) {
+        const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
+        __syncthreads();  // the previous rows' readers
+        if (threadIdx.x < ne) sfield[threadIdx.x] = field_of(sub_start, nf, lj[e0 + threadIdx.x]);
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
+            const uint32_t e = p / kRowWords, w = p % kRowWords;
+            const int f = sfield[e];
+            const uint64_t j = lj[e0 + e] - sub_start[f];
+            StreamView sv;
+            sv.setup(fd[f].a.stream, fd[f].a.stream_bits);
+            rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
+        }
+        __syncthreads();
+        if (threadIdx.x < ne) {
+            const int f = sfield[threadIdx.x];
+            const DecodeArgs& a = fd[f].a;
+            const SubState& s = fd[f].s;
+            const Decoder<T16> dec = global_decoder<T16>(a);
+            StreamView sv;
+            sv.setup(a.stream, a.stream_bits);
+            const uint32_t J = lj[e0 + threadIdx.x];
+            const uint64_t j = J - sub_start[f];
+            const uint64_t start = ls[e0 + threadIdx.x];
+            const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
+            const RowWin win{rows + threadIdx.x * kRowStride};
+            const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
+            const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
+            uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
+            uint32_t cnt = 0, z = 0;
+            bool bad = false;
+            while (lp < tl) {
+                uint32_t len;
+                const uint32_t sym = dec.take_w(win, lp, len);
+                bad |= len == 0;
+                lp += len ? len : 1;
+                cnt++;
+                z += sym == 0;
+            }
+            const uint64_t pos = rowbit + lp - sv.shift0;
+            s.b[j] = start;
+            s.cnt[j] = cnt;
+            s.zer[j] = z;
+            if (pos != s.e[j]) {
+                s.e[j] = pos;
+                if (j + 1 < a.nsub) {
+                    const uint32_t k = atomicAdd(on, 1u);
+                    oj[k] = J + 1;
+                    os[k] = pos;
+                }
+            }
+            if (bad) atomicOr(&fd[f].sum->error, 1u);
+        }
+    }
+}
+
+__global__ void k_dec_field_totals(const DecField* __restrict__ fd, int nf, uint64_t* __restrict__ tot) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const uint64_t ns = fd[f].a.nsub;
+    tot[f] = ns ? fd[f].sym_off[ns - 1] + fd[f].s.cnt[ns - 1] - fd[f].sym_off[0] : 0;
+}
+
+template <int KIND, int B>
+__global__ void __launch_bounds__(256) k_recon_multi(const DecField* __restrict__ fd,
+                                                     const uint64_t* __restrict__ start, int nf, uint64_t ntot) {
+    const uint64_t wave = blockIdx.x * 4ull + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint64_t nwaves = gridDim.x * 4ull;
+    for (uint64_t T = wave; T < ntot; T += nwaves) {
+        const int f = field_of(start, nf, T);
+        const uint64_t t = T - start[f];
+        const ReconArgs ra = fd[f].ra;
+        const uint16_t* codes = fd[f].codes;
+        const float* sparse = fd[f].sparse;
+        float* out = fd[f].out;
+        uint32_t* fl = fd[f].flagged;
+        uint32_t* nfl = fd[f].nflagged;
+        if constexpr (KIND == kRc1x4)
+            recon_x4_tile<1, B>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc2x4)
+            recon_x4_tile<2, B>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc3x4)
+            recon_x4_tile<3, B>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc3)
+            recon_3d_tile<B, false>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc2)
+            recon_2d_tile<B, false>(t, codes, sparse, out, ra, fl, nfl);
+        else
+            recon_1d_tile<B, false>(t, codes, sparse, out, ra, fl, nfl);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gather_heads(const HeadCopy* __restrict__ hc, uint8_t* __restrict__ dst) {
+    const HeadCopy h = hc[blockIdx.x];
+    for (uint64_t i = threadIdx.x; i < h.len; i += blockDim.x) dst[h.off + i] = h.src[i];
+}
+
+// blockIdx.y: the field.
+__global__ void __launch_bounds__(64) k_recon_fallback_multi(int B, const DecField* __restrict__ fd) {
+    const DecField& F = fd[blockIdx.y];
+    if (F.kind < 0) return;
+    recon_fallback_blocks(F.ndim, B, F.codes, F.sparse, F.out, F.ra, F.flagged, F.nflagged,
+                          blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1488,6 +1736,92 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
 void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
                                  const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st) {
     hipLaunchKernelGGL(k_recon_fallback, dim3(256), dim3(64), 0, st, ndim, B, codes, sparse, out, a, flagged, nflagged);
+}
+
+void launch_decode_sync_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
+                              uint64_t nunits, hipStream_t st) {
+    allow_big_lds(k_decode_sync_multi<true>);
+    allow_big_lds(k_decode_sync_multi<false>);
+    if (!nunits) return;
+    DecodeArgs probe{};
+    probe.n_sub16 = max_sub16;
+    const size_t lds = table_lds_bytes(probe, t16) + (size_t)kSyncLdsWords * 4;
+    if (t16)
+        hipLaunchKernelGGL(k_decode_sync_multi<true>, dim3((unsigned)nunits), dim3(kSyncWg), lds, st, fd, start, nf);
+    else
+        hipLaunchKernelGGL(k_decode_sync_multi<false>, dim3((unsigned)nunits), dim3(kSyncWg), lds, st, fd, start, nf);
+}
+void launch_decode_pass_multi(bool t16, const DecField* fd, const uint64_t* sub_start, int nf, SubState S,
+                              uint32_t pass, uint64_t nsub_tot, hipStream_t st) {
+    // persistent workgroups; those past the listed count return at once
+    const dim3 grid((unsigned)std::min<uint64_t>(kPassGrid, std::max<uint64_t>(1, grid_for(nsub_tot, 256))));
+    if (t16)
+        hipLaunchKernelGGL(k_decode_pass_multi<true>, grid, dim3(256), 0, st, fd, sub_start, nf, S, pass);
+    else
+        hipLaunchKernelGGL(k_decode_pass_multi<false>, grid, dim3(256), 0, st, fd, sub_start, nf, S, pass);
+}
+void launch_decode_write_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
+                               uint64_t nunits, hipStream_t st) {
+    allow_big_lds(k_decode_write_multi<true>);
+    allow_big_lds(k_decode_write_multi<false>);
+    if (!nunits) return;
+    DecodeArgs probe{};
+    probe.n_sub16 = max_sub16;
+    const size_t lds = table_lds_bytes(probe, t16) + (size_t)kWriteStageLds * 4;
+    if (t16)
+        hipLaunchKernelGGL(k_decode_write_multi<true>, dim3((unsigned)nunits), dim3(kSyncWg), lds, st, fd, start, nf);
+    else
+        hipLaunchKernelGGL(k_decode_write_multi<false>, dim3((unsigned)nunits), dim3(kSyncWg), lds, st, fd, start, nf);
+}
+void launch_dec_field_totals(const DecField* fd, int nf, uint64_t* tot, hipStream_t st) {
+    if (nf > 0) hipLaunchKernelGGL(k_dec_field_totals, dim3(grid_for((uint64_t)nf, 256)), dim3(256), 0, st, fd, nf, tot);
+}
+uint64_t recon_tiles(int kind, const ReconArgs& a) {
+    const bool x4 = kind == kRc1x4 || kind == kRc2x4 || kind == kRc3x4;
+    const int nd = kind <= kRc1x4 ? 1 : kind <= kRc2x4 ? 2 : 3;
+    const uint64_t tx = x4 ? (a.d0 + 255) / 256 : a.ntiles_x;
+    return tx * (nd >= 2 ? a.ntiles_y : 1) * (nd >= 3 ? a.ntiles_z : 1);
+}
+void launch_reconstruct_multi(int kind, int B, const DecField* fd, const uint64_t* start, int nf, uint64_t ntiles,
+                              hipStream_t st) {
+    if (!ntiles) return;
+    const dim3 grid((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((ntiles + 3) / 4, 2048)));
+#define RM(K, BB)                                                                                \
+    if (kind == K && B == BB) {                                                                  \
+        hipLaunchKernelGGL((k_recon_multi<K, BB>), grid, dim3(256), 0, st, fd, start, nf, ntiles); \
+        return;                                                                                  \
+    }
+    RM(kRc3x4, 8)
+    RM(kRc3x4, 16)
+    RM(kRc2x4, 8)
+    RM(kRc2x4, 16)
+    RM(kRc2x4, 32)
+    RM(kRc2x4, 64)
+    RM(kRc1x4, 8)
+    RM(kRc1x4, 16)
+    RM(kRc1x4, 32)
+    RM(kRc1x4, 64)
+    RM(kRc3, 8)
+    RM(kRc3, 16)
+    RM(kRc3, 32)
+    RM(kRc3, 64)
+    RM(kRc2, 8)
+    RM(kRc2, 16)
+    RM(kRc2, 32)
+    RM(kRc2, 64)
+    RM(kRc1, 8)
+    RM(kRc1, 16)
+    RM(kRc1, 32)
+    RM(kRc1, 64)
+#undef RM
+}
+uint64_t decode_sync_units(uint64_t nsub) { return (nsub + kSyncStride - 1) / kSyncStride; }
+uint64_t decode_write_units(uint64_t nsub) { return (nsub + kSyncWg - 1) / kSyncWg; }
+void launch_gather_heads(const HeadCopy* hc, int n, uint8_t* dst, hipStream_t st) {
+    if (n > 0) hipLaunchKernelGGL(k_gather_heads, dim3((unsigned)n), dim3(256), 0, st, hc, dst);
+}
+void launch_reconstruct_fallback_multi(int B, const DecField* fd, int nf, hipStream_t st) {
+    if (nf > 0) hipLaunchKernelGGL(k_recon_fallback_multi, dim3(16, (unsigned)nf), dim3(64), 0, st, B, fd);
 }
 
 }  // namespace vsz

@@ -309,8 +309,8 @@ class Context:
         self._check(self._lib.vecsz_ctx_set_fused_encode(self._ctx, 1 if on else 0))
 
     def set_batch_fused(self, on: bool):
-        """Fused compress batches (one launch per stage for all fields) when
-        they qualify; False forces the worker-context path."""
+        """Fused compress and decompress batches (one launch per stage for
+        all fields) when they qualify; False forces the worker-context path."""
         self._check(self._lib.vecsz_ctx_set_batch_fused(self._ctx, 1 if on else 0))
 
     def compress_bound(self, params: Params) -> int:
@@ -379,7 +379,9 @@ class Context:
         self._check(self._lib.vecsz_compress_batch(self._ctx, nf, P, ins, outs, cap, sizes, nout, st))
         return [(sizes[k], nout[k]) for k in range(nf)]
 
-    def decompress_batch_into(self, params, d_in, in_bytes, d_out, caps):
+    def decompress_batch_into(self, params, d_in, in_bytes, d_out, caps, check: bool = True):
+        """Raw device pointers, one entry per field.  Raises on failure; with
+        check=False returns (status of the call, [status of each field])."""
         nf = len(params)
         P = (Params * nf)(*params)
         vp = ctypes.c_void_p
@@ -388,7 +390,10 @@ class Context:
         outs = (vp * nf)(*[vp(int(x)) for x in d_out])
         cap = (ctypes.c_uint64 * nf)(*[int(c) for c in caps])
         st = (ctypes.c_int * nf)()
-        self._check(self._lib.vecsz_decompress_batch(self._ctx, nf, P, ins, nb, outs, cap, st))
+        rc = self._lib.vecsz_decompress_batch(self._ctx, nf, P, ins, nb, outs, cap, st)
+        if check:
+            self._check(rc)
+        return rc, [st[k] for k in range(nf)]
 
     def compress_batch(self, fields, ebs, modes="abs", block: int = 8, dict_size: int = 4096):
         """Compress a list of float32 CUDA tensors in one batched call;
