@@ -270,15 +270,27 @@ def bench_batch(args):
         dplain()
     torch.cuda.synchronize()
     tdp = (time.perf_counter() - t0) / args.steps
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    # the batch call as it runs by default (these fields average ~85 MB: the
+    # worker contexts), over set_workers values
+    dsweep = {}
+    for nw in reversed([int(w) for w in args.workers.split(",")] + [1, 2, 4]):
+        if nw in dsweep:
+            continue
+        ctx.set_workers(nw)
         dbatch()
-    torch.cuda.synchronize()
-    tdb = (time.perf_counter() - t0) / args.steps
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            dbatch()
+        torch.cuda.synchronize()
+        dsweep[nw] = (time.perf_counter() - t0) / args.steps
+        log(f"decompress batch with {nw} workers: {dsweep[nw] * 1e3:.3f} ms")
+    ctx.set_workers(int(args.workers.split(",")[0]))
+    tdb = dsweep[int(args.workers.split(",")[0])]
     dsame = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
-    # the worker-context path on the same archives (the fused path is the default)
-    log("timing the worker-context decompress batch")
-    ctx.set_batch_fused(False)
+    # the fused path forced on the same archives
+    log("timing the fused decompress batch")
+    ctx.set_batch_fused("always")
     ys.clear()
     ys.extend(torch.empty_like(f) for f in fields)
     dbatch()
@@ -287,17 +299,20 @@ def bench_batch(args):
     for _ in range(args.steps):
         dbatch()
     torch.cuda.synchronize()
-    tdw = (time.perf_counter() - t0) / args.steps
-    ctx.set_batch_fused(True)
+    tdf = (time.perf_counter() - t0) / args.steps
     dsame = dsame and all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
     # stage times of the fused batch calls (HIP events on the context's stream)
     ctx.enable_timing(True)
-    bst = stage_avg(ctx, batch, min(args.steps, 3))
     dbst = stage_avg(ctx, dbatch, min(args.steps, 3))
+    ctx.set_batch_fused(True)
+    bst = stage_avg(ctx, batch, min(args.steps, 3))
     ctx.enable_timing(False)
     log("fused batch compress stages (ms):", {k: round(v, 4) for k, v in bst.items()})
     log("fused batch decompress stages (ms):", {k: round(v, 4) for k, v in dbst.items()})
     arch = sum(s for s, _ in rb)
+    del ys, ys1
+    torch.cuda.empty_cache()
+    small = bench_small_batch(args, ctx)
     out = {
         "metric": METRIC + " [batched form, 96-field mix]",
         "value": round(total / tb / 1e9, 2),
@@ -324,13 +339,90 @@ def bench_batch(args):
         "decompress_per_field_calls_ms": round(tdp * 1e3, 3),
         "decompress_batch_ms": round(tdb * 1e3, 3),
         "decompress_batch_speedup": round(tdp / tdb, 3),
-        "decompress_batch_worker_contexts_ms": round(tdw * 1e3, 3),
+        "decompress_batch_ms_by_workers": {str(k): round(v * 1e3, 3) for k, v in sorted(dsweep.items())},
+        "decompress_batch_fused_ms": round(tdf * 1e3, 3),
         "batch_stages_ms": {"compress": {k: round(v, 4) for k, v in bst.items()},
                             "decompress": {k: round(v, 4) for k, v in dbst.items()}},
         "identical_outputs": bool(dsame),
         "archive_bytes": arch,
+        "small_fields": small,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_small_batch(args, ctx, nfields=256, seed=1):
+    """A batch of small fields beside the C4 mix: 256 GRFs of 16K..1M
+    elements (log-uniform; half 2D, half 3D; r2r eb per field from {1e-2,
+    1e-3, 1e-4}).  Per-field calls against the batch calls as they run by
+    default (both fused for fields this small) and against the worker
+    contexts; results must be identical."""
+    import numpy as np
+    import torch
+
+    from vecsz_amd import datagen, make_params
+
+    rng = np.random.default_rng(seed)
+    fields, params, ebs = [], [], []
+    for k in range(nfields):
+        n = float(np.exp(rng.uniform(np.log(16 << 10), np.log(1 << 20))))
+        if k % 2 == 0:
+            y = max(8, int(round(np.sqrt(n))))
+            shape = (y, max(8, int(round(n / y))))
+        else:
+            z = max(8, int(round(np.cbrt(n))))
+            shape = (z, z, max(8, int(round(n / (z * z)))))
+        eb = float(rng.choice([1e-2, 1e-3, 1e-4]))
+        fields.append(datagen.grf(shape, seed=1000 + k, device="cuda", pow2=True))
+        params.append(make_params(tuple(reversed(shape)), eb, "r2r", args.block, args.dict))
+        ebs.append(eb)
+    torch.cuda.synchronize()
+    total = sum(4 * f.numel() for f in fields)
+    log(f"small-field batch: {nfields} fields, {total / 2**20:.1f} MiB")
+    caps = [ctx.compress_bound(p) for p in params]
+    outs = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+    outs1 = [torch.empty(c, dtype=torch.uint8, device="cuda") for c in caps]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps, r
+
+    plain = lambda: [ctx.compress_into(p, f.data_ptr(), o.data_ptr(), o.numel())
+                     for p, f, o in zip(params, fields, outs1)]
+    batch = lambda: ctx.compress_batch_into(params, [f.data_ptr() for f in fields], [o.data_ptr() for o in outs], caps)
+    tp, r1 = timed(plain)
+    tb, rb = timed(batch)
+    ctx.set_batch_fused(False)
+    tw, rw = timed(batch)
+    ctx.set_batch_fused(True)
+    same = r1 == rb == rw and all(torch.equal(a[:s], b[:s]) for a, b, (s, _) in zip(outs, outs1, rb))
+    vr = [float((f.max() - f.min()).item()) for f in fields]
+    dparams = [make_params((1,), 1e-4, "r2r", args.block, args.dict, v) for v in vr]
+    ys = [torch.empty_like(f) for f in fields]
+    ys1 = [torch.empty_like(f) for f in fields]
+    dplain = lambda: [ctx.decompress_into(p, a.data_ptr(), s, y.data_ptr(), y.numel())
+                      for p, a, (s, _), y in zip(dparams, outs, rb, ys1)]
+    dbatch = lambda: ctx.decompress_batch_into(dparams, [a.data_ptr() for a in outs], [s for s, _ in rb],
+                                               [y.data_ptr() for y in ys], [y.numel() for y in ys])
+    tdp, _ = timed(dplain)
+    tdb, _ = timed(dbatch)
+    dsame = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
+    ctx.set_batch_fused(False)
+    tdw, _ = timed(dbatch)
+    ctx.set_batch_fused(True)
+    dsame = dsame and all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ys, ys1))
+    res = {"fields": nfields, "bytes": total,
+           "compress_ms": {"per_field_calls": round(tp * 1e3, 3), "batch_fused": round(tb * 1e3, 3),
+                           "batch_worker_contexts": round(tw * 1e3, 3)},
+           "decompress_ms": {"per_field_calls": round(tdp * 1e3, 3), "batch_fused": round(tdb * 1e3, 3),
+                             "batch_worker_contexts": round(tdw * 1e3, 3)},
+           "identical_archives": bool(same), "identical_outputs": bool(dsame)}
+    log("small-field batch:", res)
+    return res
 
 
 def bench_serve(args):

@@ -154,8 +154,9 @@ int vecsz_archive_info(const uint8_t* h_archive, uint64_t bytes, int* ndim, uint
  * field (a work queue of Lorenzo-block tiles and encode chunks spanning all
  * fields, per-field histograms and codebooks, the codebooks built on
  * set_workers host threads).  Decompress of two or more archives with one
- * block size and dict_size (at most 4096) and no pad section runs fused the
- * same way: one copy of every archive's header and tree to the host, the
+ * block size and dict_size (at most 4096), no pad section and output
+ * capacities of at most 4M elements on average runs fused the same way (see
+ * vecsz_ctx_set_batch_fused): one copy of every archive's header and tree to the host, the
  * decode tables built on host threads, then one launch per stage (sync pass,
  * re-decode passes over one batch-wide list, scan, write pass, one
  * reconstruction launch per tile form).  Other batches are spread, largest
@@ -171,9 +172,12 @@ int vecsz_decompress_batch(vecsz_ctx* ctx, int nfields, const vecsz_params* para
                            const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status);
 /* Worker contexts used by the batched calls (1..16, default 8). */
 int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
-/* 0: batches always take the worker-context path; 1 (default): the fused
- * path when the batch qualifies.  Both write identical archives and
- * identical decompressed fields. */
+/* 0: batches always take the worker-context path.  1 (default): compress
+ * batches take the fused path when they qualify; decompress batches when
+ * they qualify and their output capacities average at most 4M elements
+ * (larger fields decompress faster as independent per-field pipelines on
+ * the worker contexts).  2: the fused path whenever a batch qualifies.  All
+ * write identical archives and identical decompressed fields. */
 int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
 
 /* 1: a single-field compress encodes in one pass (bit counts, prefix by a

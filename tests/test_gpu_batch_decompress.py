@@ -57,7 +57,7 @@ def test_fused_decompress_batch_matches_worker_path_single_calls_and_oracle(gpu_
     ranges = [po.value_range(f) if m == "r2r" else 0.0 for f, m in zip(fields, modes)]
     params = _params(archives, modes, ranges, block, dict_size)
     fused = [torch.empty(f.shape, dtype=torch.float32, device="cuda") for f in fields]
-    _run(gpu_ctx, archives, params, fused)
+    _run(gpu_ctx, archives, params, fused)  # small fields: the fused path by default
     try:
         gpu_ctx.set_batch_fused(False)
         pool = [torch.empty(f.shape, dtype=torch.float32, device="cuda") for f in fields]
@@ -110,15 +110,22 @@ def test_fused_decompress_batch_reports_failing_fields(gpu_ctx):
     caps = [f.size for f in fields]
     caps[1] = 2 * fields[1].size
     caps[6] = fields[6].size - 1  # field 6: output one element short
-    rc, st = _run(gpu_ctx, archives, params, outs, caps, check=False)
-    assert rc != 0
-    assert st[1] == VECSZ_ECORRUPT and st[3] == VECSZ_ECORRUPT and st[6] == VECSZ_ENOMEM, st
-    for k in range(8):
-        if k in (1, 3, 6):
-            continue
-        assert st[k] == 0, (k, st)
-        y1 = gpu_ctx.decompress(archives[k], modes[k], ranges[k])
-        assert torch.equal(outs[k][:fields[k].size].view(torch.int32), y1.reshape(-1).view(torch.int32)), k
+    for nw in (True, False):  # the fused path, then the worker contexts
+        gpu_ctx.set_batch_fused(nw)
+        try:
+            for o in outs:
+                o.fill_(-1.0)
+            rc, st = _run(gpu_ctx, archives, params, outs, caps, check=False)
+        finally:
+            gpu_ctx.set_batch_fused(True)
+        assert rc != 0
+        assert st[1] == VECSZ_ECORRUPT and st[3] == VECSZ_ECORRUPT and st[6] == VECSZ_ENOMEM, (nw, st)
+        for k in range(8):
+            if k in (1, 3, 6):
+                continue
+            assert st[k] == 0, (nw, k, st)
+            y1 = gpu_ctx.decompress(archives[k], modes[k], ranges[k])
+            assert torch.equal(outs[k][:fields[k].size].view(torch.int32), y1.reshape(-1).view(torch.int32)), (nw, k)
 
 
 def test_padded_archive_takes_the_worker_path(gpu_ctx):
@@ -134,3 +141,27 @@ def test_padded_archive_takes_the_worker_path(gpu_ctx):
     for k in range(3):
         y1 = gpu_ctx.decompress(archives[k], "abs")
         assert torch.equal(outs[k].view(torch.int32), y1.view(torch.int32)), k
+
+
+def test_batch_of_large_fields_takes_the_worker_path_unless_asked(gpu_ctx):
+    """Fields of more than 4M elements on average decompress on the worker
+    contexts by default; "always" runs them fused; the outputs agree."""
+    from vecsz_amd import datagen as dg
+
+    xs = [dg.grf((64, 256, 320), seed=31, device="cuda", pow2=True),
+          dg.grf((80, 256, 256), seed=32, device="cuda", pow2=True)]  # 5.2M elements each
+    archives = [gpu_ctx.compress(x, 1e-3, "r2r")[0] for x in xs]
+    ranges = [float((x.max() - x.min()).item()) for x in xs]
+    params = _params(archives, ["r2r"] * 2, ranges, 8, 4096)
+    outs = {}
+    for mode in (True, "always"):
+        gpu_ctx.set_batch_fused(mode)
+        try:
+            outs[mode] = [torch.empty_like(x) for x in xs]
+            _run(gpu_ctx, archives, params, outs[mode])
+        finally:
+            gpu_ctx.set_batch_fused(True)
+    for k in range(2):
+        y1 = gpu_ctx.decompress(archives[k], "r2r", ranges[k])
+        for mode in outs:
+            assert torch.equal(outs[mode][k].view(torch.int32), y1.view(torch.int32)), (mode, k)

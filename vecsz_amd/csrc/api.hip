@@ -107,6 +107,7 @@ struct vecsz_ctx {
     std::vector<vecsz_ctx*> workers;
     int nworkers = 8;
     bool fused_batch = true;  // batches as one launch per stage when they qualify
+    bool fused_batch_always = false;  // decompress batches of large fields fused as well
     bool fused_encode = false; // single-field encode as one pass (k_encode): measured slower, off by default
     bool device_codebook = false; // build the codebook on the device (no stop; measured slower, off by default)
     // launch shapes (workgroup caps of the grid-stride compress kernels), set
@@ -246,6 +247,9 @@ bool valid_block(int B) { return B == 8 || B == 16 || B == 32 || B == 64; }
 // any one GPU's memory), so that every per-element size it computes (at most
 // 16 bytes per element) stays far below 2^64.
 constexpr uint64_t kMaxElems = 1ull << 37;
+// Decompress batches whose fields average at most this many elements run
+// fused by default (a 16 MiB f32 field; larger ones take the worker contexts).
+constexpr uint64_t kDBatchFuseElems = 4ull << 20;
 
 // Element count of an ndim field with the given extents (x first), each in
 // [1, 2^31); false when an extent is out of range or the product exceeds
@@ -1924,8 +1928,8 @@ static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* 
         ra.halo_stride = 0;
         ra.hb0 = ref_nblk(ra.d0, B);
         ra.hb1 = h.ndim >= 2 ? ref_nblk(ra.d1, B) : 1;
-        const bool x4 = ra.d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(d_out[k]) & 15) == 0 && (h.ndim < 3 || B <= 16);
-        F.kind = h.ndim == 1 ? (x4 ? kRc1x4 : kRc1) : h.ndim == 2 ? (x4 ? kRc2x4 : kRc2) : (x4 ? kRc3x4 : kRc3);
+        // (code regions start on 64-element boundaries: aligned like a null pointer)
+        F.kind = recon_kind(h.ndim, B, ra.d0, nullptr, d_out[k]);
         rk[F.kind * ns + k + 1] = rk[F.kind * ns + k] + recon_tiles(F.kind, ra);
         fl_off[k] = fl_tot;
         fl_tot += ((ra.d0 + B - 1) / B) * (h.ndim >= 2 ? (ra.d1 + B - 1) / B : 1) * (h.ndim >= 3 ? (ra.d2 + B - 1) / B : 1);
@@ -2140,8 +2144,9 @@ int vecsz_ctx_set_fused_encode(vecsz_ctx* c, int on) {
 }
 
 int vecsz_ctx_set_batch_fused(vecsz_ctx* c, int on) {
-    if (!c) return VECSZ_EINVAL;
+    if (!c || on < 0 || on > 2) return VECSZ_EINVAL;
     c->fused_batch = on != 0;
+    c->fused_batch_always = on == 2;
     return VECSZ_OK;
 }
 
@@ -2255,7 +2260,16 @@ int vecsz_decompress_batch(vecsz_ctx* c, int nfields, const vecsz_params* p, con
                            const uint64_t* in_bytes, float* const* d_out, const uint64_t* out_cap, int* status) {
     if (!c || nfields < 0 || (nfields && (!p || !d_in || !in_bytes || !d_out || !out_cap || !status)))
         return fail(c, VECSZ_EINVAL, "NULL argument");
-    if (c->fused_batch && dbatch_fusable(c, nfields, p, d_in, d_out)) {
+    // Fused when the batch qualifies and its fields are small on average
+    // (per-field launches and host round trips then dominate).  Batches of
+    // large fields go to the worker contexts, whose independent pipelines
+    // overlap one field's latency-bound decode with another's streaming
+    // reconstruction and host work, where the fused stages run one after
+    // another (measured faster on the ~8 GiB mix of ~85 MB fields).
+    uint64_t cap_sum = 0;
+    for (int k = 0; k < nfields; k++) cap_sum += std::min<uint64_t>(out_cap[k], kMaxElems);
+    const bool small = nfields > 0 && cap_sum / (uint64_t)nfields <= kDBatchFuseElems;
+    if (c->fused_batch && (small || c->fused_batch_always) && dbatch_fusable(c, nfields, p, d_in, d_out)) {
         bool fallback = false;  // an archive with a pad section: the per-field path below
         const int rc = decompress_batch_fused(c, nfields, p, d_in, in_bytes, d_out, out_cap, status, &fallback);
         if (!fallback) return rc;

@@ -172,7 +172,11 @@ void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const f
 // subsequence state, the code-count prefixes and the re-decode lists are
 // batch-wide arrays (a field's subsequences are numbered from sub0); s,
 // sym_off and zero_off are views of its part.
-enum ReconKind { kRc1 = 0, kRc1x4, kRc2, kRc2x4, kRc3, kRc3x4, kRcKinds };
+// 64 x per wavefront (kRcN), four x per lane with vector (kRcNx4) or element
+// (kRcNx4u) loads and stores.
+enum ReconKind { kRc1 = 0, kRc1x4, kRc2, kRc2x4, kRc3, kRc3x4, kRc1x4u, kRc2x4u, kRc3x4u, kRcKinds };
+int recon_kind(int ndim, int B, uint64_t d0, const void* codes, const void* out);
+int recon_kind_ndim(int kind);
 struct DecField {
     DecodeArgs a;              // nsub = 0: nothing to decode on the device
     SubState s;

@@ -1300,7 +1300,11 @@ __device__ __forceinline__ void row_scan4(int32_t (&D)[4], const int32_t (&g)[4]
     }
 }
 
-template <int ND, int B>
+// ALN: the x extent is a multiple of 4 and codes / output are 8- / 16-byte
+// aligned, so a lane's four codes and outputs move as one uint2 / float4;
+// otherwise element by element, elements past the row's end reading as the
+// radius code (no change) and storing nothing.
+template <int ND, int B, bool ALN>
 __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, const ReconArgs& a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
@@ -1318,7 +1322,10 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
         const uint64_t ty = (t / ntx) % nty;
         const uint64_t tz = t / (ntx * nty);
         const uint64_t x0 = tx * 256 + 4 * lane;
-        const bool xin = x0 < a.d0;  // the dimension is a multiple of 4: all four or none
+        const bool xin = x0 < a.d0;  // ALN: all four or none
+        bool ein[4];                 // element x0 + i inside the row
+#pragma unroll
+        for (int i = 0; i < 4; i++) ein[i] = ALN ? xin : x0 + i < a.d0;
         const bool rowstart = (lane % L) == 0;
         bool bad = false;
         int32_t prev[ZB > 1 ? B : 1][4];
@@ -1335,9 +1342,18 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
                 const uint64_t y = ND >= 2 ? ty * B + yy : 0;
                 if (y < a.d1) {
                     const uint64_t idx = x0 + y * sy + z * sz;
-                    uint2 cw = make_uint2(R | (R << 16), R | (R << 16));
-                    if (xin) cw = *reinterpret_cast<const uint2*>(codes + idx);
-                    const uint32_t c[4] = {cw.x & 0xFFFF, cw.x >> 16, cw.y & 0xFFFF, cw.y >> 16};
+                    uint32_t c[4];
+                    if constexpr (ALN) {
+                        uint2 cw = make_uint2(R | (R << 16), R | (R << 16));
+                        if (xin) cw = *reinterpret_cast<const uint2*>(codes + idx);
+                        c[0] = cw.x & 0xFFFF;
+                        c[1] = cw.x >> 16;
+                        c[2] = cw.y & 0xFFFF;
+                        c[3] = cw.y >> 16;
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) c[i] = ein[i] ? (uint32_t)codes[idx + i] : R;
+                    }
                     int32_t g[4], base[4], Zv[4], D[4];
                     bool rs[4], isout[4];
                     float v[4];
@@ -1347,7 +1363,7 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
                         Zv[i] = (ND >= 3 && zz > 0) ? prev[ZB > 1 ? yy : 0][i] : 0;
                         const int32_t YZ = (ND >= 3 && yy > 0 && zz > 0) ? zrow[i] : 0;
                         base[i] = Y + Zv[i] - YZ;
-                        isout[i] = xin && c[i] == 0;
+                        isout[i] = ein[i] && c[i] == 0;
                         v[i] = 0.f;
                         if (isout[i]) {
                             v[i] = sparse[idx + i];
@@ -1371,7 +1387,13 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
                         }
                         srow[i] = sv;
                     }
-                    if (xin) *reinterpret_cast<float4*>(out + idx) = make_float4(o[0], o[1], o[2], o[3]);
+                    if constexpr (ALN) {
+                        if (xin) *reinterpret_cast<float4*>(out + idx) = make_float4(o[0], o[1], o[2], o[3]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            if (ein[i]) out[idx + i] = o[i];
+                    }
                 }
             }
         }
@@ -1379,7 +1401,7 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
     }
 }
 
-template <int ND, int B>
+template <int ND, int B, bool ALN>
 __global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ codes, const float* __restrict__ sparse,
                                                   float* __restrict__ out, ReconArgs a, uint32_t* flagged,
                                                   uint32_t* nflagged) {
@@ -1388,7 +1410,8 @@ __global__ void __launch_bounds__(256) k_recon_x4(const uint16_t* __restrict__ c
     const uint64_t ntiles = ntx * nty * ntz;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t t = wave; t < ntiles; t += nwaves) recon_x4_tile<ND, B>(t, codes, sparse, out, a, flagged, nflagged);
+    for (uint64_t t = wave; t < ntiles; t += nwaves)
+        recon_x4_tile<ND, B, ALN>(t, codes, sparse, out, a, flagged, nflagged);
 }
 
 // ------------------------------------------------------------------ batched fields
@@ -1534,11 +1557,17 @@ __global__ void __launch_bounds__(256) k_recon_multi(const DecField* __restrict_
         uint32_t* fl = fd[f].flagged;
         uint32_t* nfl = fd[f].nflagged;
         if constexpr (KIND == kRc1x4)
-            recon_x4_tile<1, B>(t, codes, sparse, out, ra, fl, nfl);
+            recon_x4_tile<1, B, true>(t, codes, sparse, out, ra, fl, nfl);
         else if constexpr (KIND == kRc2x4)
-            recon_x4_tile<2, B>(t, codes, sparse, out, ra, fl, nfl);
+            recon_x4_tile<2, B, true>(t, codes, sparse, out, ra, fl, nfl);
         else if constexpr (KIND == kRc3x4)
-            recon_x4_tile<3, B>(t, codes, sparse, out, ra, fl, nfl);
+            recon_x4_tile<3, B, true>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc1x4u)
+            recon_x4_tile<1, B, false>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc2x4u)
+            recon_x4_tile<2, B, false>(t, codes, sparse, out, ra, fl, nfl);
+        else if constexpr (KIND == kRc3x4u)
+            recon_x4_tile<3, B, false>(t, codes, sparse, out, ra, fl, nfl);
         else if constexpr (KIND == kRc3)
             recon_3d_tile<B, false>(t, codes, sparse, out, ra, fl, nfl);
         else if constexpr (KIND == kRc2)
@@ -1690,16 +1719,20 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
 #undef RC_PAD
         return;
     }
-    const bool x4 = a.d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
-                    (reinterpret_cast<uintptr_t>(codes) & 7) == 0 && (ndim < 3 || B <= 16);
-    if (x4) {
-        const uint64_t ntx = (a.d0 + 255) / 256;
-        const uint64_t work = ntx * (ndim >= 2 ? a.ntiles_y : 1) * (ndim >= 3 ? a.ntiles_z : 1);
+    const int kind = recon_kind(ndim, B, a.d0, codes, out);
+    if (kind != kRc1 && kind != kRc2 && kind != kRc3) {
+        const uint64_t work = recon_tiles(kind, a);
         const dim3 g4((unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 2048)));
-#define RX4(ND, BB)                                                                                       \
-    if (ndim == ND && B == BB) {                                                                          \
-        hipLaunchKernelGGL((k_recon_x4<ND, BB>), g4, dim3(256), 0, st, codes, sparse, out, a, flagged, nflagged); \
-        return;                                                                                           \
+        const bool aln = kind == kRc1x4 || kind == kRc2x4 || kind == kRc3x4;
+#define RX4(ND, BB)                                                                                              \
+    if (ndim == ND && B == BB) {                                                                                 \
+        if (aln)                                                                                                 \
+            hipLaunchKernelGGL((k_recon_x4<ND, BB, true>), g4, dim3(256), 0, st, codes, sparse, out, a, flagged,  \
+                               nflagged);                                                                        \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_recon_x4<ND, BB, false>), g4, dim3(256), 0, st, codes, sparse, out, a, flagged, \
+                               nflagged);                                                                        \
+        return;                                                                                                  \
     }
         RX4(3, 8)
         RX4(3, 16)
@@ -1776,9 +1809,27 @@ void launch_decode_write_multi(bool t16, uint32_t max_sub16, const DecField* fd,
 void launch_dec_field_totals(const DecField* fd, int nf, uint64_t* tot, hipStream_t st) {
     if (nf > 0) hipLaunchKernelGGL(k_dec_field_totals, dim3(grid_for((uint64_t)nf, 256)), dim3(256), 0, st, fd, nf, tot);
 }
+int recon_kind_ndim(int kind) {
+    switch (kind) {
+        case kRc1: case kRc1x4: case kRc1x4u: return 1;
+        case kRc2: case kRc2x4: case kRc2x4u: return 2;
+        default: return 3;
+    }
+}
+int recon_kind(int ndim, int B, uint64_t d0, const void* codes, const void* out) {
+    // four x per lane unless a 3D tile's B x 4 running values per lane would
+    // not fit (B > 16); vector loads and stores when rows are whole multiples
+    // of 4 and codes / output are 8- / 16-byte aligned
+    if (ndim == 3 && B > 16) return kRc3;
+    const bool aln = d0 % 4 == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(codes) & 7) == 0;
+    if (ndim == 1) return aln ? kRc1x4 : kRc1x4u;
+    if (ndim == 2) return aln ? kRc2x4 : kRc2x4u;
+    return aln ? kRc3x4 : kRc3x4u;
+}
 uint64_t recon_tiles(int kind, const ReconArgs& a) {
-    const bool x4 = kind == kRc1x4 || kind == kRc2x4 || kind == kRc3x4;
-    const int nd = kind <= kRc1x4 ? 1 : kind <= kRc2x4 ? 2 : 3;
+    const bool x4 = kind != kRc1 && kind != kRc2 && kind != kRc3;
+    const int nd = recon_kind_ndim(kind);
     const uint64_t tx = x4 ? (a.d0 + 255) / 256 : a.ntiles_x;
     return tx * (nd >= 2 ? a.ntiles_y : 1) * (nd >= 3 ? a.ntiles_z : 1);
 }
@@ -1813,6 +1864,16 @@ void launch_reconstruct_multi(int kind, int B, const DecField* fd, const uint64_
     RM(kRc1, 16)
     RM(kRc1, 32)
     RM(kRc1, 64)
+    RM(kRc3x4u, 8)
+    RM(kRc3x4u, 16)
+    RM(kRc2x4u, 8)
+    RM(kRc2x4u, 16)
+    RM(kRc2x4u, 32)
+    RM(kRc2x4u, 64)
+    RM(kRc1x4u, 8)
+    RM(kRc1x4u, 16)
+    RM(kRc1x4u, 32)
+    RM(kRc1x4u, 64)
 #undef RM
 }
 uint64_t decode_sync_units(uint64_t nsub) { return (nsub + kSyncStride - 1) / kSyncStride; }

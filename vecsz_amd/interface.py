@@ -308,10 +308,13 @@ class Context:
         default: faster on an MI355X); identical archives."""
         self._check(self._lib.vecsz_ctx_set_fused_encode(self._ctx, 1 if on else 0))
 
-    def set_batch_fused(self, on: bool):
-        """Fused compress and decompress batches (one launch per stage for
-        all fields) when they qualify; False forces the worker-context path."""
-        self._check(self._lib.vecsz_ctx_set_batch_fused(self._ctx, 1 if on else 0))
+    def set_batch_fused(self, on):
+        """True (default): fused batches (one launch per stage for all
+        fields) when they qualify, decompress batches only when their fields
+        are small on average; "always": decompress batches of large fields
+        fused as well; False: the worker-context path."""
+        mode = 2 if on == "always" else 1 if on else 0
+        self._check(self._lib.vecsz_ctx_set_batch_fused(self._ctx, mode))
 
     def compress_bound(self, params: Params) -> int:
         b = ctypes.c_uint64()
