@@ -1044,52 +1044,58 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             launch_decode_sync(da, ss, dsum, c->stream);
         }
         // Re-decode passes, two without a host check (most streams need one or
-        // none), then batches of them while the lists ask for more: each pass
-        // resolves at least the first subsequence of every chain, so nsub
-        // passes suffice.
+        // none), then the scan and the write pass, which do nothing unless the
+        // list the third pass would read is empty: in the common case the
+        // decode needs one host check, after the write.  Otherwise passes run
+        // in growing batches between host checks while the lists ask for more
+        // (each pass resolves at least the first subsequence of every chain,
+        // so nsub passes suffice), and then the scan and the write again.
         constexpr uint32_t kPassesAhead = 2;
         uint32_t pass = 0;
         {
             StageScope s(c, "decode_redo");
             for (; pass < kPassesAhead; pass++) launch_decode_pass(da, ss, pass, dsum, c->stream);
         }
-        HIPCHK(c, hipGetLastError());
-        DecSummary ds;
-        uint32_t left = 0, batch = 2;
-        for (;;) {
-            HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(h + 64, ss.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            memcpy(&ds, h, sizeof(ds));
-            memcpy(&left, h + 64, 4);
-            if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
-            if (left == 0) break;
-            if (pass > nsub + 2) return fail(c, VECSZ_ECORRUPT, "subsequence boundaries do not converge");
-            // Long chains (slowly self-synchronising streams) take many
-            // passes: run them in growing batches between host checks (a pass
-            // over an empty list does nothing but clear the list after next).
-            StageScope s(c, "decode_redo");
-            for (uint32_t b = 0; b < batch; b++) launch_decode_pass(da, ss, pass++, dsum, c->stream);
-            batch = std::min<uint32_t>(2 * batch, 16);
-        }
-        {
-            StageScope s(c, "decode_scan");
-            launch_scan_u32x2(ss.cnt, ss.zer, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
-                              c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream);
-        }
-        {
+        auto scan_write = [&](const uint32_t* guard) {
+            {
+                StageScope s(c, "decode_scan");
+                launch_scan_u32x2(ss.cnt, ss.zer, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
+                                  c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream, guard);
+            }
             StageScope s(c, "decode_write");
             launch_decode_write(da, ss, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(), codes, sparse, dsum,
-                                c->stream);
-        }
+                                c->stream, guard);
+        };
+        scan_write(ss.list_n + pass % 3);
         HIPCHK(c, hipGetLastError());
-        HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(h + 64, c->scan_tot.p, 16, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        memcpy(&ds, h, sizeof(ds));
+        DecSummary ds;
         uint64_t tot[2];
-        memcpy(tot, h + 64, 16);
-        if (ds.error) return fail(c, VECSZ_ECORRUPT, "bitstream does not decode");
+        uint32_t left = 0;
+        auto check = [&]() -> int {  // summary, symbol totals and the pending list count
+            HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(h + 64, c->scan_tot.p, 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(h + 96, ss.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            memcpy(&ds, h, sizeof(ds));
+            memcpy(tot, h + 64, 16);
+            memcpy(&left, h + 96, 4);
+            return ds.error ? fail(c, VECSZ_ECORRUPT, "bitstream does not decode") : VECSZ_OK;
+        };
+        if (int rc2 = check()) return rc2;
+        if (left) {
+            for (uint32_t batch = 2; left; batch = std::min<uint32_t>(2 * batch, 16)) {
+                if (pass > nsub + 2) return fail(c, VECSZ_ECORRUPT, "subsequence boundaries do not converge");
+                // (a pass over an empty list does nothing but clear the list after next)
+                {
+                    StageScope s(c, "decode_redo");
+                    for (uint32_t b = 0; b < batch; b++) launch_decode_pass(da, ss, pass++, dsum, c->stream);
+                }
+                if (int rc2 = check()) return rc2;
+            }
+            scan_write(nullptr);
+            HIPCHK(c, hipGetLastError());
+            if (int rc2 = check()) return rc2;
+        }
         if (tot[0] < n) return fail(c, VECSZ_ECORRUPT, "bitstream holds fewer symbols than the field");
         }
     }

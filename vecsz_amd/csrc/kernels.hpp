@@ -132,10 +132,12 @@ struct SubState {
 void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st);
 // Re-decode pass `pass` over the list the previous pass (or the sync pass) built.
 void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSummary* sum, hipStream_t st);
+// guard (nullable): every kernel of the scan does nothing while *guard is nonzero.
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
-                       uint64_t* totals, hipStream_t st);
+                       uint64_t* totals, hipStream_t st, const uint32_t* guard = nullptr);
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
-                         uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st);
+                         uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st,
+                         const uint32_t* guard = nullptr);
 void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st);
 // Decode with a decode index (index = its kIdxHead header words first): codes
 // and outliers as k_decode_write leaves them.

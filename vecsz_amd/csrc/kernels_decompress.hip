@@ -495,9 +495,11 @@ __global__ void __launch_bounds__(256) k_decode_pass(DecodeArgs a, SubState s, u
 constexpr int kScanPer = 4;
 constexpr int kScanBlock = 1024 * kScanPer;
 
+// guard (nullable): the scan does nothing while *guard is nonzero.
 __global__ void __launch_bounds__(1024) k_scan_local(const uint32_t* __restrict__ a, const uint32_t* __restrict__ b,
                                                      uint64_t n, uint64_t* __restrict__ oa, uint64_t* __restrict__ ob,
-                                                     uint64_t* __restrict__ bsum) {
+                                                     uint64_t* __restrict__ bsum, const uint32_t* guard) {
+    if (guard && *guard) return;
     __shared__ uint64_t sa[16], sbb[16];
     const uint64_t base = blockIdx.x * (uint64_t)kScanBlock + threadIdx.x * kScanPer;
     uint64_t va[kScanPer], vb[kScanPer], ta = 0, tb = 0;
@@ -552,7 +554,9 @@ __global__ void __launch_bounds__(1024) k_scan_local(const uint32_t* __restrict_
 
 // Exclusive scan of the per-block sums, 1024 at a time on one workgroup
 // (an LDS Hillis-Steele scan per pass, carried across passes).
-__global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsum, uint64_t nb, uint64_t* __restrict__ totals) {
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsum, uint64_t nb, uint64_t* __restrict__ totals,
+                                                      const uint32_t* guard) {
+    if (guard && *guard) return;
     __shared__ uint64_t sa[1024], sb[1024];
     const uint32_t t = threadIdx.x;
     uint64_t ca = 0, cb = 0;
@@ -584,7 +588,8 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsu
 }
 
 __global__ void __launch_bounds__(1024) k_scan_add(uint64_t* __restrict__ oa, uint64_t* __restrict__ ob, uint64_t n,
-                                                   const uint64_t* __restrict__ bsum) {
+                                                   const uint64_t* __restrict__ bsum, const uint32_t* guard) {
+    if (guard && *guard) return;
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i < n) {
         const uint64_t blk = i / kScanBlock;
@@ -842,7 +847,9 @@ template <bool T16>
 __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const SubState& s,
                                                   const uint64_t* __restrict__ sym_off,
                                                   const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
-                                                  float* __restrict__ sparse, DecSummary* sum, uint64_t unit) {
+                                                  float* __restrict__ sparse, DecSummary* sum, uint64_t unit,
+                                                  const uint32_t* guard = nullptr) {
+    if (guard && *guard) return;  // (the whole workgroup)
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
     uint32_t* st = lds + dec.load(lds, a);
@@ -930,8 +937,8 @@ __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const Sub
 template <bool T16>
 __global__ void __launch_bounds__(256) k_decode_write(DecodeArgs a, SubState s, const uint64_t* __restrict__ sym_off,
                                                       const uint64_t* __restrict__ zero_off, uint16_t* __restrict__ codes,
-                                                      float* __restrict__ sparse, DecSummary* sum) {
-    decode_write_unit<T16>(a, s, sym_off, zero_off, codes, sparse, sum, blockIdx.x);
+                                                      float* __restrict__ sparse, DecSummary* sum, const uint32_t* guard) {
+    decode_write_unit<T16>(a, s, sym_off, zero_off, codes, sparse, sum, blockIdx.x, guard);
 }
 
 __global__ void k_fill_u16(uint16_t* p, uint64_t n, uint16_t v) {
@@ -1665,14 +1672,15 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
 #undef DI
 }
 void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_t* oa, uint64_t* ob, uint64_t* block_tmp,
-                       uint64_t* totals, hipStream_t st) {
+                       uint64_t* totals, hipStream_t st, const uint32_t* guard) {
     const uint64_t nb = (n + kScanBlock - 1) / kScanBlock;
-    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(1024), 0, st, a, b, n, oa, ob, block_tmp);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_tmp, nb, totals);
-    hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 1024)), dim3(1024), 0, st, oa, ob, n, block_tmp);
+    hipLaunchKernelGGL(k_scan_local, dim3((unsigned)nb), dim3(1024), 0, st, a, b, n, oa, ob, block_tmp, guard);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_tmp, nb, totals, guard);
+    hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 1024)), dim3(1024), 0, st, oa, ob, n, block_tmp, guard);
 }
 void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
-                         uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st) {
+                         uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st,
+                         const uint32_t* guard) {
     allow_big_lds(k_decode_write<true>);
     allow_big_lds(k_decode_write<false>);
     const bool t16 = a.lut16 != nullptr;
@@ -1680,10 +1688,10 @@ void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_of
     const dim3 grid(grid_for(a.nsub, kSyncWg));
     if (t16)
         hipLaunchKernelGGL(k_decode_write<true>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
-                           sum);
+                           sum, guard);
     else
         hipLaunchKernelGGL(k_decode_write<false>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
-                           sum);
+                           sum, guard);
 }
 void launch_fill_u16(uint16_t* p, uint64_t n, uint16_t v, hipStream_t st) {
     unsigned g = (unsigned)std::min<uint64_t>(grid_for(n, 256), 8192);
