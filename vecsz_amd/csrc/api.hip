@@ -991,8 +991,10 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             !c->dec_sum.ensure(sizeof(DecSummary)))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
         // Decode tables in one pinned staging copy: the compact form when the
-        // symbols fit 12 bits and the deeper tables fit in LDS, else 32-bit.
-        const bool t16 = !dt.lut16.empty() && dt.sub16.size() <= kMaxSub16;
+        // symbols fit 12 bits (the kernels keep its deeper tables in LDS when
+        // they fit beside the first-level table, else read them in global
+        // memory), else 32-bit.
+        const bool t16 = !dt.lut16.empty();
         const size_t tab_bytes = t16 ? (dt.lut16.size() + dt.sub16.size()) * 2 : (dt.lut.size() + dt.sub.size()) * 4;
         if (!c->tabs.ensure(tab_bytes + 16) || !c->pin_tab.ensure(tab_bytes + 16))
             return fail(c, VECSZ_ENOMEM, "decode table allocation failed");
@@ -1017,7 +1019,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         da.sub = t16 ? nullptr : c->tabs.as<uint32_t>() + dt.lut.size();
         da.lut16 = t16 ? c->tabs.as<uint16_t>() : nullptr;
         da.sub16 = t16 ? c->tabs.as<uint16_t>() + dt.lut16.size() : nullptr;
-        da.n_sub16 = t16 ? (uint32_t)dt.sub16.size() : 0;
+        da.n_sub16 = t16 && dt.sub16.size() <= kMaxSub16 ? (uint32_t)dt.sub16.size() : 0;
         da.nsub = nsub;
         da.outliers = d_outl;
         da.n_outlier = nout;
@@ -1873,8 +1875,8 @@ static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* 
     for (int k = 0; k < nf; k++) {
         if (status[k] || c->db_dt[k].root_leaf) continue;
         const DecodeTables& dt = c->db_dt[k];
-        if (dt.lut16.empty() || dt.sub16.size() > kMaxSub16) t16 = false;
-        max_sub16 = std::max<uint32_t>(max_sub16, (uint32_t)dt.sub16.size());
+        if (dt.lut16.empty()) t16 = false;
+        if (dt.sub16.size() <= kMaxSub16) max_sub16 = std::max<uint32_t>(max_sub16, (uint32_t)dt.sub16.size());
     }
     std::vector<uint64_t> off(nf, 0), toff(nf, 0), fl_off(nf, 0);
     uint64_t ntot = 0, tab_tot = 0, nsub_tot = 0, fl_tot = 0;
@@ -1998,7 +2000,7 @@ static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* 
             memcpy(tp + dt.lut16.size() * 2, dt.sub16.data(), dt.sub16.size() * 2);
             F.a.lut16 = reinterpret_cast<const uint16_t*>(tdev);
             F.a.sub16 = reinterpret_cast<const uint16_t*>(tdev) + dt.lut16.size();
-            F.a.n_sub16 = (uint32_t)dt.sub16.size();
+            F.a.n_sub16 = dt.sub16.size() <= kMaxSub16 ? (uint32_t)dt.sub16.size() : 0;
         } else {
             memcpy(tp, dt.lut.data(), dt.lut.size() * 4);
             memcpy(tp + dt.lut.size() * 4, dt.sub.data(), dt.sub.size() * 4);

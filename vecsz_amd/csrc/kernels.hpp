@@ -106,7 +106,7 @@ struct DecodeArgs {
     const uint32_t* sub;     // deeper tables
     const uint16_t* lut16;   // compact tables (nullptr: use lut / sub)
     const uint16_t* sub16;
-    uint32_t n_sub16;        // entries of sub16
+    uint32_t n_sub16;        // entries of sub16 the kernels copy to LDS; 0: read in global memory
     uint64_t nsub;           // subsequences of kSubBits bits
     const uint8_t* outliers; // outlier section of the archive (any alignment)
     uint64_t n_outlier;

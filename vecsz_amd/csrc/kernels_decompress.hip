@@ -147,7 +147,7 @@ struct Decoder<true> {
         for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) l[i] = a.lut16[i];
         for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) l[nl + i] = a.sub16[i];
         lut = l;
-        sub = l + nl;
+        sub = ns ? l + nl : a.sub16;  // n_sub16 0: the deeper tables stay in global memory
         return (nl + ns + 1) / 2;
     }
     template <typename Win, typename P>
