@@ -165,3 +165,15 @@ def test_batch_of_large_fields_takes_the_worker_path_unless_asked(gpu_ctx):
         y1 = gpu_ctx.decompress(archives[k], "r2r", ranges[k])
         for mode in outs:
             assert torch.equal(outs[mode][k].view(torch.int32), y1.view(torch.int32)), (mode, k)
+
+
+def test_set_batch_fused_modes(gpu_ctx):
+    from vecsz_amd import VecszError
+
+    for mode in (False, True, "always", True):
+        gpu_ctx.set_batch_fused(mode)
+    rc = gpu_ctx._lib.vecsz_ctx_set_batch_fused(gpu_ctx._ctx, 3)
+    assert rc != 0
+    gpu_ctx.set_batch_fused(True)
+    with pytest.raises(VecszError):
+        gpu_ctx._check(rc)

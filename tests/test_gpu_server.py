@@ -149,3 +149,44 @@ def test_server_folds_queued_decompress_requests(gpu_ctx):
     # requests that shared a launch started and ended together
     runs = [q["run_ms"] for q in res if q is not None]
     assert len(set(runs)) < len(runs), runs
+
+
+def test_server_fold_members_fail_alone(gpu_ctx):
+    """A folded compress batch with one output buffer too small, and a folded
+    decompress batch holding an archive with a pad section (which sends that
+    fold to single calls): the short request fails alone, every other
+    result equals the request's standalone result."""
+    from vecsz_amd import Server, VecszError, datagen as dg, make_params
+
+    xs = [torch.from_numpy(dg.grf((40, 48 + i, 56), seed=200 + i)).cuda() for i in range(12)]
+    alone = [gpu_ctx.compress(x, 1e-3, "abs") for x in xs]
+    outs = [torch.empty(int(x.numel() * 12 + 4096), dtype=torch.uint8, device="cuda") for x in xs]
+    caps = [o.numel() for o in outs]
+    short = 7
+    caps[short] = alone[short][0].numel() - 1
+    torch.cuda.synchronize()
+    srv = Server(0, slots=1)
+    print("folded compress with a short buffer", flush=True)
+    tickets = [srv.submit_compress(make_params(tuple(reversed(x.shape)), 1e-3, "abs"), x.data_ptr(), o.data_ptr(), c)
+               for x, o, c in zip(xs, outs, caps)]
+    for k, t in enumerate(tickets):
+        if k == short:
+            with pytest.raises(VecszError):
+                srv.wait(t)
+        else:
+            q = srv.wait(t)
+            assert q["out_bytes"] == alone[k][0].numel() and torch.equal(outs[k][: q["out_bytes"]], alone[k][0]), k
+    # decompress: archive 5 has a pad section
+    archives = [a.clone() for a, _ in alone]
+    archives[5] = gpu_ctx.compress(xs[5], 1e-3, "abs", pad_mode="block", pad_type="avg")[0].clone()
+    want = [gpu_ctx.decompress(a, "abs") for a in archives]
+    ys = [torch.empty_like(x) for x in xs]
+    torch.cuda.synchronize()
+    print("folded decompress with a padded archive", flush=True)
+    pd = make_params((1,), 1e-4, "abs")
+    tickets = [srv.submit_decompress(pd, a.data_ptr(), a.numel(), y.data_ptr(), y.numel()) for a, y in zip(archives, ys)]
+    for t in tickets:
+        srv.wait(t)
+    srv.close()
+    for k in range(len(xs)):
+        assert torch.equal(ys[k].view(torch.int32), want[k].view(torch.int32)), k
