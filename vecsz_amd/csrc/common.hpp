@@ -123,6 +123,22 @@ __host__ __device__ inline IdxLayout idx_layout(uint64_t n) {
     return L;
 }
 
+// Workgroup barriers.  __syncthreads alone does not wait for the calling
+// wave's own LDS operations, and on gfx950 a wave past the barrier has read
+// LDS before another wave's atomic without return (issued before the
+// barrier) had landed: histogram counts were lost, a different few on every
+// run.  So every barrier here first waits for the wave's LDS operations
+// (lds_sync), or for its global memory operations as well where waves
+// exchange data through global memory (mem_sync).
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __syncthreads();
+}
+__device__ __forceinline__ void mem_sync() {
+    __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
+    __syncthreads();
+}
+
 // Field of work unit u in a batch: the f with start[f] <= u < start[f + 1]
 // (start is nondecreasing, start[0] = 0, start[nf] = total units > u).
 // Where u is uniform across the workgroup the search runs on scalar loads.

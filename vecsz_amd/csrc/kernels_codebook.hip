@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
         s_maxlen = 0;
         s_status = VECSZ_OK;
     }
-    __syncthreads();
+    mem_sync();
     uint32_t off = 0;
     for (uint32_t i = 0; i < t; i++) off += s_scan[i];  // 256 entries: a short serial prefix per thread
     for (uint32_t s = b0; s < b1; s++) {
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
         }
     }
     if (t == 255) s_nsym = off;
-    __syncthreads();
+    mem_sync();
     const uint32_t nsym = s_nsym;
     if (nsym == 0) {
         if (t == 0) info->status = VECSZ_EINVAL;
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
             qinsert(((qf(a) + qf(b)) << kCbNodeBits) | n);
         }
     }
-    __syncthreads();
+    mem_sync();
 
     // parents, and the leaf count of every subtree
     for (uint32_t v = t; v < nc; v += 256) {
@@ -138,13 +138,13 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
         par[kid[2 * k]] = (uint16_t)(m << 1);
         par[kid[2 * k + 1]] = (uint16_t)((m << 1) | 1);
     }
-    __syncthreads();
+    mem_sync();
     // (every walk is bounded by the node count, so a damaged record cannot loop)
     for (uint32_t v = t; v < nsym; v += 256) {
         uint32_t steps = 0;
         for (uint32_t p = par[v]; p != 0xFFFF && steps < nc; p = par[p >> 1], steps++) atomicAdd(&leaves[p >> 1], 1u);
     }
-    __syncthreads();
+    mem_sync();
 
     // depth, codeword and preorder number of every node; leaves fill the codebook
     for (uint32_t s = t; s < D; s += 256) {
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
         book[D + s] = 0;
         reinterpret_cast<uint8_t*>(book + 2 * D)[s] = 0;
     }
-    __syncthreads();
+    mem_sync();
     for (uint32_t v = t; v < nc; v += 256) {
         uint32_t depth = 0, idx = 0;
         uint64_t bits = 0;
@@ -179,14 +179,14 @@ __global__ void __launch_bounds__(256) k_codebook(const unsigned long long* __re
             }
         }
     }
-    __syncthreads();
+    mem_sync();
 
     // Huffman header and tree: [u32be node count][u32be dict][u8 0][L][R][C u32 LE][t u8],
     // L/R 1 byte wide up to 256 nodes, else 2 (src/huffman.hh:374-423)
     const uint32_t w = nc <= 256 ? 1u : 2u;
     const uint64_t tb = 1 + (uint64_t)nc * (2 * w + 4 + 1);
     if (8 + tb > huff_cap && t == 0) s_status = s_status ? s_status : VECSZ_ENOMEM;
-    __syncthreads();
+    mem_sync();
     if (s_status == VECSZ_OK) {
         uint8_t* tree = huff + 8;
         uint8_t* Lp = tree + 1;

@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) k_range(const float* __restrict__ x, uint
         smn[w] = mn;
         smx[w] = mx;
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x == 0) {
         for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
             mn = fminf(mn, smn[i]);
@@ -147,7 +147,7 @@ __global__ void __launch_bounds__(256) k_ebfam(uint32_t* __restrict__ mm, uint32
         slo[w] = lo;
         shi[w] = hi;
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x == 0) {
         for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
             lo = min(lo, slo[i]);
@@ -205,26 +205,26 @@ __device__ void hist_init(LdsHist& H, uint32_t* lds, int dict, int radius, unsig
     H.global = global;
     int total = kHotBins * 64 + (H.lds_cold ? dict : 0);
     for (int i = threadIdx.x; i < total; i += blockDim.x) lds[i] = 0;
-    __syncthreads();
+    lds_sync();
 }
 
 // Folds the lane-private hot copies into the bins, then writes this
 // workgroup's histogram densely to partials[blockIdx.x][*] (k_hist_reduce sums
 // them) -- or, for dictionaries too large for LDS, adds to the global bins.
 __device__ void hist_flush_row(LdsHist& H, uint32_t* __restrict__ dst) {
-    __syncthreads();
+    lds_sync();
     for (int h = threadIdx.x; h < kHotBins; h += blockDim.x) {
         uint32_t s = 0;
         for (int c = 0; c < 64; c++) s += H.hot[h * 64 + c];
         const int bin = H.lo + h;
         if (s && bin >= 0 && bin < H.dict) H.cold[bin] += s;
     }
-    __syncthreads();
+    lds_sync();
     for (int b = threadIdx.x; b < H.dict; b += blockDim.x) dst[b] = H.cold[b];
 }
 
 __device__ void hist_flush(LdsHist& H, uint32_t* __restrict__ partials) {
-    __syncthreads();
+    lds_sync();
     for (int h = threadIdx.x; h < kHotBins; h += blockDim.x) {
         uint32_t s = 0;
         for (int c = 0; c < 64; c++) s += H.hot[h * 64 + c];
@@ -236,7 +236,7 @@ __device__ void hist_flush(LdsHist& H, uint32_t* __restrict__ partials) {
                 atomicAdd(&H.global[bin], (unsigned long long)s);
         }
     }
-    __syncthreads();
+    lds_sync();
     if (H.lds_cold) {
         uint32_t* dst = partials + (uint64_t)blockIdx.x * H.dict;
         for (int b = threadIdx.x; b < H.dict; b += blockDim.x) dst[b] = H.cold[b];
@@ -826,7 +826,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* total, T* smem /* >= 4 */) 
     T inc = wave_incl_scan(v);
     const int w = threadIdx.x >> 6;
     if (lane_id() == 63) smem[w] = inc;
-    __syncthreads();
+    lds_sync();
     T off = 0, tot = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -834,7 +834,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* total, T* smem /* >= 4 */) 
         if (i < w) off += s;
         tot += s;
     }
-    __syncthreads();
+    lds_sync();
     *total = tot;
     return off + inc - v;
 }
@@ -867,7 +867,7 @@ __global__ void __launch_bounds__(256) k_chunk_bits(const uint16_t* __restrict__
     __shared__ uint8_t slen[LDS_TAB ? kMaxLdsLenTab : 4];
     if (LDS_TAB)
         for (int i = threadIdx.x; i < dict; i += blockDim.x) slen[i] = len_tab[i];
-    __syncthreads();
+    lds_sync();
     const uint32_t lane = lane_id();
     const uint64_t nw = (uint64_t)gridDim.x * (blockDim.x >> 6);
     for (uint64_t k = blockIdx.x * (uint64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); k < nchunks; k += nw) {
@@ -1283,7 +1283,7 @@ __global__ void __launch_bounds__(256) k_pack(const uint16_t* __restrict__ codes
             tlen[i] = i < dict ? len_tab[i] : 0u;
         }
     }
-    __syncthreads();
+    lds_sync();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint32_t* stage = lds + book_words(BOOK, dict) + wave * stage_words;
     const EncSummary S = *sum;
@@ -1369,7 +1369,7 @@ __device__ void lookback_wg(const uint64_t* st_b, const uint64_t* st_z, uint64_t
             s_mask[4 * wave + 2] = iz;
             s_mask[4 * wave + 3] = rz;
         }
-        __syncthreads();
+        lds_sync();
         // nearest inclusive prefix (j = 256: none in this window) and whether
         // every partition up to it has published
         int jb = 256, jz = 256;
@@ -1397,7 +1397,7 @@ __device__ void lookback_wg(const uint64_t* st_b, const uint64_t* st_z, uint64_t
                 s_sum[2 * wave + 1] = sz;
             }
         }
-        __syncthreads();
+        lds_sync();
         if (ok) {
 #pragma unroll
             for (int w = 0; w < 4; w++) {
@@ -1410,7 +1410,7 @@ __device__ void lookback_wg(const uint64_t* st_b, const uint64_t* st_z, uint64_t
         } else {
             __builtin_amdgcn_s_sleep(1);
         }
-        __syncthreads();  // the masks and sums are read before the next round writes them
+        lds_sync();  // the masks and sums are read before the next round writes them
         if (done_b && done_z) break;
     }
     pre_b = sum_b;
@@ -1470,7 +1470,7 @@ __global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ cod
     const uint32_t fill = (uint32_t)dict;  // symbols past the field: an LDS slot holding 0
     for (;;) {
         if (threadIdx.x == 0) s_part = atomicAdd(part_ctr, 1u);
-        __syncthreads();  // (also orders the codebook fill before the first lookup)
+        lds_sync();  // (also orders the codebook fill before the first lookup)
         const uint64_t part = s_part;
         if (part >= nparts) break;
         const uint64_t kw = part * (4 * WC) + (uint64_t)wave * WC;
@@ -1508,7 +1508,7 @@ __global__ void __launch_bounds__(256) k_encode(const uint16_t* __restrict__ cod
             s_wb[wave] = wb;
             s_wz[wave] = wz;
         }
-        __syncthreads();
+        lds_sync();
 
         // the partition's prefix: publish the aggregate, look back (the whole
         // workgroup), publish the inclusive prefix
@@ -1663,7 +1663,7 @@ __global__ void __launch_bounds__(256) k_range_multi(const BatchField* __restric
             smn[w] = mn;
             smx[w] = mx;
         }
-        __syncthreads();
+        lds_sync();
         if (threadIdx.x == 0) {
             for (int i = 1; i < 4; i++) {
                 mn = fminf(mn, smn[i]);
@@ -1672,7 +1672,7 @@ __global__ void __launch_bounds__(256) k_range_multi(const BatchField* __restric
             if (mn == mn) atomicMin(&fd[f].range[0], f2ord(mn));
             if (mx == mx) atomicMax(&fd[f].range[1], f2ord(mx));
         }
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -1756,7 +1756,7 @@ __global__ void __launch_bounds__(256) k_hist_codes_multi(const BatchField* __re
     for (int f = field_of(start, nf, u); u < u1; f++) {
         const uint64_t ue = min(u1, start[f + 1]);
         if (u >= ue) continue;
-        __syncthreads();  // the previous field's row has been read out of LDS
+        lds_sync();  // the previous field's row has been read out of LDS
         hist_init(H, lds, dict, radius, nullptr);
         const uint16_t* codes = fd[f].codes;  // 128-byte aligned
         const uint64_t n = fd[f].a.n;
@@ -1820,9 +1820,9 @@ __global__ void __launch_bounds__(256) k_chunk_bits_multi(const BatchField* __re
         const uint64_t ue = min(u1, start[f + 1]);
         if (u >= ue) continue;
         const uint8_t* len_tab = reinterpret_cast<const uint8_t*>(fd[f].book + 2 * dict);
-        __syncthreads();
+        lds_sync();
         for (int i = threadIdx.x; i < dict; i += blockDim.x) slen[i] = len_tab[i];
-        __syncthreads();
+        lds_sync();
         const uint16_t* codes = fd[f].codes;
         const uint64_t n = fd[f].a.n, nch = fd[f].nchunks, c0 = fd[f].chunk0, ub = start[f];
         // the next unit's codes are in flight while this one is summed
@@ -1893,7 +1893,7 @@ __global__ void __launch_bounds__(256) k_pack_multi(const BatchField* __restrict
         const uint32_t* cw_hi = fd[f].book;
         const uint32_t* cw_lo = cw_hi + dict;
         const uint8_t* len_tab = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
-        __syncthreads();
+        lds_sync();
         if (BOOK == kBookPacked) {
             for (int i = threadIdx.x; i <= dict; i += blockDim.x) {
                 const uint32_t l = i < dict ? len_tab[i] : 0u;
@@ -1905,7 +1905,7 @@ __global__ void __launch_bounds__(256) k_pack_multi(const BatchField* __restrict
                 tlen[i] = i < dict ? len_tab[i] : 0u;
             }
         }
-        __syncthreads();
+        lds_sync();
         if (!errs[f]) {
             const EncSummary S = sums[f];
             const uint16_t* codes = fd[f].codes;

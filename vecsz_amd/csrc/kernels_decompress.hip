@@ -302,7 +302,7 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
     sv.setup(a.stream, a.stream_bits);
     const uint64_t J0 = unit * kSyncStride;
     stage_words<kStageWords>(st, sv, J0 * (kSubBits / 32));
-    __syncthreads();
+    lds_sync();
     const uint32_t t = threadIdx.x;
     const uint64_t j = J0 + t;
     // local position of stream bit p: p - J0*kSubBits + shift0
@@ -341,7 +341,7 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             s.zer[0] = z;
         }
     }
-    __syncthreads();
+    lds_sync();
 
     // 2. continuation into subsequence j + 1 from this thread's own end
     if (t < kSyncStride && j + 1 < a.nsub) {
@@ -436,13 +436,13 @@ __device__ __forceinline__ void redecode_entries(const DecodeArgs& a, const SubS
     uint64_t* os = s.list_start[(pass + 1) & 1];
     uint32_t* on = &s.list_n[(pass + 1) % 3];
     const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
-    __syncthreads();  // the tables (first time) / the previous rows' readers
+    lds_sync();  // the tables (first time) / the previous rows' readers
     for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
         const uint32_t e = p / kRowWords, w = p % kRowWords;
         const uint64_t j = lj[e0 + e];
         rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
     }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < ne) {
         const uint64_t j = lj[e0 + threadIdx.x];
         const uint64_t start = ls[e0 + threadIdx.x];
@@ -525,7 +525,7 @@ __global__ void __launch_bounds__(1024) k_scan_local(const uint32_t* __restrict_
         sa[w] = ia;
         sbb[w] = ib;
     }
-    __syncthreads();
+    lds_sync();
     uint64_t pa = 0, pb = 0, tot_a = 0, tot_b = 0;
     for (int i = 0; i < 16; i++) {
         if (i < w) {
@@ -565,13 +565,13 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsu
         const uint64_t x = i < nb ? bsum[2 * i] : 0, y = i < nb ? bsum[2 * i + 1] : 0;
         sa[t] = x;
         sb[t] = y;
-        __syncthreads();
+        lds_sync();
         for (uint32_t d = 1; d < 1024; d <<= 1) {
             const uint64_t ax = t >= d ? sa[t - d] : 0, bx = t >= d ? sb[t - d] : 0;
-            __syncthreads();
+            lds_sync();
             sa[t] += ax;
             sb[t] += bx;
-            __syncthreads();
+            lds_sync();
         }
         if (i < nb) {
             bsum[2 * i] = ca + sa[t] - x;
@@ -579,7 +579,7 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint64_t* __restrict__ bsu
         }
         ca += sa[1023];
         cb += sb[1023];
-        __syncthreads();
+        lds_sync();
     }
     if (t == 0) {
         totals[0] = ca;
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     const uint32_t tw = dec.load(lds, a);
     const uint32_t lane = threadIdx.x & 63;
     uint32_t* stage = lds + tw + (threadIdx.x >> 6) * kIdxStageWords;
-    __syncthreads();
+    lds_sync();
     const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
     const uint64_t gw = g - lane;  // the wavefront's first segment
     const uint64_t nseg = L.nseg;
@@ -857,7 +857,7 @@ __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const Sub
     sv.setup(a.stream, a.stream_bits);
     const uint64_t J0 = unit * kSyncWg;
     stage_words<kWriteStageWords>(st, sv, J0 * (kSubBits / 32));
-    __syncthreads();
+    lds_sync();
     const uint64_t j = J0 + threadIdx.x;
     if (j >= a.nsub) return;
     uint32_t lp = (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0;
@@ -1488,9 +1488,9 @@ __global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __res
     uint32_t* on = &S.list_n[(pass + 1) % 3];
     for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256) {
         const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
-        __syncthreads();  // the previous rows' readers
+        lds_sync();  // the previous rows' readers
         if (threadIdx.x < ne) sfield[threadIdx.x] = field_of(sub_start, nf, lj[e0 + threadIdx.x]);
-        __syncthreads();
+        lds_sync();
         for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
             const uint32_t e = p / kRowWords, w = p % kRowWords;
             const int f = sfield[e];
@@ -1499,7 +1499,7 @@ __global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __res
             sv.setup(fd[f].a.stream, fd[f].a.stream_bits);
             rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
         }
-        __syncthreads();
+        lds_sync();
         if (threadIdx.x < ne) {
             const int f = sfield[threadIdx.x];
             const DecodeArgs& a = fd[f].a;

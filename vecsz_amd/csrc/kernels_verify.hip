@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(kVerifyWg) k_verify_pass1(const float* __restr
     }
     const int t = threadIdx.x;
     s_so[t] = so, s_sx[t] = sx, s_sq[t] = sq, s_mn[t] = mn, s_mx[t] = mx, s_err[t] = err, s_rel[t] = rel, s_idx[t] = idx;
-    __syncthreads();
+    lds_sync();
     for (int h = kVerifyWg / 2; h > 0; h >>= 1) {
         if (t < h) {
             s_so[t] += s_so[t + h];
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(kVerifyWg) k_verify_pass1(const float* __restr
                 s_idx[t] = s_idx[t + h];
             }
         }
-        __syncthreads();
+        lds_sync();
     }
     if (t == 0) {
         VerifyPart1 p;
@@ -87,14 +87,14 @@ __global__ void __launch_bounds__(kVerifyWg) k_verify_pass2(const float* __restr
     }
     const int t = threadIdx.x;
     s_c[t] = c, s_vo[t] = vo, s_vx[t] = vx;
-    __syncthreads();
+    lds_sync();
     for (int h = kVerifyWg / 2; h > 0; h >>= 1) {
         if (t < h) {
             s_c[t] += s_c[t + h];
             s_vo[t] += s_vo[t + h];
             s_vx[t] += s_vx[t + h];
         }
-        __syncthreads();
+        lds_sync();
     }
     if (t == 0) part[blockIdx.x] = VerifyPart2{s_c[0], s_vo[0], s_vx[0]};
 }
