@@ -1,0 +1,85 @@
+"""Fused compress batch with its post-codebook launches as a captured HIP
+graph: a repeated batch layout is captured on its second call and replayed
+after that.  Every call's archives are compared with the eager path's on the
+same data, with eager work and synchronisations between the replays; new
+field data and new output buffers in the same layout replay the same graph;
+a changed layout runs eagerly; the oracle pins the eager archives."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from tests.test_gpu_batch import _mix  # noqa: E402
+
+
+def _params(fields, ebs, modes):
+    from vecsz_amd import make_params
+
+    return [make_params(tuple(reversed(f.shape)), eb, m, 8, 4096) for f, eb, m in zip(fields, ebs, modes)]
+
+
+def _run(ctx, params, xs, outs):
+    res = ctx.compress_batch_into(params, [x.data_ptr() for x in xs], [o.data_ptr() for o in outs],
+                                  [o.numel() for o in outs])
+    return [bytes(o[:s].cpu().numpy().tobytes()) for o, (s, _) in zip(outs, res)], [n for _, n in res]
+
+
+def test_batch_graph_replays_match_eager_calls(gpu_ctx):
+    fields, ebs, modes = _mix(21)
+    keep = [k for k, f in enumerate(fields) if f.size >= 64]
+    fields = [fields[k] for k in keep]
+    ebs = [ebs[k] for k in keep]
+    modes = [modes[k] for k in keep]
+    params = _params(fields, ebs, modes)
+    xs = [torch.from_numpy(np.ascontiguousarray(f)).cuda() for f in fields]
+    outs = [torch.zeros(gpu_ctx.compress_bound(p), dtype=torch.uint8, device="cuda") for p in params]
+    try:
+        gpu_ctx.set_batch_graph(False)
+        eager, n_eager = _run(gpu_ctx, params, xs, outs)
+        for k, f in enumerate(fields):
+            if f.size <= (1 << 17):
+                a_orc, n_orc, _ = po.compress(f, ebs[k], modes[k] == "r2r")
+                assert eager[k] == a_orc and n_eager[k] == n_orc, k
+        gpu_ctx.set_batch_graph(True)
+        for call in range(5):  # eager (layout seen), capture + launch, then replays
+            print("graph call", call, flush=True)
+            for o in outs:
+                o.zero_()
+            got, n_got = _run(gpu_ctx, params, xs, outs)
+            assert got == eager and n_got == n_eager, call
+            # eager work on the same context and stream between the replays
+            a1, _ = gpu_ctx.compress(xs[1], ebs[1], modes[1])
+            assert a1.cpu().numpy().tobytes() == eager[1]
+            torch.cuda.synchronize()
+
+        # new data in the same buffers: new codebooks; the same layout may replay
+        for seed in (5, 6):
+            rng = np.random.default_rng(seed)
+            for x in xs:
+                x.add_(torch.from_numpy(rng.normal(0, 0.01, tuple(x.shape)).astype(np.float32)).cuda())
+            gpu_ctx.set_batch_graph(False)
+            want, n_want = _run(gpu_ctx, params, xs, outs)
+            gpu_ctx.set_batch_graph(True)
+            for call in range(3):
+                print("new data", seed, "call", call, flush=True)
+                got, n_got = _run(gpu_ctx, params, xs, outs)
+                assert got == want and n_got == n_want, (seed, call)
+
+        # new output buffers (their addresses are descriptor contents, not graph arguments)
+        outs2 = [torch.zeros_like(o) for o in outs]
+        for call in range(3):
+            got2, n_got2 = _run(gpu_ctx, params, xs, outs2)
+            assert got2 == want and n_got2 == n_want, call
+
+        # another layout (fewer fields) runs eagerly, then the first one again
+        got3, _ = _run(gpu_ctx, params[:-2], xs[:-2], outs[:-2])
+        assert got3 == want[:-2]
+        for call in range(3):
+            got4, _ = _run(gpu_ctx, params, xs, outs)
+            assert got4 == want, call
+    finally:
+        gpu_ctx.set_batch_graph(True)

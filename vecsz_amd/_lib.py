@@ -31,6 +31,7 @@ EXPORTS = (
     "vecsz_lossless_kind", "vecsz_lossless_decompress", "vecsz_sz_pack", "vecsz_sz_unpack", "vecsz_ctx_last_sidecar",
     "vecsz_params_from_sidecar", "vecsz_verify", "vecsz_ctx_enable_timing", "vecsz_ctx_timings", "vecsz_version",
     "vecsz_compress_noalloc", "vecsz_ctx_set_fused_encode", "vecsz_ctx_set_device_codebook",
+    "vecsz_ctx_set_batch_graph",
 )
 
 
@@ -143,6 +144,7 @@ def load():
     lib.vecsz_ctx_set_batch_fused.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_set_fused_encode.argtypes = [vp, ctypes.c_int]
     lib.vecsz_ctx_set_device_codebook.argtypes = [vp, ctypes.c_int]
+    lib.vecsz_ctx_set_batch_graph.argtypes = [vp, ctypes.c_int]
     lib.vecsz_index_words.argtypes = [P, u64p]
     lib.vecsz_compress_indexed.argtypes = [vp, P, vp, vp, u64, u64p, u64p, vp, u64]
     lib.vecsz_decompress_indexed.argtypes = [vp, P, vp, u64, vp, u64, vp, u64]

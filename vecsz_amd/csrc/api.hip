@@ -123,6 +123,12 @@ struct vecsz_ctx {
     DevBuf b_desc, b_start, b_hist, b_range, b_ebd, b_sums, b_errs, b_book, b_head, b_rows;
     HostBuf b_pin, b_pin_hist;
     std::vector<Codebook> b_cb;
+    // the launches after the codebooks, captured as one HIP graph when a
+    // batch repeats and replayed while every argument they take stays the
+    // same (b_key); b_seen: the last key run eagerly
+    bool batch_graph = true;
+    hipGraphExec_t b_graph = nullptr;
+    std::vector<uint64_t> b_key, b_seen;
     // fused batch decompress: field descriptors, unit starts, head copies,
     // gathered heads, summaries, symbol totals, fallback counts; pinned
     // staging of the same; decode tables per field
@@ -1211,6 +1217,7 @@ int vecsz_ctx_destroy(vecsz_ctx* c) {
     c->workers.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1417,6 +1424,47 @@ static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, 
 // of all codebooks and headers back, then chunk bits, one scan over every
 // chunk of the batch, per-field sizes and the bit packing.  The archives are
 // the ones the single-field path writes.
+// Issues `enqueue` (asynchronous copies and launches on c->stream only)
+// eagerly, or through the context's graph.  `key` lists every argument the
+// enqueued calls take.  A key seen for the first time runs eagerly; seen
+// again right after, the calls are captured (relaxed mode: other threads'
+// calls are not affected) and the graph launched; from then on the graph is
+// replayed while the key stays the same.  Batches that change from call to
+// call therefore never pay for a capture.  Timed calls, and a stream that
+// cannot be captured, run eagerly.
+template <class F>
+static hipError_t run_captured(vecsz_ctx* c, const std::vector<uint64_t>& key, F&& enqueue) {
+    if (!c->batch_graph || c->timing) return enqueue();
+    if (c->b_graph && key == c->b_key) return hipGraphLaunch(c->b_graph, c->stream);
+    if (key != c->b_seen) {
+        c->b_seen = key;
+        return enqueue();
+    }
+    if (c->b_graph) {
+        (void)hipGraphExecDestroy(c->b_graph);
+        c->b_graph = nullptr;
+        c->b_key.clear();
+    }
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+        (void)hipGetLastError();
+        return enqueue();
+    }
+    const hipError_t e = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : hipErrorUnknown);
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return e3;
+    c->b_graph = x;
+    c->b_key = key;
+    return hipGraphLaunch(x, c->stream);
+}
+
 static bool batch_fusable(const vecsz_ctx* c, int nf, const vecsz_params* p, const float* const* d_in,
                           uint8_t* const* d_out) {
     if (nf < 2 || nf > 4096) return false;
@@ -1694,39 +1742,57 @@ static int compress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* p,
             !c->out_off.ensure(ctot * 8) || !c->scan_tmp.ensure(((ctot + 4095) / 4096 + 1) * 16) ||
             !c->scan_tot.ensure(16))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-        HIPCHK(c, hipMemcpyAsync(c->b_book.p, books_h, books_bytes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->b_head.p, heads_h, head_tot, hipMemcpyHostToDevice, c->stream));
         memcpy(fd_h, fd.data(), desc_bytes);
         memcpy(ck0_h, ck0, (size_t)(nf + 1) * 8);
-        HIPCHK(c, hipMemcpyAsync(c->b_desc.p, fd_h, desc_bytes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync((void*)ck0_d, ck0_h, (size_t)(nf + 1) * 8, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->b_errs.p, errs_h, (size_t)nf * 4, hipMemcpyHostToDevice, c->stream));
         uint32_t* errs_d = c->b_errs.as<uint32_t>();
-        launch_place_heads(fdd, nf, errs_d, c->stream);
-        {
-            StageScope s(c, "chunk_bits");
-            launch_chunk_bits_multi(fdd, nf, ck0_d, ck0[nf], dict, c->chunk_bits.as<uint32_t>(),
-                                    c->chunk_zeros.as<uint32_t>(), c->enc_grid, c->stream);
-        }
-        {
-            StageScope s(c, "scan");
-            launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), ctot,
-                              c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
-                              c->scan_tot.as<uint64_t>(), c->stream);
-            launch_enc_summary_multi(fdd, nf, c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(),
-                                     c->scan_tot.as<uint64_t>(), ctot, c->b_sums.as<EncSummary>(), errs_d, c->stream);
-        }
-        {
-            StageScope s(c, "pack");
-            launch_pack_multi(fdd, nf, ck0_d, ck0[nf], dict, max_len, c->bit_off.as<uint64_t>(),
-                              c->out_off.as<uint64_t>(), c->b_sums.as<EncSummary>(), errs_d, c->enc_grid, c->stream);
-        }
-        HIPCHK(c, hipGetLastError());
         EncSummary* sums_h = reinterpret_cast<EncSummary*>(c->b_pin_hist.as<uint8_t>());
         uint32_t* errs_back = reinterpret_cast<uint32_t*>(c->b_pin_hist.as<uint8_t>() + (size_t)nf * sizeof(EncSummary));
-        HIPCHK(c, hipMemcpyAsync(sums_h, c->b_sums.p, (size_t)nf * sizeof(EncSummary), hipMemcpyDeviceToHost,
-                                 c->stream));
-        HIPCHK(c, hipMemcpyAsync(errs_back, errs_d, (size_t)nf * 4, hipMemcpyDeviceToHost, c->stream));
+        // uploads, launches and the copies back: what the graph holds
+        auto enqueue = [&]() -> hipError_t {
+            hipError_t e;
+#define ENQ(expr) \
+    if ((e = (expr)) != hipSuccess) return e
+            ENQ(hipMemcpyAsync(c->b_book.p, books_h, books_bytes, hipMemcpyHostToDevice, c->stream));
+            ENQ(hipMemcpyAsync(c->b_head.p, heads_h, head_tot, hipMemcpyHostToDevice, c->stream));
+            ENQ(hipMemcpyAsync(c->b_desc.p, fd_h, desc_bytes, hipMemcpyHostToDevice, c->stream));
+            ENQ(hipMemcpyAsync((void*)ck0_d, ck0_h, (size_t)(nf + 1) * 8, hipMemcpyHostToDevice, c->stream));
+            ENQ(hipMemcpyAsync(errs_d, errs_h, (size_t)nf * 4, hipMemcpyHostToDevice, c->stream));
+            launch_place_heads(fdd, nf, errs_d, c->stream);
+            {
+                StageScope s(c, "chunk_bits");
+                launch_chunk_bits_multi(fdd, nf, ck0_d, ck0[nf], dict, c->chunk_bits.as<uint32_t>(),
+                                        c->chunk_zeros.as<uint32_t>(), c->enc_grid, c->stream);
+            }
+            {
+                StageScope s(c, "scan");
+                launch_scan_u32x2(c->chunk_bits.as<uint32_t>(), c->chunk_zeros.as<uint32_t>(), ctot,
+                                  c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(), c->scan_tmp.as<uint64_t>(),
+                                  c->scan_tot.as<uint64_t>(), c->stream);
+                launch_enc_summary_multi(fdd, nf, c->bit_off.as<uint64_t>(), c->out_off.as<uint64_t>(),
+                                         c->scan_tot.as<uint64_t>(), ctot, c->b_sums.as<EncSummary>(), errs_d,
+                                         c->stream);
+            }
+            {
+                StageScope s(c, "pack");
+                launch_pack_multi(fdd, nf, ck0_d, ck0[nf], dict, max_len, c->bit_off.as<uint64_t>(),
+                                  c->out_off.as<uint64_t>(), c->b_sums.as<EncSummary>(), errs_d, c->enc_grid,
+                                  c->stream);
+            }
+            ENQ(hipGetLastError());
+            ENQ(hipMemcpyAsync(sums_h, c->b_sums.p, (size_t)nf * sizeof(EncSummary), hipMemcpyDeviceToHost,
+                               c->stream));
+            ENQ(hipMemcpyAsync(errs_back, errs_d, (size_t)nf * 4, hipMemcpyDeviceToHost, c->stream));
+#undef ENQ
+            return hipSuccess;
+        };
+        auto ptr = [](const void* q) { return (uint64_t)(uintptr_t)q; };
+        const std::vector<uint64_t> key = {
+            ptr(c->stream),        ptr(c->b_book.p),     ptr(c->b_head.p),  ptr(c->b_desc.p),  ptr(c->b_start.p),
+            ptr(c->b_errs.p),      ptr(c->chunk_bits.p), ptr(c->chunk_zeros.p), ptr(c->bit_off.p), ptr(c->out_off.p),
+            ptr(c->scan_tmp.p),    ptr(c->scan_tot.p),   ptr(c->b_sums.p),  ptr(c->b_pin.p),   ptr(c->b_pin_hist.p),
+            (uint64_t)nf,          (uint64_t)dict,       (uint64_t)max_len, pin_fixed,         desc_bytes,
+            books_bytes,           head_tot,             ck0[nf],           ctot,              (uint64_t)c->enc_grid};
+        HIPCHK(c, run_captured(c, key, enqueue));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int k = 0; k < nf; k++) {
             if (status[k]) continue;
@@ -2148,6 +2214,19 @@ int vecsz_ctx_set_device_codebook(vecsz_ctx* c, int on) {
 int vecsz_ctx_set_fused_encode(vecsz_ctx* c, int on) {
     if (!c) return VECSZ_EINVAL;
     c->fused_encode = on != 0;
+    return VECSZ_OK;
+}
+
+int vecsz_ctx_set_batch_graph(vecsz_ctx* c, int on) {
+    if (!c) return VECSZ_EINVAL;
+    c->batch_graph = on != 0;
+    if (!on && c->b_graph) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipGraphExecDestroy(c->b_graph);
+        c->b_graph = nullptr;
+        c->b_key.clear();
+    }
     return VECSZ_OK;
 }
 

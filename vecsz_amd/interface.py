@@ -316,6 +316,12 @@ class Context:
         mode = 2 if on == "always" else 1 if on else 0
         self._check(self._lib.vecsz_ctx_set_batch_fused(self._ctx, mode))
 
+    def set_batch_graph(self, on: bool):
+        """True (default): a fused compress batch whose layout repeats runs
+        its post-codebook launches as one captured HIP graph; False: always
+        eagerly.  Identical archives."""
+        self._check(self._lib.vecsz_ctx_set_batch_graph(self._ctx, 1 if on else 0))
+
     def compress_bound(self, params: Params) -> int:
         b = ctypes.c_uint64()
         self._check(self._lib.vecsz_compress_bound(ctypes.byref(params), ctypes.byref(b)))
