@@ -114,7 +114,8 @@ struct vecsz_ctx {
     // by vecsz_autotune; the output does not depend on them
     int dq_grid = 768, enc_grid = 2048;
     // compress scratch
-    DevBuf codes, sparse, hist, partials, chunk_bits, chunk_zeros, bit_off, out_off, summary, err, range, ebdev, book;
+    // small: histogram, eb family, codebook tables, error word, summary (small_layout)
+    DevBuf codes, sparse, small, partials, chunk_bits, chunk_zeros, bit_off, out_off, range;
     DevBuf enc_status, outl;  // fused encode: partition status words + counter, compact outliers
     DevBuf cbinfo;            // device-built codebook: node count, tree bytes, longest codeword, status
     DevBuf padv, halo, pad_scratch;  // padding: pad values, per-block halo values, reduction scratch
@@ -363,6 +364,35 @@ struct CScratch {
     uint64_t outl_cap = 0;           // outliers outl holds
 };
 
+// The small per-call scratch of a compress in one block, so that each host
+// round trip moves one copy: [histogram | eb family] goes to the host at the
+// codebook stop, [codebook tables | error word] comes back after it (the
+// error word's zeros travel with the tables), and [error word | summary]
+// goes to the host at the end.
+struct SmallLayout {
+    size_t hist, ebdev, book, err, summary, bytes;
+};
+constexpr size_t align_up(size_t b, size_t a) { return (b + a - 1) / a * a; }
+constexpr SmallLayout small_layout(int dict) {
+    SmallLayout L{};
+    L.hist = 0;
+    L.ebdev = (size_t)dict * 8;
+    L.book = align_up(L.ebdev + sizeof(EbDev), 256);
+    L.err = L.book + align_up((size_t)dict * 9, 16);
+    L.summary = L.err + 16;
+    L.bytes = L.summary + align_up(sizeof(EncSummary), 16) + 64;
+    return L;
+}
+CScratch small_scratch(CScratch s, uint8_t* block, int dict) {
+    const SmallLayout L = small_layout(dict);
+    s.hist = reinterpret_cast<unsigned long long*>(block + L.hist);
+    s.ebdev = reinterpret_cast<EbDev*>(block + L.ebdev);
+    s.book = reinterpret_cast<uint32_t*>(block + L.book);
+    s.err = reinterpret_cast<uint32_t*>(block + L.err);
+    s.summary = reinterpret_cast<EncSummary*>(block + L.summary);
+    return s;
+}
+
 // ---- allocation-free compress (vecsz_compress_noalloc)
 //
 // Its device scratch is the file-scope array `scratch`: one copy per GPU,
@@ -377,22 +407,18 @@ constexpr uint64_t kNaChunks = kNaMaxElems / kChunk;
 constexpr size_t na_align(size_t b) { return (b + 255) / 256 * 256; }
 constexpr size_t kNaCodes = 0;
 constexpr size_t kNaSparse = kNaCodes + na_align(kNaMaxElems * 2 + 64);
-constexpr size_t kNaHist = kNaSparse + na_align(kNaMaxElems * 4 + 64);
-constexpr size_t kNaPartials = kNaHist + na_align((size_t)kNaMaxDict * 8);
+constexpr size_t kNaSmall = kNaSparse + na_align(kNaMaxElems * 4 + 64);
+constexpr size_t kNaPartials = kNaSmall + na_align(small_layout(kNaMaxDict).bytes);
 constexpr size_t kNaChunkBits = kNaPartials + na_align((size_t)kNaDqGrid * kNaMaxDict * 4);
 constexpr size_t kNaChunkZeros = kNaChunkBits + na_align(kNaChunks * 4);
 constexpr size_t kNaBitOff = kNaChunkZeros + na_align(kNaChunks * 4);
 constexpr size_t kNaOutOff = kNaBitOff + na_align(kNaChunks * 8);
 constexpr size_t kNaScanTmp = kNaOutOff + na_align(kNaChunks * 8);
 constexpr size_t kNaScanTot = kNaScanTmp + na_align(((kNaChunks + 4095) / 4096 + 1) * 16);
-constexpr size_t kNaSummary = kNaScanTot + 256;
-constexpr size_t kNaErr = kNaSummary + na_align(sizeof(EncSummary));
-constexpr size_t kNaRange = kNaErr + 256;
-constexpr size_t kNaEbDev = kNaRange + na_align(8 + 8 * (size_t)kRangeParts);
-constexpr size_t kNaBook = kNaEbDev + na_align(sizeof(EbDev));
+constexpr size_t kNaRange = kNaScanTot + 256;
 constexpr uint64_t kNaParts = (kNaChunks + kEncPartChunks - 1) / kEncPartChunks;  // fused-encode partitions
 constexpr uint64_t kNaOutl = 1ull << 20;              // outliers of the fused encode; more: three-kernel path
-constexpr size_t kNaEncStatus = kNaBook + na_align((size_t)kNaMaxDict * 9 + 64);
+constexpr size_t kNaEncStatus = kNaRange + na_align(8 + 8 * (size_t)kRangeParts);
 constexpr size_t kNaOutlOff = kNaEncStatus + na_align((2 * kNaParts + 1) * 8);
 constexpr size_t kNaScratchBytes = kNaOutlOff + na_align(kNaOutl * 4);  // about 32.3 MiB
 static_assert(kNaParts >= kNaChunks / kEncPartChunks, "status words for every partition");
@@ -436,12 +462,11 @@ void na_setup(int device) {
     g_na_state[device] = 1;
 }
 
-CScratch na_scratch(void* base) {
+CScratch na_scratch(void* base, int dict) {
     uint8_t* b = static_cast<uint8_t*>(base);
-    CScratch s;
+    CScratch s = small_scratch(CScratch{}, b + kNaSmall, dict);
     s.codes = reinterpret_cast<uint16_t*>(b + kNaCodes);
     s.sparse = reinterpret_cast<float*>(b + kNaSparse);
-    s.hist = reinterpret_cast<unsigned long long*>(b + kNaHist);
     s.partials = reinterpret_cast<uint32_t*>(b + kNaPartials);
     s.chunk_bits = reinterpret_cast<uint32_t*>(b + kNaChunkBits);
     s.chunk_zeros = reinterpret_cast<uint32_t*>(b + kNaChunkZeros);
@@ -449,11 +474,7 @@ CScratch na_scratch(void* base) {
     s.out_off = reinterpret_cast<uint64_t*>(b + kNaOutOff);
     s.scan_tmp = reinterpret_cast<uint64_t*>(b + kNaScanTmp);
     s.scan_tot = reinterpret_cast<uint64_t*>(b + kNaScanTot);
-    s.summary = reinterpret_cast<EncSummary*>(b + kNaSummary);
-    s.err = reinterpret_cast<uint32_t*>(b + kNaErr);
     s.range = reinterpret_cast<uint32_t*>(b + kNaRange);
-    s.ebdev = reinterpret_cast<EbDev*>(b + kNaEbDev);
-    s.book = reinterpret_cast<uint32_t*>(b + kNaBook);
     s.enc_status = reinterpret_cast<uint64_t*>(b + kNaEncStatus);
     s.outl = reinterpret_cast<uint32_t*>(b + kNaOutlOff);
     s.outl_cap = kNaOutl;
@@ -501,20 +522,14 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     if (NA) {
         S = *fixed;
     } else {
-        if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) || !c->hist.ensure((size_t)dict * 8) ||
-            !c->summary.ensure(sizeof(EncSummary)) || !c->err.ensure(16) ||
-            !c->range.ensure(8 + 8 * (size_t)kRangeParts) || !c->ebdev.ensure(sizeof(EbDev)) ||
-            !c->book.ensure((size_t)dict * 9 + 64))
+        if (!c->codes.ensure(n * 2 + 64) || !c->sparse.ensure(n * 4 + 64) ||
+            !c->small.ensure(small_layout(dict).bytes) || !c->range.ensure(8 + 8 * (size_t)kRangeParts))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
         if (!c->pin.ensure((size_t)dict * 8 + 4096)) return fail(c, VECSZ_ENOMEM, "pinned buffer");
+        S = small_scratch(S, c->small.as<uint8_t>(), dict);
         S.codes = c->codes.as<uint16_t>();
         S.sparse = c->sparse.as<float>();
-        S.hist = c->hist.as<unsigned long long>();
-        S.summary = c->summary.as<EncSummary>();
-        S.err = c->err.as<uint32_t>();
         S.range = c->range.as<uint32_t>();
-        S.ebdev = c->ebdev.as<EbDev>();
-        S.book = c->book.as<uint32_t>();
     }
 
     DQArgs a;
@@ -610,6 +625,8 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     const uint64_t pad_bytes = 4 * pad_len;
     const bool dev_cb = !NA && c->device_codebook && dict <= kMaxLdsBook;
     const size_t book_bytes = (size_t)dict * 9;
+    const SmallLayout SL = small_layout(dict);
+    const size_t book_up = SL.err + 16 - SL.book;  // the tables' upload: through the error word
     Codebook& cb = NA ? g_na_host.cb : c->cb;
     uint64_t hist0 = 0, tb = 0, head = 0;
     int max_len = 64;  // the longest a device-built codebook may hold
@@ -626,9 +643,9 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     };
     if (!dev_cb) {
         uint8_t* hp = c->pin.as<uint8_t>();
-        CHIP(hipMemcpyAsync(hp, S.hist, (size_t)dict * 8, hipMemcpyDeviceToHost, c->stream));
-        if (device_range)
-            CHIP(hipMemcpyAsync(hp + (size_t)dict * 8, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
+        // the histogram and, right behind it, the eb family found on the device
+        CHIP(hipMemcpyAsync(hp, S.hist, (size_t)dict * 8 + (device_range ? sizeof(EbDev) : 0), hipMemcpyDeviceToHost,
+                            c->stream));
         CHIP(hipStreamSynchronize(c->stream));
         hist0 = reinterpret_cast<const uint64_t*>(hp)[0];  // code 0: the outliers
         if (device_range) {
@@ -670,6 +687,7 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
             memcpy(bk, cb.cw_hi.data(), (size_t)dict * 4);
             memcpy(bk + (size_t)dict * 4, cb.cw_lo.data(), (size_t)dict * 4);
             memcpy(bk + (size_t)dict * 8, cb.len.data(), (size_t)dict);
+            memset(bk + book_bytes, 0, book_up - book_bytes);  // through the error word: zeros
         }
         if (pad_bytes == 0) {
             CHIP(hipMemcpyAsync(d_out, hdr, head, hipMemcpyHostToDevice, c->stream));
@@ -679,7 +697,7 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
             CHIP(hipMemcpyAsync(d_out + mb, c->padv.p, pad_bytes, hipMemcpyDeviceToDevice, c->stream));
             CHIP(hipMemcpyAsync(d_out + mb + pad_bytes, hdr + mb, head - mb, hipMemcpyHostToDevice, c->stream));
         }
-        CHIP(hipMemcpyAsync(S.book, hdr + head, book_bytes, hipMemcpyHostToDevice, c->stream));
+        CHIP(hipMemcpyAsync(S.book, hdr + head, book_up, hipMemcpyHostToDevice, c->stream));
     } else {
         if (mb + pad_bytes + 8 > out_cap) {
             *out_bytes = mb + pad_bytes + 8;
@@ -698,7 +716,7 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     const uint32_t* cw_hi = S.book;
     const uint32_t* cw_lo = cw_hi + dict;
     const uint8_t* cw_len = reinterpret_cast<const uint8_t*>(cw_hi + 2 * dict);
-    launch_zero_u32(S.err, 4, c->stream);
+    if (dev_cb) launch_zero_u32(S.err, 4, c->stream);  // (the host path's upload cleared it)
     if (NA ? c->pin2.cap < 512 : !c->pin2.ensure(512)) CFAIL(VECSZ_ENOMEM, "pinned buffer");
     if (d_index) {
         // index header words 0-2 and 5; k_enc_summary writes words 3 and 4
@@ -765,17 +783,17 @@ int compress_run(vecsz_ctx* c, const vecsz_params* p, const float* d_in, uint8_t
     }
     CHIP(hipGetLastError());
     uint8_t* h = c->pin2.as<uint8_t>();
-    CHIP(hipMemcpyAsync(h, S.summary, sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
-    CHIP(hipMemcpyAsync(h + 128, S.err, 4, hipMemcpyDeviceToHost, c->stream));
+    // the error word and, behind it, the summary
+    CHIP(hipMemcpyAsync(h, S.err, SL.summary - SL.err + sizeof(EncSummary), hipMemcpyDeviceToHost, c->stream));
     if (dev_cb) {
         CHIP(hipMemcpyAsync(h + 256, c->cbinfo.p, sizeof(CodebookInfo), hipMemcpyDeviceToHost, c->stream));
         if (device_range) CHIP(hipMemcpyAsync(h + 320, S.ebdev, sizeof(EbDev), hipMemcpyDeviceToHost, c->stream));
     }
     CHIP(hipStreamSynchronize(c->stream));
     EncSummary ES;
-    memcpy(&ES, h, sizeof(ES));
+    memcpy(&ES, h + (SL.summary - SL.err), sizeof(ES));
     uint32_t e;
-    memcpy(&e, h + 128, 4);
+    memcpy(&e, h, 4);
     timing_end(c);
     if (dev_cb) {
         // what the host path checks at its stop: the range, then the codebook
@@ -1260,7 +1278,7 @@ int vecsz_compress_noalloc(vecsz_ctx* c, const vecsz_params* p, const float* d_i
     std::lock_guard<std::mutex> lk(g_na_mutex);
     if (c->device < 0 || c->device >= kNaMaxDevices || g_na_state[c->device] != 1)
         return fail(c, VECSZ_EHIP, "allocation-free compress: no scratch address on this device");
-    const CScratch S = na_scratch(g_na_ptr[c->device]);
+    const CScratch S = na_scratch(g_na_ptr[c->device], p->dict_size);
     const bool was_timing = c->timing;
     c->timing = false;  // stage timing would create events
     rc = compress_run<true>(c, p, d_in, d_out, out_cap, out_bytes, n_outlier, nullptr, &S);
