@@ -146,8 +146,9 @@ struct vecsz_ctx {
     DevBuf vparts;
     HostBuf vpin;
     // decompress scratch
+    // dec_small: decoder summary, symbol and zero totals, list counts (kDecSmall*)
     DevBuf tabs, sub_b, sub_e, sub_cnt, sub_zeros, sub_lj, sub_ls, sub_ln, sym_off, zero_off, scan_tmp, scan_tot,
-        dec_sum, flagged, nflagged;
+        dec_small, flagged, nflagged;
     // host staging
     DevBuf stage_in, stage_out;
     HostBuf pin, pin2, pin_tab;
@@ -392,6 +393,14 @@ CScratch small_scratch(CScratch s, uint8_t* block, int dict) {
     s.summary = reinterpret_cast<EncSummary*>(block + L.summary);
     return s;
 }
+
+// The small state of a single-field decompress in one block, so that each
+// host check is one copy: [decoder summary | symbol and zero totals | the
+// three re-decode list counts].
+constexpr size_t kDecSmallTotals = 64;
+constexpr size_t kDecSmallLists = 96;
+constexpr size_t kDecSmallBytes = 112;
+static_assert(sizeof(DecSummary) <= kDecSmallTotals, "decoder summary fits its slot");
 
 // ---- allocation-free compress (vecsz_compress_noalloc)
 //
@@ -996,6 +1005,33 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
     }
     // 1D with a decode index: the reconstruction runs inside the decode
     const bool fused_1d = d_index && ndim == 1 && !dt.root_leaf && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+    ra.d0 = d0;
+    ra.d1 = d1;
+    ra.d2 = d2;
+    ra.n = n;
+    ra.radius = R;
+    ra.twoeb = eb.twoeb;
+    ra.ntiles_x = (d0 + 63) / 64;
+    ra.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
+    ra.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
+    const uint64_t nblocks = ((d0 + B - 1) / B) * (ndim >= 2 ? (d1 + B - 1) / B : 1) * (ndim >= 3 ? (d2 + B - 1) / B : 1);
+    const uint64_t rwork = ra.ntiles_x * ra.ntiles_y * ra.ntiles_z;
+    const int rgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>((rwork + 3) / 4, 2048));
+    if (!fused_1d && (!c->flagged.ensure(nblocks * 4 + 16) || !c->nflagged.ensure(16)))
+        return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
+    // Lorenzo reconstruction; guard: as the write pass's (nonzero: nothing
+    // runs, the decode is not final yet)
+    bool recon_issued = false;
+    auto reconstruct = [&](const uint32_t* guard) {
+        launch_zero_u32(c->nflagged.as<uint32_t>(), 1, c->stream);
+        StageScope s(c, "reconstruct");
+        ra.guard = guard;
+        launch_reconstruct(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(), c->nflagged.as<uint32_t>(),
+                           rgrid, c->stream);
+        launch_reconstruct_fallback(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(),
+                                    c->nflagged.as<uint32_t>(), c->stream);
+        recon_issued = true;
+    };
 
     if (dt.root_leaf) {
         StageScope s(c, "decode");
@@ -1009,10 +1045,8 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         const uint64_t nsub = std::max<uint64_t>(1, (T + kSubBits - 1) / kSubBits);
         if (!c->sub_b.ensure(nsub * 8) || !c->sub_e.ensure(nsub * 8) || !c->sub_cnt.ensure(nsub * 4) ||
             !c->sub_zeros.ensure(nsub * 4) || !c->sub_lj.ensure(nsub * 8) || !c->sub_ls.ensure(nsub * 16) ||
-            !c->sub_ln.ensure(16) ||
             !c->sym_off.ensure(nsub * 8) || !c->zero_off.ensure(nsub * 8) ||
-            !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->scan_tot.ensure(16) ||
-            !c->dec_sum.ensure(sizeof(DecSummary)))
+            !c->scan_tmp.ensure(((nsub + 4095) / 4096 + 1) * 16) || !c->dec_small.ensure(kDecSmallBytes))
             return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
         // Decode tables in one pinned staging copy: the compact form when the
         // symbols fit 12 bits (the kernels keep its deeper tables in LDS when
@@ -1033,8 +1067,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             memcpy(tp + dt.lut.size() * 4, dt.sub.data(), dt.sub.size() * 4);
         }
         HIPCHK(c, hipMemcpyAsync(c->tabs.p, tp, tab_bytes, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->dec_sum.p, 0, sizeof(DecSummary), c->stream));
-        HIPCHK(c, hipMemsetAsync(c->sub_ln.p, 0, 16, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->dec_small.p, 0, kDecSmallBytes, c->stream));
         DecodeArgs da;
         da.stream = d_stream;
         da.stream_bits = T;
@@ -1056,8 +1089,9 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         ss.list_j[1] = c->sub_lj.as<uint32_t>() + nsub;
         ss.list_start[0] = c->sub_ls.as<uint64_t>();
         ss.list_start[1] = c->sub_ls.as<uint64_t>() + nsub;
-        ss.list_n = c->sub_ln.as<uint32_t>();
-        DecSummary* dsum = c->dec_sum.as<DecSummary>();
+        ss.list_n = reinterpret_cast<uint32_t*>(c->dec_small.as<uint8_t>() + kDecSmallLists);
+        DecSummary* dsum = c->dec_small.as<DecSummary>();
+        uint64_t* stot = reinterpret_cast<uint64_t*>(c->dec_small.as<uint8_t>() + kDecSmallTotals);
         if (d_index) {
             StageScope s(c, "decode_indexed");
             Recon1D r1{d_out, B, R, eb.twoeb, ra.halo, ra.halo_stride};
@@ -1086,25 +1120,27 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
             {
                 StageScope s(c, "decode_scan");
                 launch_scan_u32x2(ss.cnt, ss.zer, nsub, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(),
-                                  c->scan_tmp.as<uint64_t>(), c->scan_tot.as<uint64_t>(), c->stream, guard);
+                                  c->scan_tmp.as<uint64_t>(), stot, c->stream, guard);
             }
             StageScope s(c, "decode_write");
             launch_decode_write(da, ss, c->sym_off.as<uint64_t>(), c->zero_off.as<uint64_t>(), codes, sparse, dsum,
                                 c->stream, guard);
         };
         scan_write(ss.list_n + pass % 3);
+        // the reconstruction goes in behind the write pass, under the same
+        // guard, before the host check: in the common case the check then
+        // waits on the whole decompress
+        if (!fused_1d) reconstruct(ss.list_n + pass % 3);
         HIPCHK(c, hipGetLastError());
         DecSummary ds;
         uint64_t tot[2];
         uint32_t left = 0;
         auto check = [&]() -> int {  // summary, symbol totals and the pending list count
-            HIPCHK(c, hipMemcpyAsync(h, dsum, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(h + 64, c->scan_tot.p, 16, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipMemcpyAsync(h + 96, ss.list_n + pass % 3, 4, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(h, c->dec_small.p, kDecSmallBytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             memcpy(&ds, h, sizeof(ds));
-            memcpy(tot, h + 64, 16);
-            memcpy(&left, h + 96, 4);
+            memcpy(tot, h + kDecSmallTotals, 16);
+            memcpy(&left, h + kDecSmallLists + 4 * (pass % 3), 4);
             return ds.error ? fail(c, VECSZ_ECORRUPT, "bitstream does not decode") : VECSZ_OK;
         };
         if (int rc2 = check()) return rc2;
@@ -1119,6 +1155,7 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
                 if (int rc2 = check()) return rc2;
             }
             scan_write(nullptr);
+            if (!fused_1d) reconstruct(nullptr);
             HIPCHK(c, hipGetLastError());
             if (int rc2 = check()) return rc2;
         }
@@ -1126,31 +1163,9 @@ int decompress_device(vecsz_ctx* c, const vecsz_params* p, const uint8_t* d_in, 
         }
     }
 
-    // Lorenzo reconstruction.
-    ra.d0 = d0;
-    ra.d1 = d1;
-    ra.d2 = d2;
-    ra.n = n;
-    ra.radius = R;
-    ra.twoeb = eb.twoeb;
-    ra.ntiles_x = (d0 + 63) / 64;
-    ra.ntiles_y = ndim >= 2 ? (d1 + B - 1) / B : 1;
-    ra.ntiles_z = ndim >= 3 ? (d2 + B - 1) / B : 1;
-    const uint64_t nblocks = ((d0 + B - 1) / B) * (ndim >= 2 ? (d1 + B - 1) / B : 1) * (ndim >= 3 ? (d2 + B - 1) / B : 1);
-    const uint64_t work = ra.ntiles_x * ra.ntiles_y * ra.ntiles_z;
-    const int grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((work + 3) / 4, 2048));
-    if (!fused_1d) {
-        if (!c->flagged.ensure(nblocks * 4 + 16) || !c->nflagged.ensure(16))
-            return fail(c, VECSZ_ENOMEM, "device scratch allocation failed");
-        HIPCHK(c, hipMemsetAsync(c->nflagged.p, 0, 4, c->stream));
-        StageScope s(c, "reconstruct");
-        launch_reconstruct(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(), c->nflagged.as<uint32_t>(),
-                           grid, c->stream);
-        launch_reconstruct_fallback(ndim, B, codes, sparse, d_out, ra, c->flagged.as<uint32_t>(),
-                                    c->nflagged.as<uint32_t>(), c->stream);
-    }
+    if (!fused_1d && !recon_issued) reconstruct(nullptr);
     HIPCHK(c, hipGetLastError());
-    if (indexed) HIPCHK(c, hipMemcpyAsync(h, c->dec_sum.p, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
+    if (indexed) HIPCHK(c, hipMemcpyAsync(h, c->dec_small.p, sizeof(DecSummary), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     timing_end(c);
     if (indexed) {

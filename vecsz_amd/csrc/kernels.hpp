@@ -164,6 +164,7 @@ struct ReconArgs {
     uint64_t ntiles_x, ntiles_y, ntiles_z;
     const float* halo;  // as DQArgs::halo; nullptr = the +0.0 halo
     uint64_t halo_stride, hb0, hb1;
+    const uint32_t* guard = nullptr;  // nonzero *guard: the launch does nothing
 };
 void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* sparse, float* out, const ReconArgs& a,
                         uint32_t* flagged, uint32_t* nflagged, int grid, hipStream_t st);
