@@ -395,7 +395,7 @@ def bench_small_batch(args, ctx, nfields=256, seed=1):
                      for p, f, o in zip(params, fields, outs1)]
     batch = lambda: ctx.compress_batch_into(params, [f.data_ptr() for f in fields], [o.data_ptr() for o in outs], caps)
     tp, r1 = timed(plain)
-    tb, rb = timed(batch)  # the post-codebook launches replay as one graph from the third call on
+    tb, rb = timed(batch)  # the batch replays as two captured graphs from the third call on
     ctx.set_batch_graph(False)
     tbe, rbe = timed(batch)
     ctx.set_batch_graph(True)

@@ -179,12 +179,13 @@ int vecsz_ctx_set_workers(vecsz_ctx* ctx, int nworkers);
  * the worker contexts).  2: the fused path whenever a batch qualifies.  All
  * write identical archives and identical decompressed fields. */
 int vecsz_ctx_set_batch_fused(vecsz_ctx* ctx, int on);
-/* 1 (default): in a fused compress batch, the uploads, launches and copies
- * that follow the codebooks are captured as one HIP graph when the same
- * batch layout comes twice in a row (same field count, buffers, sizes and
- * launch shapes; the field data may differ), and the graph is replayed for
- * as long as the layout stays the same; other batches run eagerly.  0: always
- * eagerly.  No counterpart in the reference.  Identical archives. */
+/* 1 (default): a fused compress batch is captured as two HIP graphs, the
+ * uploads, launches and copies before its host codebook stop and those
+ * after it, when the same batch layout comes twice in a row (same field
+ * count, buffers, sizes and launch shapes; the field data may differ), and
+ * the graphs are replayed for as long as the layout stays the same; other
+ * batches run eagerly.  0: always eagerly.  No counterpart in the
+ * reference.  Identical archives. */
 int vecsz_ctx_set_batch_graph(vecsz_ctx* ctx, int on);
 
 /* 1: a single-field compress encodes in one pass (bit counts, prefix by a

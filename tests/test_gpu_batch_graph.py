@@ -1,9 +1,10 @@
-"""Fused compress batch with its post-codebook launches as a captured HIP
-graph: a repeated batch layout is captured on its second call and replayed
-after that.  Every call's archives are compared with the eager path's on the
-same data, with eager work and synchronisations between the replays; new
-field data and new output buffers in the same layout replay the same graph;
-a changed layout runs eagerly; the oracle pins the eager archives."""
+"""Fused compress batch as two captured HIP graphs (before and after the
+host codebook stop): a repeated batch layout is captured on its second
+call and replayed after that.  Every call's archives are compared with the
+eager path's on the same data, with eager work and synchronisations between
+the replays; new field data and new output buffers in the same layout
+replay the same graphs; a changed layout runs eagerly; the oracle pins the
+eager archives."""
 import numpy as np
 import pytest
 

@@ -98,6 +98,19 @@ struct Stage {
 
 }  // namespace
 
+// A captured sequence of stream work and the arguments it was captured with
+// (key); seen: the last key that ran eagerly.
+struct GraphSlot {
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> key, seen;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+        seen.clear();
+    }
+};
+
 struct vecsz_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -124,12 +137,10 @@ struct vecsz_ctx {
     DevBuf b_desc, b_start, b_hist, b_range, b_ebd, b_sums, b_errs, b_book, b_head, b_rows;
     HostBuf b_pin, b_pin_hist;
     std::vector<Codebook> b_cb;
-    // the launches after the codebooks, captured as one HIP graph when a
-    // batch repeats and replayed while every argument they take stays the
-    // same (b_key); b_seen: the last key run eagerly
+    // the fused compress batch's work before and after its codebook stop,
+    // each captured as one HIP graph when a batch layout repeats
     bool batch_graph = true;
-    hipGraphExec_t b_graph = nullptr;
-    std::vector<uint64_t> b_key, b_seen;
+    GraphSlot b_graph[2];
     // fused batch decompress: field descriptors, unit starts, head copies,
     // gathered heads, summaries, symbol totals, fallback counts; pinned
     // staging of the same; decode tables per field
@@ -1250,7 +1261,7 @@ int vecsz_ctx_destroy(vecsz_ctx* c) {
     c->workers.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
+    for (GraphSlot& g : c->b_graph) g.reset();
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1458,7 +1469,7 @@ static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, 
 // chunk of the batch, per-field sizes and the bit packing.  The archives are
 // the ones the single-field path writes.
 // Issues `enqueue` (asynchronous copies and launches on c->stream only)
-// eagerly, or through the context's graph.  `key` lists every argument the
+// eagerly, or through the graph in slot g.  `key` lists every argument the
 // enqueued calls take.  A key seen for the first time runs eagerly; seen
 // again right after, the calls are captured (relaxed mode: other threads'
 // calls are not affected) and the graph launched; from then on the graph is
@@ -1466,35 +1477,35 @@ static int run_batch(vecsz_ctx* c, int nf, const uint64_t* weight, int* status, 
 // call therefore never pay for a capture.  Timed calls, and a stream that
 // cannot be captured, run eagerly.
 template <class F>
-static hipError_t run_captured(vecsz_ctx* c, const std::vector<uint64_t>& key, F&& enqueue) {
+static hipError_t run_captured(vecsz_ctx* c, GraphSlot& g, const std::vector<uint64_t>& key, F&& enqueue) {
     if (!c->batch_graph || c->timing) return enqueue();
-    if (c->b_graph && key == c->b_key) return hipGraphLaunch(c->b_graph, c->stream);
-    if (key != c->b_seen) {
-        c->b_seen = key;
+    if (g.exec && key == g.key) return hipGraphLaunch(g.exec, c->stream);
+    if (key != g.seen) {
+        g.seen = key;
         return enqueue();
     }
-    if (c->b_graph) {
-        (void)hipGraphExecDestroy(c->b_graph);
-        c->b_graph = nullptr;
-        c->b_key.clear();
+    if (g.exec) {
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.key.clear();
     }
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
         (void)hipGetLastError();
         return enqueue();
     }
     const hipError_t e = enqueue();
-    hipGraph_t g = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (e != hipSuccess || e2 != hipSuccess || !g) {
-        if (g) (void)hipGraphDestroy(g);
+    hipGraph_t gr = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+    if (e != hipSuccess || e2 != hipSuccess || !gr) {
+        if (gr) (void)hipGraphDestroy(gr);
         return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : hipErrorUnknown);
     }
     hipGraphExec_t x = nullptr;
-    const hipError_t e3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    const hipError_t e3 = hipGraphInstantiate(&x, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
     if (e3 != hipSuccess) return e3;
-    c->b_graph = x;
-    c->b_key = key;
+    g.exec = x;
+    g.key = key;
     return hipGraphLaunch(x, c->stream);
 }
 
@@ -1644,42 +1655,63 @@ static int compress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* p,
     memcpy(c->b_pin.p, fd.data(), desc_bytes);
     memcpy(c->b_pin.as<uint8_t>() + desc_bytes, st.data(), start_bytes);
     memcpy(c->b_pin.as<uint8_t>() + pin_fixed, rowtab.data(), rowtab.size() * 4);
-    HIPCHK(c, hipMemcpyAsync(c->b_desc.p, c->b_pin.p, desc_bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b_start.p, c->b_pin.as<uint8_t>() + desc_bytes, start_bytes, hipMemcpyHostToDevice,
-                             c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->b_rows.p, c->b_pin.as<uint8_t>() + pin_fixed, rowtab.size() * 4,
-                             hipMemcpyHostToDevice, c->stream));
     const uint64_t* rg0_d = c->b_start.as<uint64_t>();
     const uint64_t* ck0_d = rg0_d + (nf + 1);
     const uint64_t* dqk_d = rg0_d + 2 * (nf + 1);
-    {
-        StageScope s(c, "range");
-        launch_range_multi(fdd, nf, rg0_d, rg0[nf], c->stream);
-    }
-    HIPCHK(c, hipMemsetAsync(c->b_hist.p, 0, (size_t)nf * dict * 8, c->stream));
-    static const char* const kDqStage[kDqKinds] = {"dualquant_1d", "dualquant_2d",     "dualquant_2d_x4",
-                                                   "dualquant_3d", "dualquant_3d_x4", "dualquant_2d_x4u",
-                                                   "dualquant_3d_x4u"};
-    for (int kind = 0; kind < kDqKinds; kind++) {
-        const uint64_t nu = dqk[(size_t)kind * (nf + 1) + nf];
-        if (!nu) continue;
-        StageScope s(c, kDqStage[kind]);
-        launch_dualquant_multi(B, kind, fdd, nf, dqk_d + (size_t)kind * (nf + 1), nu, c->dq_grid, c->stream);
-    }
-    {
-        StageScope s(c, "histogram");
-        const uint32_t* rows_d = c->b_rows.as<uint32_t>();
-        launch_hist_multi(fdd, nf, ck0_d, U, dict, c->partials.as<uint32_t>(), rows_d, rows_d + G, ngroups,
-                          c->b_hist.as<unsigned long long>(), c->stream);
-    }
-    HIPCHK(c, hipGetLastError());
     const size_t hist_bytes = (size_t)nf * dict * 8;
     if (!c->b_pin_hist.ensure(hist_bytes + (size_t)nf * sizeof(EbDev) + 64))
         return fail(c, VECSZ_ENOMEM, "pinned buffer");
     uint8_t* hp = c->b_pin_hist.as<uint8_t>();
-    HIPCHK(c, hipMemcpyAsync(hp, c->b_hist.p, hist_bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hp + hist_bytes, c->b_ebd.p, (size_t)nf * sizeof(EbDev), hipMemcpyDeviceToHost,
-                             c->stream));
+    // uploads, range, dual quantization, histograms and the copy down: what
+    // the graph before the stop holds
+    auto enqueue_pre = [&]() -> hipError_t {
+        hipError_t e;
+#define ENQ(expr) \
+    if ((e = (expr)) != hipSuccess) return e
+        ENQ(hipMemcpyAsync(c->b_desc.p, c->b_pin.p, desc_bytes, hipMemcpyHostToDevice, c->stream));
+        ENQ(hipMemcpyAsync(c->b_start.p, c->b_pin.as<uint8_t>() + desc_bytes, start_bytes, hipMemcpyHostToDevice,
+                           c->stream));
+        ENQ(hipMemcpyAsync(c->b_rows.p, c->b_pin.as<uint8_t>() + pin_fixed, rowtab.size() * 4, hipMemcpyHostToDevice,
+                           c->stream));
+        {
+            StageScope s(c, "range");
+            launch_range_multi(fdd, nf, rg0_d, rg0[nf], c->stream);
+        }
+        // (a kernel, not hipMemsetAsync: this may be captured)
+        launch_zero_u32(reinterpret_cast<uint32_t*>(c->b_hist.p), (uint64_t)nf * dict * 2, c->stream);
+        static const char* const kDqStage[kDqKinds] = {"dualquant_1d", "dualquant_2d",     "dualquant_2d_x4",
+                                                       "dualquant_3d", "dualquant_3d_x4", "dualquant_2d_x4u",
+                                                       "dualquant_3d_x4u"};
+        for (int kind = 0; kind < kDqKinds; kind++) {
+            const uint64_t nu = dqk[(size_t)kind * (nf + 1) + nf];
+            if (!nu) continue;
+            StageScope s(c, kDqStage[kind]);
+            launch_dualquant_multi(B, kind, fdd, nf, dqk_d + (size_t)kind * (nf + 1), nu, c->dq_grid, c->stream);
+        }
+        {
+            StageScope s(c, "histogram");
+            const uint32_t* rows_d = c->b_rows.as<uint32_t>();
+            launch_hist_multi(fdd, nf, ck0_d, U, dict, c->partials.as<uint32_t>(), rows_d, rows_d + G, ngroups,
+                              c->b_hist.as<unsigned long long>(), c->stream);
+        }
+        ENQ(hipGetLastError());
+        ENQ(hipMemcpyAsync(hp, c->b_hist.p, hist_bytes, hipMemcpyDeviceToHost, c->stream));
+        ENQ(hipMemcpyAsync(hp + hist_bytes, c->b_ebd.p, (size_t)nf * sizeof(EbDev), hipMemcpyDeviceToHost,
+                           c->stream));
+#undef ENQ
+        return hipSuccess;
+    };
+    {
+        auto ptr = [](const void* q) { return (uint64_t)(uintptr_t)q; };
+        std::vector<uint64_t> key = {ptr(c->stream),  ptr(c->b_desc.p),   ptr(c->b_start.p), ptr(c->b_rows.p),
+                                     ptr(c->b_hist.p), ptr(c->b_ebd.p),    ptr(c->partials.p), ptr(c->b_pin.p),
+                                     ptr(c->b_pin_hist.p), (uint64_t)nf,  (uint64_t)dict,    (uint64_t)B,
+                                     desc_bytes,       start_bytes,        pin_fixed,         rowtab.size(),
+                                     rg0[nf],          U,                  G,                 ngroups,
+                                     (uint64_t)c->dq_grid};
+        for (int kind = 0; kind < kDqKinds; kind++) key.push_back(dqk[(size_t)kind * (nf + 1) + nf]);
+        HIPCHK(c, run_captured(c, c->b_graph[0], key, enqueue_pre));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
 
     // codebooks on host threads
@@ -1825,7 +1857,7 @@ static int compress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* p,
             ptr(c->scan_tmp.p),    ptr(c->scan_tot.p),   ptr(c->b_sums.p),  ptr(c->b_pin.p),   ptr(c->b_pin_hist.p),
             (uint64_t)nf,          (uint64_t)dict,       (uint64_t)max_len, pin_fixed,         desc_bytes,
             books_bytes,           head_tot,             ck0[nf],           ctot,              (uint64_t)c->enc_grid};
-        HIPCHK(c, run_captured(c, key, enqueue));
+        HIPCHK(c, run_captured(c, c->b_graph[1], key, enqueue));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int k = 0; k < nf; k++) {
             if (status[k]) continue;
@@ -2253,12 +2285,10 @@ int vecsz_ctx_set_fused_encode(vecsz_ctx* c, int on) {
 int vecsz_ctx_set_batch_graph(vecsz_ctx* c, int on) {
     if (!c) return VECSZ_EINVAL;
     c->batch_graph = on != 0;
-    if (!on && c->b_graph) {
+    if (!on) {
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
-        (void)hipGraphExecDestroy(c->b_graph);
-        c->b_graph = nullptr;
-        c->b_key.clear();
+        for (GraphSlot& g : c->b_graph) g.reset();
     }
     return VECSZ_OK;
 }

@@ -318,8 +318,8 @@ class Context:
 
     def set_batch_graph(self, on: bool):
         """True (default): a fused compress batch whose layout repeats runs
-        its post-codebook launches as one captured HIP graph; False: always
-        eagerly.  Identical archives."""
+        as two captured HIP graphs (before and after its host codebook
+        stop); False: always eagerly.  Identical archives."""
         self._check(self._lib.vecsz_ctx_set_batch_graph(self._ctx, 1 if on else 0))
 
     def compress_bound(self, params: Params) -> int:
