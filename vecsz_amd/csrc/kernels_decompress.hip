@@ -603,6 +603,36 @@ __device__ __forceinline__ float read_f32_bytes(const uint8_t* p) {
     return __uint_as_float(v);
 }
 
+// The stream span of workgroup w's segments [256 w, 256 w + 256) in the
+// indexed decode: from the first segment's first codeword to the next
+// workgroup's (the stream end for the last), as aligned-space words w_lo ..
+// w_lo + nwords (plus kIdxSpanPad words the last codewords' windows read).
+// fits: the span is staged in LDS by k_decode_indexed_staged (a span that
+// runs backwards, from a damaged index, never fits).
+constexpr uint32_t kIdxSpanWords = 9984;  // stage capacity: ~9.75 bits per symbol over 256 segments
+constexpr uint32_t kIdxSpanPad = 6;
+constexpr uint32_t kIdxSpanOver = 264;    // a damaged stream: 128 codewords of up to 64 bits past the span, and the windows past those
+struct IdxSpan {
+    uint64_t bs, be;  // stream bits of the span
+    uint64_t w_lo, nwords;
+    bool fits;
+};
+__device__ __forceinline__ IdxSpan idx_span(const uint64_t* __restrict__ index, const IdxLayout& L,
+                                            uint64_t total_bits, const StreamView& sv, uint64_t w) {
+    const uint64_t* grp = index + L.grp;
+    const uint32_t* rel_bits = reinterpret_cast<const uint32_t*>(index + L.rel_bits);
+    const uint64_t g0 = w * 256, g1 = min<uint64_t>(g0 + 256, L.nseg);
+    const uint64_t bs = grp[2 * (g0 / kIdxGroup)] + rel_bits[g0];
+    const uint64_t be = g1 < L.nseg ? grp[2 * (g1 / kIdxGroup)] + rel_bits[g1] : total_bits;
+    IdxSpan sp;
+    sp.bs = bs;
+    sp.be = be;
+    sp.w_lo = (bs + sv.shift0) >> 5;
+    sp.nwords = be >= bs ? ((be + sv.shift0) >> 5) - sp.w_lo + kIdxSpanPad : 0;
+    sp.fits = be >= bs && sp.nwords <= kIdxSpanWords;
+    return sp;
+}
+
 // Window over the stream for the indexed decode: 64 bits held in a register
 // (relative bit b0 at its MSB, refilled a word at a time by the caller);
 // windows beyond it (the deeper levels of codewords over 32 bits) are read
@@ -639,7 +669,13 @@ constexpr uint32_t kIdxStageWords = 64 * kIdxLaneWords;  // per wavefront
 template <bool T16, bool R1D>
 __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ index,
                                                         IdxLayout L, uint64_t total_bits, uint16_t* __restrict__ codes,
-                                                        float* __restrict__ sparse, Recon1D r, DecSummary* sum) {
+                                                        float* __restrict__ sparse, Recon1D r, DecSummary* sum,
+                                                        bool skip_staged) {
+    if (skip_staged) {  // workgroups k_decode_indexed_staged has decoded (the whole workgroup)
+        StreamView sv0;
+        sv0.setup(a.stream, a.stream_bits);
+        if (idx_span(index, L, total_bits, sv0, blockIdx.x).fits) return;
+    }
     // symbols per lane between flushes (16 words of stage either way) and
     // per 16-byte piece of output
     constexpr uint32_t kFlush = R1D ? 16 : kIdxFlush;
@@ -828,6 +864,100 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
     // every segment ends where the next starts, the last at the stream's
     // last codeword bit
     if (live) bad |= 32 * cw.w0 + lp - sv.shift0 != end;
+    if (bad) atomicOr(&sum->error, 1u);
+}
+
+// Decode with a decode index, the bits staged in LDS: the workgroup's 256
+// segments are one contiguous span of the stream, staged with coalesced
+// loads, and thread g decodes segment g out of LDS as the write pass decodes
+// a subsequence.  A segment holds kIdxSeg = 128 symbols, 16 whole 16-byte
+// units, so every thread writes its own units, in groups of four held in
+// registers and stored back to back.  Workgroups whose span does not fit
+// the stage return at once and leave their segments to k_decode_indexed.
+// Deeper decode tables are read in global memory (the stage needs the LDS).
+template <bool T16>
+__global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, const uint64_t* __restrict__ index,
+                                                               IdxLayout L, uint64_t total_bits,
+                                                               uint16_t* __restrict__ codes,
+                                                               float* __restrict__ sparse, DecSummary* sum) {
+    StreamView sv;
+    sv.setup(a.stream, a.stream_bits);
+    const IdxSpan sp = idx_span(index, L, total_bits, sv, blockIdx.x);
+    if (!sp.fits) return;  // (the whole workgroup)
+    extern __shared__ uint32_t lds[];
+    Decoder<T16> dec;
+    a.n_sub16 = 0;
+    uint32_t* stage = lds + dec.load(lds, a);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)sp.nwords; i += 256) stage[i] = sv.word(sp.w_lo + i);
+    lds_sync();
+    const uint64_t g = blockIdx.x * 256ull + threadIdx.x;
+    if (g >= L.nseg) return;  // no barrier follows
+    const uint64_t* grp = index + L.grp;
+    const uint32_t* rel_bits = reinterpret_cast<const uint32_t*>(index + L.rel_bits);
+    const uint16_t* rel_outs = reinterpret_cast<const uint16_t*>(index + L.rel_outs);
+    const uint64_t start = grp[2 * (g / kIdxGroup)] + rel_bits[g];
+    const uint64_t end = g + 1 < L.nseg ? grp[2 * ((g + 1) / kIdxGroup)] + rel_bits[g + 1] : total_bits;
+    uint64_t rank = grp[2 * (g / kIdxGroup) + 1] + rel_outs[g];
+    const uint64_t s0 = g * kIdxSeg;
+    const uint64_t span_lo = 32 * sp.w_lo;  // aligned-space bit of stage word 0
+    // a segment outside the span (a damaged index) is an error, not decoded
+    bool bad = start > end || end > a.stream_bits || start < sp.bs || end > sp.be;
+    const uint32_t ns = bad ? 0u : (uint32_t)min<uint64_t>(kIdxSeg, a.n - s0);
+    const RowWin win{stage};
+    uint32_t lp = bad ? 0u : (uint32_t)(start + sv.shift0 - span_lo);
+    // the loops count symbols, so an invalid codeword (len 0, an error)
+    // needs no forced progress; a damaged stream stays within kIdxSpanOver
+    // words past the span
+    auto next = [&]() -> uint32_t {
+        uint32_t len;
+        const uint32_t sym = dec.take_w(win, lp, len);
+        bad |= len == 0;
+        lp += len;
+        return sym;
+    };
+    auto outlier = [&](uint64_t i) {
+        if (rank < a.n_outlier)
+            sparse[i] = read_f32_bytes(a.outliers + 4 * rank);
+        else
+            bad = true;
+        rank++;
+    };
+    const uint32_t nfull = ns & ~7u;  // whole units
+    for (uint32_t G = 0; G < nfull; G += 32) {
+        // outliers are rare: one test per group, then in symbol order
+        uint32_t w[16], omask = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool mine = G + 8 * u < nfull;  // differs across lanes only for the field's last segment
+#pragma unroll
+            for (int m = 0; m < 4; m++) w[4 * u + m] = 0;
+            if (mine) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    const uint32_t lo = next();
+                    const uint32_t hi = next();
+                    omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (8 * u + 2 * m);
+                    w[4 * u + m] = lo | (hi << 16);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (G + 8 * u < nfull)
+                *reinterpret_cast<uint4*>(codes + s0 + G + 8 * u) =
+                    make_uint4(w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]);
+        if (omask) {
+            for (int k = 0; k < 32; k++)
+                if ((omask >> k) & 1) outlier(s0 + G + k);
+        }
+    }
+    for (uint32_t i = nfull; i < ns; i++) {  // the field's last, partial unit
+        const uint32_t sym = next();
+        codes[s0 + i] = (uint16_t)sym;
+        if (sym == 0) outlier(s0 + i);
+    }
+    // the segment ends where the next one starts
+    if (!bad) bad = span_lo + lp - sv.shift0 != end;
     if (bad) atomicOr(&sum->error, 1u);
 }
 
@@ -1656,13 +1786,30 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     allow_big_lds((k_decode_indexed<false, false>));
     allow_big_lds((k_decode_indexed<true, true>));
     allow_big_lds((k_decode_indexed<false, true>));
+    allow_big_lds(k_decode_indexed_staged<true>);
+    allow_big_lds(k_decode_indexed_staged<false>);
     const bool t16 = a.lut16 != nullptr;
     const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
     const IdxLayout L = idx_layout(a.n);
     const dim3 grid((unsigned)std::max<uint64_t>(1, (L.nseg + 255) / 256));
     const Recon1D r = r1d ? *r1d : Recon1D{};
-#define DI(T, R) \
-    hipLaunchKernelGGL((k_decode_indexed<T, R>), grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse, r, sum)
+    // codes: workgroups whose span fits the LDS stage first, then the others
+    // through the register-window kernel (which skips the staged ones)
+    const bool staged = r1d == nullptr;
+    if (staged) {
+        DecodeArgs a0 = a;
+        a0.n_sub16 = 0;
+        const size_t lds_s = table_lds_bytes(a0, t16) + (size_t)(kIdxSpanWords + kIdxSpanOver) * 4;
+        if (t16)
+            hipLaunchKernelGGL(k_decode_indexed_staged<true>, grid, dim3(256), lds_s, st, a, index, L, total_bits,
+                               codes, sparse, sum);
+        else
+            hipLaunchKernelGGL(k_decode_indexed_staged<false>, grid, dim3(256), lds_s, st, a, index, L, total_bits,
+                               codes, sparse, sum);
+    }
+#define DI(T, R)                                                                                                   \
+    hipLaunchKernelGGL((k_decode_indexed<T, R>), grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse, r, \
+                       sum, staged)
     if (r1d) {
         if (t16)
             DI(true, true);
