@@ -651,7 +651,9 @@ struct CursorWin {
     }
 };
 
-// Decode with a decode index: thread g decodes segment g (kIdxSeg symbols)
+// Decode with a decode index, for the workgroups whose span does not fit the
+// LDS stage of k_decode_indexed_staged (it returns at once for the others):
+// thread g decodes segment g (kIdxSeg symbols)
 // from the segment's first codeword, reading the stream through a 64-bit
 // register window refilled from 16-byte groups loaded ahead, so no LDS
 // staging of the bits and no synchronisation is needed.  Codes leave through
@@ -669,9 +671,8 @@ constexpr uint32_t kIdxStageWords = 64 * kIdxLaneWords;  // per wavefront
 template <bool T16, bool R1D>
 __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint64_t* __restrict__ index,
                                                         IdxLayout L, uint64_t total_bits, uint16_t* __restrict__ codes,
-                                                        float* __restrict__ sparse, Recon1D r, DecSummary* sum,
-                                                        bool skip_staged) {
-    if (skip_staged) {  // workgroups k_decode_indexed_staged has decoded (the whole workgroup)
+                                                        float* __restrict__ sparse, Recon1D r, DecSummary* sum) {
+    {  // workgroups k_decode_indexed_staged has decoded (the whole workgroup)
         StreamView sv0;
         sv0.setup(a.stream, a.stream_bits);
         if (idx_span(index, L, total_bits, sv0, blockIdx.x).fits) return;
@@ -872,14 +873,17 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
 // loads, and thread g decodes segment g out of LDS as the write pass decodes
 // a subsequence.  A segment holds kIdxSeg = 128 symbols, 16 whole 16-byte
 // units, so every thread writes its own units, in groups of four held in
-// registers and stored back to back.  Workgroups whose span does not fit
-// the stage return at once and leave their segments to k_decode_indexed.
-// Deeper decode tables are read in global memory (the stage needs the LDS).
-template <bool T16>
+// registers and stored back to back (R1D: the 1D reconstruction in the
+// same pass, a segment holding whole blocks for every B, writing floats in
+// groups of 16).  Workgroups whose span does not fit the stage return at
+// once and leave their segments to k_decode_indexed.  Deeper decode tables
+// are read in global memory (the stage needs the LDS).
+template <bool T16, bool R1D>
 __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, const uint64_t* __restrict__ index,
                                                                IdxLayout L, uint64_t total_bits,
                                                                uint16_t* __restrict__ codes,
-                                                               float* __restrict__ sparse, DecSummary* sum) {
+                                                               float* __restrict__ sparse, Recon1D r,
+                                                               DecSummary* sum) {
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     const IdxSpan sp = idx_span(index, L, total_bits, sv, blockIdx.x);
@@ -922,39 +926,93 @@ __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, con
             bad = true;
         rank++;
     };
-    const uint32_t nfull = ns & ~7u;  // whole units
-    for (uint32_t G = 0; G < nfull; G += 32) {
-        // outliers are rare: one test per group, then in symbol order
-        uint32_t w[16], omask = 0;
+    if constexpr (R1D) {
+        // x_lorenzo_1d1l (src/dualquant.hh:915-931), as in k_decode_indexed
+        float sprev = 0.f;
+        auto value = [&](uint32_t c, float ov, uint64_t x) -> float {
+            if ((x % (uint64_t)r.B) == 0) sprev = r.halo ? r.halo[r.halo_stride * (x / r.B) + 2] : 0.f;
+            const float sx = c == 0 ? ov : sprev + (float)((int32_t)c - r.radius);
+            sprev = sx;
+            return (float)((double)sx * r.twoeb);
+        };
+        const uint32_t nfull16 = ns & ~15u;
+        for (uint32_t G = 0; G < nfull16; G += 16) {
+            uint32_t c[16], omask = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const bool mine = G + 8 * u < nfull;  // differs across lanes only for the field's last segment
+            for (int k = 0; k < 16; k++) {
+                c[k] = next();
+                omask |= (uint32_t)(c[k] == 0) << k;
+            }
+            float ov[16];
 #pragma unroll
-            for (int m = 0; m < 4; m++) w[4 * u + m] = 0;
-            if (mine) {
+            for (int k = 0; k < 16; k++) ov[k] = 0.f;
+            if (omask) {
 #pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    const uint32_t lo = next();
-                    const uint32_t hi = next();
-                    omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (8 * u + 2 * m);
-                    w[4 * u + m] = lo | (hi << 16);
+                for (int k = 0; k < 16; k++) {
+                    if ((omask >> k) & 1) {
+                        if (rank < a.n_outlier)
+                            ov[k] = read_f32_bytes(a.outliers + 4 * rank);
+                        else
+                            bad = true;
+                        rank++;
+                    }
                 }
             }
-        }
+            float o[16];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (G + 8 * u < nfull)
-                *reinterpret_cast<uint4*>(codes + s0 + G + 8 * u) =
-                    make_uint4(w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]);
-        if (omask) {
-            for (int k = 0; k < 32; k++)
-                if ((omask >> k) & 1) outlier(s0 + G + k);
+            for (int k = 0; k < 16; k++) o[k] = value(c[k], ov[k], s0 + G + k);
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                *reinterpret_cast<float4*>(r.out + s0 + G + 4 * q) =
+                    make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
         }
-    }
-    for (uint32_t i = nfull; i < ns; i++) {  // the field's last, partial unit
-        const uint32_t sym = next();
-        codes[s0 + i] = (uint16_t)sym;
-        if (sym == 0) outlier(s0 + i);
+        for (uint32_t i = nfull16; i < ns; i++) {  // the field's last, partial group
+            const uint32_t c = next();
+            float ov = 0.f;
+            if (c == 0) {
+                if (rank < a.n_outlier)
+                    ov = read_f32_bytes(a.outliers + 4 * rank);
+                else
+                    bad = true;
+                rank++;
+            }
+            r.out[s0 + i] = value(c, ov, s0 + i);
+        }
+    } else {
+        const uint32_t nfull = ns & ~7u;  // whole units
+        for (uint32_t G = 0; G < nfull; G += 32) {
+            // outliers are rare: one test per group, then in symbol order
+            uint32_t w[16], omask = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool mine = G + 8 * u < nfull;  // differs across lanes only for the field's last segment
+#pragma unroll
+                for (int m = 0; m < 4; m++) w[4 * u + m] = 0;
+                if (mine) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const uint32_t lo = next();
+                        const uint32_t hi = next();
+                        omask |= ((uint32_t)(lo == 0) | ((uint32_t)(hi == 0) << 1)) << (8 * u + 2 * m);
+                        w[4 * u + m] = lo | (hi << 16);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (G + 8 * u < nfull)
+                    *reinterpret_cast<uint4*>(codes + s0 + G + 8 * u) =
+                        make_uint4(w[4 * u], w[4 * u + 1], w[4 * u + 2], w[4 * u + 3]);
+            if (omask) {
+                for (int k = 0; k < 32; k++)
+                    if ((omask >> k) & 1) outlier(s0 + G + k);
+            }
+        }
+        for (uint32_t i = nfull; i < ns; i++) {  // the field's last, partial unit
+            const uint32_t sym = next();
+            codes[s0 + i] = (uint16_t)sym;
+            if (sym == 0) outlier(s0 + i);
+        }
     }
     // the segment ends where the next one starts
     if (!bad) bad = span_lo + lp - sv.shift0 != end;
@@ -1786,30 +1844,39 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     allow_big_lds((k_decode_indexed<false, false>));
     allow_big_lds((k_decode_indexed<true, true>));
     allow_big_lds((k_decode_indexed<false, true>));
-    allow_big_lds(k_decode_indexed_staged<true>);
-    allow_big_lds(k_decode_indexed_staged<false>);
+    allow_big_lds((k_decode_indexed_staged<true, false>));
+    allow_big_lds((k_decode_indexed_staged<false, false>));
+    allow_big_lds((k_decode_indexed_staged<true, true>));
+    allow_big_lds((k_decode_indexed_staged<false, true>));
     const bool t16 = a.lut16 != nullptr;
     const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
     const IdxLayout L = idx_layout(a.n);
     const dim3 grid((unsigned)std::max<uint64_t>(1, (L.nseg + 255) / 256));
     const Recon1D r = r1d ? *r1d : Recon1D{};
-    // codes: workgroups whose span fits the LDS stage first, then the others
-    // through the register-window kernel (which skips the staged ones)
-    const bool staged = r1d == nullptr;
-    if (staged) {
+    // workgroups whose span fits the LDS stage first, then the others through
+    // the register-window kernel (which skips the staged ones)
+    {
         DecodeArgs a0 = a;
         a0.n_sub16 = 0;
         const size_t lds_s = table_lds_bytes(a0, t16) + (size_t)(kIdxSpanWords + kIdxSpanOver) * 4;
-        if (t16)
-            hipLaunchKernelGGL(k_decode_indexed_staged<true>, grid, dim3(256), lds_s, st, a, index, L, total_bits,
-                               codes, sparse, sum);
-        else
-            hipLaunchKernelGGL(k_decode_indexed_staged<false>, grid, dim3(256), lds_s, st, a, index, L, total_bits,
-                               codes, sparse, sum);
+#define DS(T, R)                                                                                                   \
+    hipLaunchKernelGGL((k_decode_indexed_staged<T, R>), grid, dim3(256), lds_s, st, a, index, L, total_bits, codes, \
+                       sparse, r, sum)
+        if (r1d) {
+            if (t16)
+                DS(true, true);
+            else
+                DS(false, true);
+        } else {
+            if (t16)
+                DS(true, false);
+            else
+                DS(false, false);
+        }
+#undef DS
     }
-#define DI(T, R)                                                                                                   \
-    hipLaunchKernelGGL((k_decode_indexed<T, R>), grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse, r, \
-                       sum, staged)
+#define DI(T, R) \
+    hipLaunchKernelGGL((k_decode_indexed<T, R>), grid, dim3(256), lds, st, a, index, L, total_bits, codes, sparse, r, sum)
     if (r1d) {
         if (t16)
             DI(true, true);
