@@ -2009,6 +2009,7 @@ static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* 
         if (dt.lut16.empty()) t16 = false;
         if (dt.sub16.size() <= kMaxSub16) max_sub16 = std::max<uint32_t>(max_sub16, (uint32_t)dt.sub16.size());
     }
+    max_sub16 = batch_sub16_in_lds(max_sub16);
     std::vector<uint64_t> off(nf, 0), toff(nf, 0), fl_off(nf, 0);
     uint64_t ntot = 0, tab_tot = 0, nsub_tot = 0, fl_tot = 0;
     const size_t ns = (size_t)nf + 1;
@@ -2131,7 +2132,7 @@ static int decompress_batch_fused_run(vecsz_ctx* c, int nf, const vecsz_params* 
             memcpy(tp + dt.lut16.size() * 2, dt.sub16.data(), dt.sub16.size() * 2);
             F.a.lut16 = reinterpret_cast<const uint16_t*>(tdev);
             F.a.sub16 = reinterpret_cast<const uint16_t*>(tdev) + dt.lut16.size();
-            F.a.n_sub16 = dt.sub16.size() <= kMaxSub16 ? (uint32_t)dt.sub16.size() : 0;
+            F.a.n_sub16 = max_sub16 && dt.sub16.size() <= max_sub16 ? (uint32_t)dt.sub16.size() : 0;
         } else {
             memcpy(tp, dt.lut.data(), dt.lut.size() * 4);
             memcpy(tp + dt.lut.size() * 4, dt.sub.data(), dt.sub.size() * 4);

@@ -197,6 +197,10 @@ struct DecField {
 };
 // t16: every field has compact tables; max_sub16: the largest n_sub16.
 // start: per-field unit starts (nf + 1), nunits = start[nf].
+// The largest deeper-table size of a batch's fields (entries), or 0 when
+// keeping tables that size in LDS would cost the batch's sync pass a
+// resident workgroup per CU (the fields then read them in global memory).
+uint32_t batch_sub16_in_lds(uint32_t max_sub16);
 void launch_decode_sync_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
                               uint64_t nunits, hipStream_t st);
 // sub_start: batch-wide number of each field's subsequence 0 (nf + 1 entries);

@@ -876,8 +876,7 @@ __global__ void __launch_bounds__(256) k_decode_indexed(DecodeArgs a, const uint
 // registers and stored back to back (R1D: the 1D reconstruction in the
 // same pass, a segment holding whole blocks for every B, writing floats in
 // groups of 16).  Workgroups whose span does not fit the stage return at
-// once and leave their segments to k_decode_indexed.  Deeper decode tables
-// are read in global memory (the stage needs the LDS).
+// once and leave their segments to k_decode_indexed.
 template <bool T16, bool R1D>
 __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, const uint64_t* __restrict__ index,
                                                                IdxLayout L, uint64_t total_bits,
@@ -890,7 +889,6 @@ __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, con
     if (!sp.fits) return;  // (the whole workgroup)
     extern __shared__ uint32_t lds[];
     Decoder<T16> dec;
-    a.n_sub16 = 0;
     uint32_t* stage = lds + dec.load(lds, a);
     for (uint32_t i = threadIdx.x; i < (uint32_t)sp.nwords; i += 256) stage[i] = sv.word(sp.w_lo + i);
     lds_sync();
@@ -1799,6 +1797,23 @@ static size_t table_lds_bytes(const DecodeArgs& a, bool t16) {
     return t16 ? (((size_t)(1u << kLutBits) + a.n_sub16 + 1) / 2) * 4 : (size_t)(1u << kLutBits) * 4;
 }
 
+// The deeper 16-bit tables serve only the codewords over kLutBits bits (a
+// few percent of them at eb 1e-4), so a kernel copies them to LDS only when
+// that costs no resident workgroup per CU, given its other LDS (own_bytes
+// besides the first-level table); otherwise it reads them in global memory
+// (n_sub16 0).  Measured on the 1 GiB field, whose deeper tables took 22 KB:
+// the write pass 0.75 -> 0.53 ms, the sync pass 1.08 -> 1.04 ms in global
+// memory.
+static DecodeArgs with_lds_sub16(DecodeArgs a, size_t own_bytes) {
+    if (!a.lut16 || !a.n_sub16) return a;
+    constexpr size_t kCuLds = 160 * 1024;
+    DecodeArgs b = a;
+    b.n_sub16 = 0;
+    const size_t without = table_lds_bytes(b, true) + own_bytes, with = table_lds_bytes(a, true) + own_bytes;
+    if (kCuLds / with < kCuLds / without) a.n_sub16 = 0;
+    return a;
+}
+
 // Lets a kernel take more than the default 64 KiB of dynamic LDS (gfx950
 // has 160 KiB per workgroup; the kernels here have no static LDS).  Set once
 // per kernel: each call site keeps its own once-flag (kernels of one
@@ -1815,10 +1830,11 @@ static bool set_big_lds(const void* kernel) {
         (void)once_;                                                                     \
     } while (0)
 
-void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStream_t st) {
+void launch_decode_sync(const DecodeArgs& a0, SubState s, DecSummary* sum, hipStream_t st) {
     allow_big_lds(k_decode_sync<true>);
     allow_big_lds(k_decode_sync<false>);
-    const bool t16 = a.lut16 != nullptr;
+    const bool t16 = a0.lut16 != nullptr;
+    const DecodeArgs a = with_lds_sub16(a0, (size_t)kSyncLdsWords * 4);
     const size_t lds = table_lds_bytes(a, t16) + (size_t)kSyncLdsWords * 4;
     const dim3 grid(grid_for(a.nsub, kSyncStride));
     if (t16)
@@ -1826,10 +1842,11 @@ void launch_decode_sync(const DecodeArgs& a, SubState s, DecSummary* sum, hipStr
     else
         hipLaunchKernelGGL(k_decode_sync<false>, grid, dim3(kSyncWg), lds, st, a, s, sum);
 }
-void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSummary* sum, hipStream_t st) {
+void launch_decode_pass(const DecodeArgs& a0, SubState s, uint32_t pass, DecSummary* sum, hipStream_t st) {
     allow_big_lds(k_decode_pass<true>);
     allow_big_lds(k_decode_pass<false>);
-    const bool t16 = a.lut16 != nullptr;
+    const bool t16 = a0.lut16 != nullptr;
+    const DecodeArgs a = with_lds_sub16(a0, (size_t)256 * kRowStride * 4);
     const size_t lds = table_lds_bytes(a, t16) + (size_t)256 * kRowStride * 4;
     // persistent workgroups; those past the listed count return at once
     const dim3 grid((unsigned)std::min<uint64_t>(kPassGrid, std::max<uint64_t>(1, grid_for(a.nsub, 256))));
@@ -1838,7 +1855,7 @@ void launch_decode_pass(const DecodeArgs& a, SubState s, uint32_t pass, DecSumma
     else
         hipLaunchKernelGGL(k_decode_pass<false>, grid, dim3(256), lds, st, a, s, pass, sum);
 }
-void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
+void launch_decode_indexed(const DecodeArgs& a0, const uint64_t* index, uint64_t total_bits, uint16_t* codes,
                            float* sparse, const Recon1D* r1d, DecSummary* sum, hipStream_t st) {
     allow_big_lds((k_decode_indexed<true, false>));
     allow_big_lds((k_decode_indexed<false, false>));
@@ -1848,7 +1865,8 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     allow_big_lds((k_decode_indexed_staged<false, false>));
     allow_big_lds((k_decode_indexed_staged<true, true>));
     allow_big_lds((k_decode_indexed_staged<false, true>));
-    const bool t16 = a.lut16 != nullptr;
+    const bool t16 = a0.lut16 != nullptr;
+    const DecodeArgs a = with_lds_sub16(a0, (size_t)4 * kIdxStageWords * 4);
     const size_t lds = table_lds_bytes(a, t16) + (size_t)4 * kIdxStageWords * 4;
     const IdxLayout L = idx_layout(a.n);
     const dim3 grid((unsigned)std::max<uint64_t>(1, (L.nseg + 255) / 256));
@@ -1856,11 +1874,10 @@ void launch_decode_indexed(const DecodeArgs& a, const uint64_t* index, uint64_t 
     // workgroups whose span fits the LDS stage first, then the others through
     // the register-window kernel (which skips the staged ones)
     {
-        DecodeArgs a0 = a;
-        a0.n_sub16 = 0;
-        const size_t lds_s = table_lds_bytes(a0, t16) + (size_t)(kIdxSpanWords + kIdxSpanOver) * 4;
-#define DS(T, R)                                                                                                   \
-    hipLaunchKernelGGL((k_decode_indexed_staged<T, R>), grid, dim3(256), lds_s, st, a, index, L, total_bits, codes, \
+        const DecodeArgs as = with_lds_sub16(a0, (size_t)(kIdxSpanWords + kIdxSpanOver) * 4);
+        const size_t lds_s = table_lds_bytes(as, t16) + (size_t)(kIdxSpanWords + kIdxSpanOver) * 4;
+#define DS(T, R)                                                                                                    \
+    hipLaunchKernelGGL((k_decode_indexed_staged<T, R>), grid, dim3(256), lds_s, st, as, index, L, total_bits, codes, \
                        sparse, r, sum)
         if (r1d) {
             if (t16)
@@ -1897,12 +1914,13 @@ void launch_scan_u32x2(const uint32_t* a, const uint32_t* b, uint64_t n, uint64_
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, block_tmp, nb, totals, guard);
     hipLaunchKernelGGL(k_scan_add, dim3(grid_for(n, 1024)), dim3(1024), 0, st, oa, ob, n, block_tmp, guard);
 }
-void launch_decode_write(const DecodeArgs& a, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
+void launch_decode_write(const DecodeArgs& a0, SubState s, const uint64_t* sym_off, const uint64_t* zero_off,
                          uint16_t* codes, float* sparse, DecSummary* sum, hipStream_t st,
                          const uint32_t* guard) {
     allow_big_lds(k_decode_write<true>);
     allow_big_lds(k_decode_write<false>);
-    const bool t16 = a.lut16 != nullptr;
+    const bool t16 = a0.lut16 != nullptr;
+    const DecodeArgs a = with_lds_sub16(a0, (size_t)kWriteStageLds * 4);
     const size_t lds = table_lds_bytes(a, t16) + (size_t)kWriteStageLds * 4;
     const dim3 grid(grid_for(a.nsub, kSyncWg));
     if (t16)
@@ -1996,6 +2014,13 @@ void launch_reconstruct(int ndim, int B, const uint16_t* codes, const float* spa
 void launch_reconstruct_fallback(int ndim, int B, const uint16_t* codes, const float* sparse, float* out,
                                  const ReconArgs& a, const uint32_t* flagged, const uint32_t* nflagged, hipStream_t st) {
     hipLaunchKernelGGL(k_recon_fallback, dim3(256), dim3(64), 0, st, ndim, B, codes, sparse, out, a, flagged, nflagged);
+}
+
+uint32_t batch_sub16_in_lds(uint32_t max_sub16) {
+    DecodeArgs probe{};
+    probe.lut16 = reinterpret_cast<const uint16_t*>(&probe);  // (only tested for null)
+    probe.n_sub16 = max_sub16;
+    return with_lds_sub16(probe, (size_t)kSyncLdsWords * 4).n_sub16;
 }
 
 void launch_decode_sync_multi(bool t16, uint32_t max_sub16, const DecField* fd, const uint64_t* start, int nf,
