@@ -285,7 +285,7 @@ __device__ __forceinline__ uint32_t ckpt(uint32_t off, uint32_t cnt, uint32_t z)
 }
 
 // Dynamic LDS words of the sync kernel after the tables.
-constexpr uint32_t kSyncLdsWords = kStageLds + kSyncWg * kCkStride + 3 * kSyncWg;
+constexpr uint32_t kSyncLdsWords = kStageLds + kSyncWg * kCkStride + 2 * kSyncWg;
 
 // Workgroup `unit` of the sync pass: subsequences unit * kSyncStride on.
 template <bool T16>
@@ -295,9 +295,8 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
     Decoder<T16> dec;
     uint32_t* st = lds + dec.load(lds, a);
     uint32_t* ck = st + kStageLds;
-    uint32_t* own_cnt = ck + kSyncWg * kCkStride;
-    uint32_t* own_z = own_cnt + kSyncWg;
-    uint32_t* own_e = own_z + kSyncWg;
+    uint32_t* own_cz = ck + kSyncWg * kCkStride;  // own codeword count | zero count << 16 (both <= kSubBits)
+    uint32_t* own_e = own_cz + kSyncWg;
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     const uint64_t J0 = unit * kSyncStride;
@@ -331,8 +330,7 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             cnt++;
             z += sym == 0;
         }
-        own_cnt[t] = cnt;
-        own_z[t] = z;
+        own_cz[t] = cnt | (z << 16);
         own_e[t] = lp;
         if (j == 0) {  // the stream's first subsequence starts at a true boundary
             s.b[0] = 0;
@@ -360,8 +358,8 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             if (q >= eu) {
                 if (q == own_e[u]) {
                     met = true;
-                    before_c = own_cnt[u];
-                    before_z = own_z[u];
+                    before_c = own_cz[u] & 0xFFFF;
+                    before_z = own_cz[u] >> 16;
                 }
                 break;
             }
@@ -390,8 +388,8 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             s.list_j[0][k] = s.jbase + (uint32_t)(j + 1);
             s.list_start[0][k] = base + q0;
         } else if (met) {
-            s.cnt[j + 1] = cc + own_cnt[u] - before_c;
-            s.zer[j + 1] = cz + own_z[u] - before_z;
+            s.cnt[j + 1] = cc + (own_cz[u] & 0xFFFF) - before_c;
+            s.zer[j + 1] = cz + (own_cz[u] >> 16) - before_z;
             s.e[j + 1] = base + own_e[u];
         } else {
             // the continuation decoded all of j + 1 from its true start; the
@@ -1806,10 +1804,12 @@ static size_t table_lds_bytes(const DecodeArgs& a, bool t16) {
 // memory.
 static DecodeArgs with_lds_sub16(DecodeArgs a, size_t own_bytes) {
     if (!a.lut16 || !a.n_sub16) return a;
-    constexpr size_t kCuLds = 160 * 1024;
+    constexpr size_t kCuLds = 160 * 1024, kGranule = 1024;  // (workgroup LDS counted in whole KiB, conservatively)
     DecodeArgs b = a;
     b.n_sub16 = 0;
-    const size_t without = table_lds_bytes(b, true) + own_bytes, with = table_lds_bytes(a, true) + own_bytes;
+    auto granules = [](size_t bytes) { return (bytes + kGranule - 1) / kGranule * kGranule; };
+    const size_t without = granules(table_lds_bytes(b, true) + own_bytes);
+    const size_t with = granules(table_lds_bytes(a, true) + own_bytes);
     if (kCuLds / with < kCuLds / without) a.n_sub16 = 0;
     return a;
 }
