@@ -416,14 +416,42 @@ __global__ void __launch_bounds__(256) k_decode_sync(DecodeArgs a, SubState s, D
 // listed start (the true end of j - 1); this gives its start, counts and
 // end.  An end other than the one j + 1's data was computed from lists j + 1,
 // from that end, for the next pass.  Chains shorten by at least one per
-// pass.  Workgroups take 256 entries at a time; their words (the
-// subsequence plus the tail its last codeword may read) are staged in LDS
-// with coalesced loads, one row per entry, then each thread decodes its row.
-constexpr uint32_t kRowWords = 38;   // 32 words + shift0 + the last codeword and window reads
-constexpr uint32_t kRowStride = 39;  // odd: one bank per thread
-constexpr uint32_t kPassGrid = 512;  // persistent workgroups of a pass
-// Entries e0 .. e0 + 255 (those below m) of pass `pass`'s list: staged rows,
-// one entry per thread.
+// pass.  Workgroups take 256 entries at a time, one per thread; a thread
+// stages its entry's row (the subsequence plus the tail its last codeword
+// may read) in LDS with independent 16-byte loads, all in flight together,
+// then decodes it.  (Staging rows with coalesced word loads, one dependent
+// load of the entry number per word, put a memory latency per word on the
+// round.)
+//
+// A row is aligned-space words 32 j .. 32 j + kRowWords - 1, 16-byte aligned
+// like the stream base: the decode starts at local bit start + shift0 - 1024 j
+// (>= 0, since a subsequence's true start is at or past its first bit), the
+// last codeword starts before local bit 1024 + shift0 (<= 1144), and a
+// decode reads at most 72 + 32 bits past a codeword start (the deepest
+// table walk of an invalid path), so the last word read is word 38.
+constexpr uint32_t kRowWords = 40;   // ten 16-byte loads
+constexpr uint32_t kRowStride = 41;  // odd: the lanes' reads of one word index fall in different banks
+constexpr uint32_t kPassGrid = 1024;  // persistent workgroups of a pass (at most ~3 resident per CU)
+__device__ __forceinline__ void stage_row(uint32_t* __restrict__ row, const StreamView& sv, uint64_t j) {
+    const uint64_t w0 = j * (kSubBits / 32);
+    if (4 * (w0 + kRowWords) <= sv.nbytes) {
+        const uint4* src = reinterpret_cast<const uint4*>(sv.base) + w0 / 4;
+        uint4 v[kRowWords / 4];
+#pragma unroll
+        for (uint32_t m = 0; m < kRowWords / 4; m++) v[m] = src[m];
+#pragma unroll
+        for (uint32_t m = 0; m < kRowWords / 4; m++) {
+            row[4 * m] = __builtin_bswap32(v[m].x);
+            row[4 * m + 1] = __builtin_bswap32(v[m].y);
+            row[4 * m + 2] = __builtin_bswap32(v[m].z);
+            row[4 * m + 3] = __builtin_bswap32(v[m].w);
+        }
+    } else {  // the stream's last rows: bytes past its end read as zero
+        for (uint32_t w = 0; w < kRowWords; w++) row[w] = sv.word(w0 + w);
+    }
+}
+// Entries e0 .. e0 + 255 (those below m) of pass `pass`'s list: one entry
+// per thread, staged in the thread's own LDS row.
 template <bool T16>
 __device__ __forceinline__ void redecode_entries(const DecodeArgs& a, const SubState& s, uint32_t pass,
                                                  const Decoder<T16>& dec, uint32_t* rows, const StreamView& sv,
@@ -434,17 +462,11 @@ __device__ __forceinline__ void redecode_entries(const DecodeArgs& a, const SubS
     uint64_t* os = s.list_start[(pass + 1) & 1];
     uint32_t* on = &s.list_n[(pass + 1) % 3];
     const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
-    lds_sync();  // the tables (first time) / the previous rows' readers
-    for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
-        const uint32_t e = p / kRowWords, w = p % kRowWords;
-        const uint64_t j = lj[e0 + e];
-        rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
-    }
-    lds_sync();
     if (threadIdx.x < ne) {
         const uint64_t j = lj[e0 + threadIdx.x];
         const uint64_t start = ls[e0 + threadIdx.x];
-        const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
+        stage_row(rows + threadIdx.x * kRowStride, sv, j);  // the thread's own row: no barrier
+        const uint64_t rowbit = j * kSubBits;  // aligned-space bit of word 0
         const RowWin win{rows + threadIdx.x * kRowStride};
         const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
         const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
@@ -484,6 +506,7 @@ __global__ void __launch_bounds__(256) k_decode_pass(DecodeArgs a, SubState s, u
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
     bool bad = false;
+    lds_sync();  // the tables
     for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256)
         redecode_entries<T16>(a, s, pass, dec, rows, sv, e0, m, bad);
     if (bad) atomicOr(&sum->error, 1u);
@@ -1669,7 +1692,6 @@ __global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __res
     if (blockIdx.x == 0 && threadIdx.x == 0) S.list_n[(pass + 2) % 3] = 0;  // the list after next
     if ((uint64_t)blockIdx.x * 256 >= m) return;                            // whole workgroup
     __shared__ uint32_t rows[256 * kRowStride];
-    __shared__ int sfield[256];
     const uint32_t* lj = S.list_j[pass & 1];
     const uint64_t* ls = S.list_start[pass & 1];
     uint32_t* oj = S.list_j[(pass + 1) & 1];
@@ -1677,29 +1699,18 @@ __global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __res
     uint32_t* on = &S.list_n[(pass + 1) % 3];
     for (uint64_t e0 = blockIdx.x * 256ull; e0 < m; e0 += (uint64_t)gridDim.x * 256) {
         const uint32_t ne = (uint32_t)min<uint64_t>(256, m - e0);
-        lds_sync();  // the previous rows' readers
-        if (threadIdx.x < ne) sfield[threadIdx.x] = field_of(sub_start, nf, lj[e0 + threadIdx.x]);
-        lds_sync();
-        for (uint32_t p = threadIdx.x; p < ne * kRowWords; p += 256) {
-            const uint32_t e = p / kRowWords, w = p % kRowWords;
-            const int f = sfield[e];
-            const uint64_t j = lj[e0 + e] - sub_start[f];
-            StreamView sv;
-            sv.setup(fd[f].a.stream, fd[f].a.stream_bits);
-            rows[e * kRowStride + w] = sv.word(((j * kSubBits + sv.shift0) >> 5) + w);
-        }
-        lds_sync();
         if (threadIdx.x < ne) {
-            const int f = sfield[threadIdx.x];
+            const uint32_t J = lj[e0 + threadIdx.x];
+            const int f = field_of(sub_start, nf, J);
             const DecodeArgs& a = fd[f].a;
             const SubState& s = fd[f].s;
             const Decoder<T16> dec = global_decoder<T16>(a);
             StreamView sv;
             sv.setup(a.stream, a.stream_bits);
-            const uint32_t J = lj[e0 + threadIdx.x];
             const uint64_t j = J - sub_start[f];
             const uint64_t start = ls[e0 + threadIdx.x];
-            const uint64_t rowbit = ((j * kSubBits + sv.shift0) >> 5) << 5;  // aligned-space bit of word 0
+            stage_row(rows + threadIdx.x * kRowStride, sv, j);  // the thread's own row: no barrier
+            const uint64_t rowbit = j * kSubBits;  // aligned-space bit of word 0
             const RowWin win{rows + threadIdx.x * kRowStride};
             const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
             const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
