@@ -74,6 +74,37 @@ def test_fused_decompress_batch_matches_worker_path_single_calls_and_oracle(gpu_
             assert np.array_equal(fused[k].cpu().numpy().view(np.uint32), y_orc.view(np.uint32)), (case, k)
 
 
+def test_fused_decompress_batch_of_slowly_synchronising_walks(gpu_ctx):
+    """1D random walks with spikes synchronise slowly: their re-decode lists
+    hold chains (a subsequence's corrected end moves its successor's start,
+    and so on) that take several passes over the batch-wide list.  Large
+    fields fused ("always"), twice, against single calls and the oracle."""
+    from vecsz_amd import datagen as dg
+
+    fields = [dg.walk_spikes_1d(n, seed=s) for s, n in ((1, 1 << 22), (2, 3 << 20), (3, (1 << 21) + 77))]
+    fields.append(dg.grf((300, 500), seed=4))
+    ebs, modes = [1e-4, 1e-4, 1e-5, 1e-4], ["r2r", "r2r", "r2r", "abs"]
+    print("fused decompress batch of walks", [f.size for f in fields], flush=True)
+    archives = _archives(gpu_ctx, fields, ebs, modes)
+    ranges = [po.value_range(f) if m == "r2r" else 0.0 for f, m in zip(fields, modes)]
+    params = _params(archives, modes, ranges, 8, 4096)
+    fused = [torch.empty(f.shape, dtype=torch.float32, device="cuda") for f in fields]
+    try:
+        gpu_ctx.set_batch_fused("always")
+        for rep in range(2):
+            for o in fused:
+                o.fill_(-1.0)
+            _run(gpu_ctx, archives, params, fused)
+            for k, a in enumerate(archives):
+                y1 = gpu_ctx.decompress(a, modes[k], ranges[k])
+                assert torch.equal(fused[k].view(torch.int32), y1.view(torch.int32)), (rep, k)
+    finally:
+        gpu_ctx.set_batch_fused(True)
+    for k, a in enumerate(archives):
+        y_orc = po.decompress(a.cpu().numpy().tobytes(), fields[k].shape, modes[k] == "r2r", ranges[k], 4096, 8)
+        assert np.array_equal(fused[k].cpu().numpy().view(np.uint32), y_orc.view(np.uint32)), k
+
+
 def test_fused_decompress_batch_unaligned_outputs(gpu_ctx):
     """Outputs one float past a 16-byte boundary take the element-wise
     reconstruction forms; the values are the same."""
