@@ -1045,7 +1045,14 @@ __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, con
 // following subsequences where the unit ends beyond it (the decode is on the
 // true path there), and skips the symbols before its first unit (written by
 // a predecessor).  Only the field's last unit may be partial.
-constexpr uint32_t kWriteStageWords = kSyncWg * (kSubBits / 32) + 32;  // + up to 7 codewords past the last range
+//
+// A workgroup (256 threads) takes kWriteWg = 240 subsequences, so its LDS
+// (the 8 KiB first-level table and ~31 KiB of staged bits) stays under
+// 40 KiB and four workgroups fit a CU instead of three: the decode is
+// latency-bound, and the resident lanes count for more than the 16 that
+// only help stage.
+constexpr uint32_t kWriteWg = 240;
+constexpr uint32_t kWriteStageWords = kWriteWg * (kSubBits / 32) + 32;  // + up to 7 codewords past the last range
 constexpr uint32_t kWriteStageLds = kWriteStageWords + kWriteStageWords / 32 + 3;
 // sym_off / zero_off: exclusive prefixes of the subsequences' codeword and
 // zero counts, counted from the field's subsequence 0 (entry 0 is the base
@@ -1062,11 +1069,11 @@ __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const Sub
     uint32_t* st = lds + dec.load(lds, a);
     StreamView sv;
     sv.setup(a.stream, a.stream_bits);
-    const uint64_t J0 = unit * kSyncWg;
+    const uint64_t J0 = unit * kWriteWg;
     stage_words<kWriteStageWords>(st, sv, J0 * (kSubBits / 32));
     lds_sync();
     const uint64_t j = J0 + threadIdx.x;
-    if (j >= a.nsub) return;
+    if (threadIdx.x >= kWriteWg || j >= a.nsub) return;
     uint32_t lp = (uint32_t)(s.b[j] - J0 * kSubBits) + sv.shift0;
     const uint64_t O = sym_off[j] - sym_off[0];
     const uint64_t E = min(O + s.cnt[j], a.n);
@@ -1933,7 +1940,7 @@ void launch_decode_write(const DecodeArgs& a0, SubState s, const uint64_t* sym_o
     const bool t16 = a0.lut16 != nullptr;
     const DecodeArgs a = with_lds_sub16(a0, (size_t)kWriteStageLds * 4);
     const size_t lds = table_lds_bytes(a, t16) + (size_t)kWriteStageLds * 4;
-    const dim3 grid(grid_for(a.nsub, kSyncWg));
+    const dim3 grid((unsigned)decode_write_units(a.nsub));
     if (t16)
         hipLaunchKernelGGL(k_decode_write<true>, grid, dim3(kSyncWg), lds, st, a, s, sym_off, zero_off, codes, sparse,
                            sum, guard);
@@ -2140,7 +2147,7 @@ void launch_reconstruct_multi(int kind, int B, const DecField* fd, const uint64_
 #undef RM
 }
 uint64_t decode_sync_units(uint64_t nsub) { return (nsub + kSyncStride - 1) / kSyncStride; }
-uint64_t decode_write_units(uint64_t nsub) { return (nsub + kSyncWg - 1) / kSyncWg; }
+uint64_t decode_write_units(uint64_t nsub) { return (nsub + kWriteWg - 1) / kWriteWg; }
 void launch_gather_heads(const HeadCopy* hc, int n, uint8_t* dst, hipStream_t st) {
     if (n > 0) hipLaunchKernelGGL(k_gather_heads, dim3((unsigned)n), dim3(256), 0, st, hc, dst);
 }
