@@ -196,6 +196,18 @@ struct LdsHist {
     }
 };
 
+// The same when every bin is in LDS (dict <= kMaxLdsHistBins): the add is
+// one atomic with no branch.  The kernels run their tile loop with this
+// form when it applies, so the choice is made once per kernel, not per code.
+struct LdsHistAll {
+    uint32_t* hot;  // kHotBins * 64 lane-private hot bins, then the dict bins
+    int lo;
+    __device__ __forceinline__ void add(int bin) {
+        const int h = bin - lo;
+        atomicAdd(&hot[(unsigned)h < (unsigned)kHotBins ? h * 64 + (int)lane_id() : kHotBins * 64 + bin], 1u);
+    }
+};
+
 __device__ void hist_init(LdsHist& H, uint32_t* lds, int dict, int radius, unsigned long long* global) {
     H.dict = dict;
     H.lo = radius - kHotHalf;
@@ -373,8 +385,14 @@ __global__ void __launch_bounds__(256) k_dualquant_3d(const float* __restrict__ 
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq3_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    if (H.lds_cold) {
+        LdsHistAll L{H.hot, H.lo};
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq3_tile<B, PAD>(in, codes, sparse, L, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    } else {
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq3_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    }
     hist_flush(H, partials);
 }
 
@@ -438,8 +456,14 @@ __global__ void __launch_bounds__(256) k_dualquant_2d(const float* __restrict__ 
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq2_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    if (H.lds_cold) {
+        LdsHistAll L{H.hot, H.lo};
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq2_tile<B, PAD>(in, codes, sparse, L, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    } else {
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq2_tile<B, PAD>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    }
     hist_flush(H, partials);
 }
 
@@ -563,13 +587,15 @@ __device__ __forceinline__ void dq3_x4_tile(const float* __restrict__ in, uint16
                 }
                 if (xin) {
                     store_x4<ALN>(codes + idx, c, nv);
+                    // outliers are rare: one test for the lane's four codes
+                    if ((c[0] == 0) | (c[1] == 0) | (c[2] == 0) | (c[3] == 0)) {
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (ALN || (uint32_t)k < nv) {
-                            if (c[k] == 0) sparse[idx + k] = q[k];
-                            H.add((int)c[k]);
-                        }
+                        for (int k = 0; k < 4; k++)
+                            if ((ALN || (uint32_t)k < nv) && c[k] == 0) sparse[idx + k] = q[k];
                     }
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (ALN || (uint32_t)k < nv) H.add((int)c[k]);
                 }
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
@@ -600,7 +626,7 @@ __device__ __forceinline__ void dq3_x4_at(const float* __restrict__ in, uint16_t
         dq3_x4_tile<B, false, ALN>(in, codes, sparse, H, a, x, ty * B, tz * B);
 }
 
-template <int B, bool ALN>
+template <int B, bool ALN, bool COLD>
 __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                          float* __restrict__ sparse,
                                                          unsigned long long* __restrict__ hist,
@@ -613,8 +639,14 @@ __global__ void __launch_bounds__(256) k_dualquant_3d_x4(const float* __restrict
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y * a.ntiles_z;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq3_x4_at<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    if constexpr (COLD) {  // every bin in LDS (dict <= kMaxLdsHistBins)
+        LdsHistAll L{H.hot, H.lo};
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq3_x4_at<B, ALN>(in, codes, sparse, L, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    } else {
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq3_x4_at<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    }
     hist_flush(H, partials);
 }
 
@@ -655,13 +687,15 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
             }
             if (xin) {
                 store_x4<ALN>(codes + idx, c, nv);
+                // outliers are rare: one test for the lane's four codes
+                if ((c[0] == 0) | (c[1] == 0) | (c[2] == 0) | (c[3] == 0)) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (ALN || (uint32_t)k < nv) {
-                        if (c[k] == 0) sparse[idx + k] = lost ? 0.f : q[k];
-                        H.add((int)c[k]);
-                    }
+                    for (int k = 0; k < 4; k++)
+                        if ((ALN || (uint32_t)k < nv) && c[k] == 0) sparse[idx + k] = lost ? 0.f : q[k];
                 }
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (ALN || (uint32_t)k < nv) H.add((int)c[k]);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) {
@@ -672,7 +706,7 @@ __device__ __forceinline__ void dq2_x4_tile(const float* __restrict__ in, uint16
     }
 }
 
-template <int B, bool ALN>
+template <int B, bool ALN, bool COLD>
 __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict__ in, uint16_t* __restrict__ codes,
                                                          float* __restrict__ sparse,
                                                          unsigned long long* __restrict__ hist,
@@ -685,8 +719,14 @@ __global__ void __launch_bounds__(256) k_dualquant_2d_x4(const float* __restrict
     const uint64_t ntiles = a.ntiles_x * a.ntiles_y;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
-        dq2_x4_tile<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    if constexpr (COLD) {  // every bin in LDS (dict <= kMaxLdsHistBins)
+        LdsHistAll L{H.hot, H.lo};
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq2_x4_tile<B, ALN>(in, codes, sparse, L, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    } else {
+        for (uint64_t tt = wave; tt < ntiles; tt += nwaves)
+            dq2_x4_tile<B, ALN>(in, codes, sparse, H, a, a.reverse ? ntiles - 1 - tt : tt, lane);
+    }
     hist_flush(H, partials);
 }
 
@@ -774,8 +814,14 @@ __global__ void __launch_bounds__(256) k_dualquant_1d(const float* __restrict__ 
     const bool aligned = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
     const uint64_t wave = blockIdx.x * 4ull + (threadIdx.x >> 6);
     const uint64_t nwaves = gridDim.x * 4ull;
-    for (uint64_t wb = wave * 64; wb < ngroups; wb += nwaves * 64)
-        dq1_tile<B, PAD>(in, codes, sparse, H, a, wb, lane, aligned);
+    if (H.lds_cold) {
+        LdsHistAll L{H.hot, H.lo};
+        for (uint64_t wb = wave * 64; wb < ngroups; wb += nwaves * 64)
+            dq1_tile<B, PAD>(in, codes, sparse, L, a, wb, lane, aligned);
+    } else {
+        for (uint64_t wb = wave * 64; wb < ngroups; wb += nwaves * 64)
+            dq1_tile<B, PAD>(in, codes, sparse, H, a, wb, lane, aligned);
+    }
     hist_flush(H, partials);
 }
 
@@ -1984,20 +2030,28 @@ void launch_dualquant(int ndim, int B, const float* in, uint16_t* codes, float* 
     const bool pad = a.halo != nullptr;
     const bool x4 = !pad && dualquant_use_x4(ndim, B, in, a.d0, a.d1, a.d2, false);
     const bool aln = dualquant_x4_aligned(in, a.d0);
+    const bool cold = a.dict <= kMaxLdsHistBins;  // every histogram bin in LDS
     if (x4) a.ntiles_x = (a.d0 + 255) / 256;
 #define DQ_CASE(ND, KERN, BB)                                                                              \
     if (ndim == ND && B == BB) {                                                                           \
         hipLaunchKernelGGL(KERN<BB>, dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
         goto launched;                                                                                     \
     }
-#define DQ_X4(ND, KERN, BB)                                                                                          \
-    if (ndim == ND && B == BB) {                                                                                     \
-        if (aln)                                                                                                     \
-            hipLaunchKernelGGL((KERN<BB, true>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials, a); \
-        else                                                                                                         \
-            hipLaunchKernelGGL((KERN<BB, false>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, partials,  \
-                               a);                                                                                   \
-        goto launched;                                                                                               \
+#define DQ_X4(ND, KERN, BB)                                                                                    \
+    if (ndim == ND && B == BB) {                                                                               \
+        if (aln && cold)                                                                                       \
+            hipLaunchKernelGGL((KERN<BB, true, true>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist,  \
+                               partials, a);                                                                   \
+        else if (aln)                                                                                          \
+            hipLaunchKernelGGL((KERN<BB, true, false>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, \
+                               partials, a);                                                                   \
+        else if (cold)                                                                                         \
+            hipLaunchKernelGGL((KERN<BB, false, true>), dim3(grid), dim3(256), lds, st, in, codes, sparse, hist, \
+                               partials, a);                                                                   \
+        else                                                                                                   \
+            hipLaunchKernelGGL((KERN<BB, false, false>), dim3(grid), dim3(256), lds, st, in, codes, sparse,      \
+                               hist, partials, a);                                                             \
+        goto launched;                                                                                         \
     }
     if (x4) {
         DQ_X4(3, k_dualquant_3d_x4, 8)
