@@ -1590,27 +1590,39 @@ __device__ __forceinline__ void recon_x4_tile(uint64_t t, const uint16_t* __rest
                         base[i] = Y + Zv[i] - YZ;
                         isout[i] = ein[i] && c[i] == 0;
                         v[i] = 0.f;
-                        if (isout[i]) {
-                            v[i] = sparse[idx + i];
-                            if (!exact_outlier(v[i])) bad = true;
-                            g[i] = (int32_t)v[i] - base[i];
-                        } else {
-                            g[i] = (int32_t)c[i] - a.radius;
-                        }
-                        rs[i] = isout[i] || (i == 0 && rowstart);
+                        g[i] = (int32_t)c[i] - a.radius;
                     }
+                    // outliers are rare: one test for the lane's four elements
+                    const bool anyout = isout[0] | isout[1] | isout[2] | isout[3];
+                    if (anyout) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            if (isout[i]) {
+                                v[i] = sparse[idx + i];
+                                if (!exact_outlier(v[i])) bad = true;
+                                g[i] = (int32_t)v[i] - base[i];
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++) rs[i] = isout[i] || (i == 0 && rowstart);
                     row_scan4<B>(D, g, rs);
                     float o[4];
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const int32_t sv = D[i] + base[i];
                         if (sv >= kExact || sv <= -kExact) bad = true;
-                        o[i] = isout[i] ? scale_outlier(v[i], a.twoeb) : scale_out(sv, a.twoeb);
+                        o[i] = scale_out(sv, a.twoeb);
                         if (ND >= 3) {
                             zrow[i] = Zv[i];
                             prev[ZB > 1 ? yy : 0][i] = sv;
                         }
                         srow[i] = sv;
+                    }
+                    if (anyout) {
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            if (isout[i]) o[i] = scale_outlier(v[i], a.twoeb);
                     }
                     if constexpr (ALN) {
                         if (xin) *reinterpret_cast<float4*>(out + idx) = make_float4(o[0], o[1], o[2], o[3]);
