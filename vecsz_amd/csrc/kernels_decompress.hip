@@ -314,7 +314,9 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
         const uint32_t e0 = local(min((j + 1) * (uint64_t)kSubBits, a.stream_bits));
 #pragma unroll
         for (int k = 0; k < kCkpts; k++) ck[t * kCkStride + k] = kNoCkpt;
-        uint32_t lp = s0, cnt = 0, z = 0, last_k = 0xFFFFFFFFu;
+        // minl: the shortest codeword length seen (0: an invalid path), a
+        // per-lane minimum rather than a flag the loop merges across lanes
+        uint32_t lp = s0, cnt = 0, z = 0, last_k = 0xFFFFFFFFu, minl = ~0u;
         while (lp < e0) {
             // the first codeword start in a window is its checkpoint; the
             // other codewords write the row's pad slot (never read), so the
@@ -325,11 +327,12 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             last_k = kk;
             uint32_t len;
             const uint32_t sym = dec.take_at(st, lp, len);
-            bad |= len == 0;
-            lp += len ? len : 1;
+            minl = min(minl, len);
+            lp += max(len, 1u);
             cnt++;
             z += sym == 0;
         }
+        bad |= minl == 0;
         own_cz[t] = cnt | (z << 16);
         own_e[t] = lp;
         if (j == 0) {  // the stream's first subsequence starts at a true boundary
@@ -347,7 +350,7 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
         const uint32_t su = u * kSubBits + sv.shift0;
         const uint32_t eu = local(min((j + 2) * (uint64_t)kSubBits, a.stream_bits));
         const uint32_t q0 = own_e[t];
-        uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0;
+        uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0, minl = ~0u;
         bool met = false, capped = false;
         uint32_t q = q0;
         for (;;) {
@@ -373,11 +376,12 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             }
             uint32_t len;
             const uint32_t sym = dec.take_at(st, q, len);
-            bad |= len == 0;
-            q += len ? len : 1;
+            minl = min(minl, len);
+            q += max(len, 1u);
             cc++;
             cz += sym == 0;
         }
+        bad |= minl == 0;
         const uint64_t base = J0 * kSubBits - sv.shift0;
         s.b[j + 1] = base + q0;
         if (capped) {
@@ -471,15 +475,16 @@ __device__ __forceinline__ void redecode_entries(const DecodeArgs& a, const SubS
         const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
         const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
         uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
-        uint32_t cnt = 0, z = 0;
+        uint32_t cnt = 0, z = 0, minl = ~0u;
         while (lp < tl) {
             uint32_t len;
             const uint32_t sym = dec.take_w(win, lp, len);
-            bad |= len == 0;
-            lp += len ? len : 1;
+            minl = min(minl, len);
+            lp += max(len, 1u);
             cnt++;
             z += sym == 0;
         }
+        bad |= minl == 0;
         const uint64_t pos = rowbit + lp - sv.shift0;
         s.b[j] = start;
         s.cnt[j] = cnt;
@@ -1734,16 +1739,16 @@ __global__ void __launch_bounds__(256) k_decode_pass_multi(const DecField* __res
             const uint64_t target = min((j + 1) * (uint64_t)kSubBits, a.stream_bits);
             const uint32_t tl = (uint32_t)(target + sv.shift0 - rowbit);
             uint32_t lp = (uint32_t)(start + sv.shift0 - rowbit);
-            uint32_t cnt = 0, z = 0;
-            bool bad = false;
+            uint32_t cnt = 0, z = 0, minl = ~0u;
             while (lp < tl) {
                 uint32_t len;
                 const uint32_t sym = dec.take_w(win, lp, len);
-                bad |= len == 0;
-                lp += len ? len : 1;
+                minl = min(minl, len);
+                lp += max(len, 1u);
                 cnt++;
                 z += sym == 0;
             }
+            const bool bad = minl == 0;
             const uint64_t pos = rowbit + lp - sv.shift0;
             s.b[j] = start;
             s.cnt[j] = cnt;
