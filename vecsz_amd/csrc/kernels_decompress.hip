@@ -936,10 +936,11 @@ __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, con
     // the loops count symbols, so an invalid codeword (len 0, an error)
     // needs no forced progress; a damaged stream stays within kIdxSpanOver
     // words past the span
+    uint32_t minl = ~0u;  // shortest codeword length seen (0: an invalid path)
     auto next = [&]() -> uint32_t {
         uint32_t len;
         const uint32_t sym = dec.take_w(win, lp, len);
-        bad |= len == 0;
+        minl = min(minl, len);
         lp += len;
         return sym;
     };
@@ -1038,6 +1039,7 @@ __global__ void __launch_bounds__(256) k_decode_indexed_staged(DecodeArgs a, con
             if (sym == 0) outlier(s0 + i);
         }
     }
+    bad |= minl == 0;
     // the segment ends where the next one starts
     if (!bad) bad = span_lo + lp - sv.shift0 != end;
     if (bad) atomicOr(&sum->error, 1u);
@@ -1086,10 +1088,11 @@ __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const Sub
     bool bad = false;
     // the loops below count symbols, so an invalid codeword (len 0, an
     // error) needs no forced progress
+    uint32_t minl = ~0u;  // shortest codeword length seen (0: an invalid path)
     auto next = [&]() -> uint32_t {
         uint32_t len;
         const uint32_t sym = dec.take_at(st, lp, len);
-        bad |= len == 0;
+        minl = min(minl, len);
         lp += len;
         return sym;
     };
@@ -1150,6 +1153,7 @@ __device__ __forceinline__ void decode_write_unit(const DecodeArgs& a, const Sub
             if (sym == 0) outlier(i);
         }
     }
+    bad |= minl == 0;
     if (bad) atomicOr(&sum->error, 1u);
 }
 
