@@ -350,30 +350,21 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
         const uint32_t su = u * kSubBits + sv.shift0;
         const uint32_t eu = local(min((j + 2) * (uint64_t)kSubBits, a.stream_bits));
         const uint32_t q0 = own_e[t];
-        uint32_t cc = 0, cz = 0, before_c = 0, before_z = 0, minl = ~0u;
-        bool met = false, capped = false;
+        uint32_t cc = 0, cz = 0, minl = ~0u, e = 0;
         uint32_t q = q0;
+        bool hit = false;
+        // One exit, taken when the budget is spent, past u's end, or at a
+        // codeword start that meets u's checkpoint; which one is worked out
+        // after the loop (exits that each set their own results cost the
+        // divergent loop a set of lane masks per exit and iteration).  Past
+        // the budget or u's end the read goes to the row's pad slot and is
+        // not used.
         for (;;) {
-            if (q - q0 >= kContBits) {
-                capped = true;
-                break;
-            }
-            if (q >= eu) {
-                if (q == own_e[u]) {
-                    met = true;
-                    before_c = own_cz[u] & 0xFFFF;
-                    before_z = own_cz[u] >> 16;
-                }
-                break;
-            }
             const uint32_t rel = q - su;
-            const uint32_t e = ck[u * kCkStride + rel / kCkptBits];
-            if ((e & 0xFF) == rel % kCkptBits) {
-                met = true;
-                before_c = (e >> 8) & 0x7FF;
-                before_z = e >> 19;
-                break;
-            }
+            e = ck[u * kCkStride + min(rel / kCkptBits, (uint32_t)kCkpts)];
+            const bool stop = (q - q0 >= kContBits) | (q >= eu);
+            hit = !stop && (e & 0xFF) == rel % kCkptBits;
+            if (stop | hit) break;
             uint32_t len;
             const uint32_t sym = dec.take_at(st, q, len);
             minl = min(minl, len);
@@ -382,6 +373,17 @@ __device__ __forceinline__ void decode_sync_unit(const DecodeArgs& a, const SubS
             cz += sym == 0;
         }
         bad |= minl == 0;
+        const bool capped = !hit && q - q0 >= kContBits;
+        bool met = hit;
+        uint32_t before_c = 0, before_z = 0;
+        if (hit) {
+            before_c = (e >> 8) & 0x7FF;
+            before_z = e >> 19;
+        } else if (!capped && q == own_e[u]) {  // past u's end, at u's own end
+            met = true;
+            before_c = own_cz[u] & 0xFFFF;
+            before_z = own_cz[u] >> 16;
+        }
         const uint64_t base = J0 * kSubBits - sv.shift0;
         s.b[j + 1] = base + q0;
         if (capped) {
